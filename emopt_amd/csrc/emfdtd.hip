@@ -1,0 +1,1214 @@
+// libemfdtd: C ABI over the gfx950 CW-FDTD kernels (declarations and the
+// reference function each entry point replaces: include/emfdtd.h).
+//
+// One emf_solver owns every buffer of one solve on one GPU: the six real
+// time-domain fields, the raw complex materials and the derived update
+// coefficients, the CPML psi slabs and tables, the source blocks, the
+// convergence state and two complex result slots (forward, adjoint).  All
+// work is queued on the handle's own HIP stream (default priority).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/emfdtd.h"
+#include "kernels.hpp"
+
+using namespace emf;
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+#define EMF_FAIL(code, ...)   \
+    do {                      \
+        set_err(__VA_ARGS__); \
+        return code;          \
+    } while (0)
+
+#define HIPCHK(x)                                                                \
+    do {                                                                         \
+        hipError_t e_ = (x);                                                     \
+        if (e_ != hipSuccess)                                                    \
+            EMF_FAIL(EMF_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHECK(x)                  \
+    do {                          \
+        int r_ = (x);             \
+        if (r_ != EMF_OK) return r_; \
+    } while (0)
+
+template <class T>
+int dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) return EMF_OK;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        EMF_FAIL(EMF_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
+                 hipGetErrorString(e));
+    }
+    return EMF_OK;
+}
+
+template <class T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+struct SourceDev {
+    int i0, j0, k0, I, J, K;
+    double2 *f[6];   // device (amplitude, phase) per component Jx..Mz
+};
+
+// Closed forms shared with the exported scalar helpers.
+double phase_3t(double t0, double t1, double t2, double f0, double f1, double f2)
+{
+    double f10 = f1 - f0, f21 = f2 - f1;
+    if (f10 == 0 && f21 == 0) return 0.0;
+    return atan2(f10 * (sin(t2) - sin(t1)) - f21 * (sin(t1) - sin(t0)),
+                 f21 * (cos(t1) - cos(t0)) - f10 * (cos(t2) - cos(t1)));
+}
+
+double amplitude_3t(double t0, double t1, double t2, double f0, double f1, double f2, double ph)
+{
+    double f21 = f2 - f1, f10 = f1 - f0;
+    if (f21 == 0 && f10 == 0) return 0.0;
+    if (f21 * f21 >= f10 * f10)
+        return f21 / (cos(ph) * (sin(t2) - sin(t1)) + sin(ph) * (cos(t2) - cos(t1)));
+    return f10 / (cos(ph) * (sin(t1) - sin(t0)) + sin(ph) * (cos(t1) - cos(t0)));
+}
+
+double phase_2t(double t0, double t1, double f0, double f1)
+{
+    if (f0 == 0.0 && f1 == 0) return 0.0;
+    return atan((f1 * sin(t0) - f0 * sin(t1)) / (f0 * cos(t1) - f1 * cos(t0)));
+}
+
+double amplitude_2t(double t0, double t1, double f0, double f1, double ph)
+{
+    if (f0 * f0 > f1 * f1) return f1 / (sin(t1) * cos(ph) + cos(t1) * sin(ph));
+    return f0 / (sin(t0) * cos(ph) + cos(t0) * sin(ph));
+}
+
+// kappa, b, c of one CPML layer at ramp factor f (src/fdtd.cpp:992-997)
+void pml_layer(double sigma0, double alpha0, double kappa0, double dt, double f, double *kap,
+               double *b, double *c)
+{
+    double sig = sigma0 * f;
+    double alp = alpha0 * (1 - f);
+    double k = (kappa0 - 1.0) * f + 1.0;
+    double bb = exp(-dt * (sig / k + alp));
+    *kap = k;
+    *b = bb;
+    *c = (bb == 1) ? 0.0 : (bb - 1) * sig / (sig * k + k * k * alp);
+}
+
+}  // namespace
+
+struct emf_solver {
+    std::mutex mtx;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+
+    // geometry / time
+    int Nx = 0, Ny = 0, Nz = 0;
+    long long n = 0;
+    double X = 0, Y = 0, Z = 0, dx = 0, dy = 0, dz = 0;
+    double wl = 0, R = 0, dt = 0, odt = 0;
+    double src_T = 0, src_min = 1e-4, src_k = 0;
+    int ceps = 0;
+    int w[6] = { 0, 0, 0, 0, 0, 0 };
+    double sigma = 3.0, alpha = 0.0, kappa = 1.0, pw = 3.0;
+    int bc[3] = { '0', '0', '0' };
+    int nconv = 10;
+    int zc = 8;
+
+    // device state
+    double *E[3] = {}, *H[3] = {};
+    double2 *eps[3] = {};
+    double2 *mu[3] = {};           // only when mu is not uniform
+    bool mat_set = false;
+    bool mu_uniform = true;
+    double mu_re = 1.0;            // uniform mu.re
+    double *bE[3] = {}, *aE[3] = {};
+    double *mH[3] = {}, *muR[3] = {};
+    bool coef_dirty = true;
+
+    double *tab = nullptr;         // CPML tables: per axis kH bH cH kE bE cE
+    bool tab_dirty = true;
+    double *psi[2][6][2] = {};     // [pass H/E][face][component]
+    long long psi_n[6] = {};
+    bool pml_built = false;
+
+    double2 *res[2][6] = {};       // result slots: fwd, adj
+
+    std::vector<SourceDev> srcs;
+
+    ConvState *conv = nullptr;
+    long long *samples = nullptr;
+    int nsamples = 0;
+    double *hist = nullptr;
+    int hist_cap = 0;
+    int *done_host = nullptr;      // pinned ring of convergence flags
+    int nhist = 0;
+
+    double2 *scratch = nullptr;
+    size_t scratch_n = 0;
+
+    Geo geo() const
+    {
+        Geo g;
+        g.Nx = Nx; g.Ny = Ny; g.Nz = Nz;
+        g.nxy = (long long)Nx * Ny;
+        g.rx = R / dx; g.ry = R / dy; g.rz = R / dz;
+        g.dt = dt;
+        for (int q = 0; q < 6; ++q) g.w[q] = w[q];
+        for (int q = 0; q < 3; ++q) g.bc[q] = bc[q];
+        g.zc = zc;
+        return g;
+    }
+};
+
+namespace {
+
+int ensure_fields(emf_solver *s)
+{
+    if (s->Nx <= 0 || s->Ny <= 0 || s->Nz <= 0) EMF_FAIL(EMF_ERR_STATE, "grid dims not set");
+    if (s->E[0]) return EMF_OK;
+    for (int c = 0; c < 3; ++c) {
+        CHECK(dalloc(&s->E[c], s->n));
+        CHECK(dalloc(&s->H[c], s->n));
+        CHECK(dalloc(&s->eps[c], s->n));
+        CHECK(dalloc(&s->bE[c], s->n));
+        HIPCHK(hipMemsetAsync(s->E[c], 0, s->n * sizeof(double), s->stream));
+        HIPCHK(hipMemsetAsync(s->H[c], 0, s->n * sizeof(double), s->stream));
+    }
+    return EMF_OK;
+}
+
+void free_fields(emf_solver *s)
+{
+    for (int c = 0; c < 3; ++c) {
+        dfree(s->E[c]); dfree(s->H[c]); dfree(s->eps[c]); dfree(s->mu[c]);
+        dfree(s->bE[c]); dfree(s->aE[c]); dfree(s->mH[c]); dfree(s->muR[c]);
+    }
+    for (int sl = 0; sl < 2; ++sl)
+        for (int c = 0; c < 6; ++c) dfree(s->res[sl][c]);
+    s->mat_set = false;
+    s->coef_dirty = true;
+}
+
+int ensure_slot(emf_solver *s, int slot)
+{
+    if (slot < 0 || slot > 1) EMF_FAIL(EMF_ERR_ARG, "result slot must be 0 or 1, got %d", slot);
+    if (s->res[slot][0]) return EMF_OK;
+    for (int c = 0; c < 6; ++c) {
+        CHECK(dalloc(&s->res[slot][c], s->n));
+        HIPCHK(hipMemsetAsync(s->res[slot][c], 0, s->n * sizeof(double2), s->stream));
+    }
+    return EMF_OK;
+}
+
+int ensure_scratch(emf_solver *s, size_t n)
+{
+    if (s->scratch_n >= n) return EMF_OK;
+    dfree(s->scratch);
+    s->scratch_n = 0;
+    CHECK(dalloc(&s->scratch, n));
+    s->scratch_n = n;
+    return EMF_OK;
+}
+
+unsigned grid1d(long long n, int block)
+{
+    long long b = (n + block - 1) / block;
+    return (unsigned)std::min<long long>(std::max<long long>(b, 1), 65535LL * 16);
+}
+
+int refresh_coeffs(emf_solver *s)
+{
+    if (!s->mat_set) EMF_FAIL(EMF_ERR_STATE, "materials not set (emf_set_mat_arrays)");
+    if (!s->coef_dirty) return EMF_OK;
+    if (s->dt <= 0) EMF_FAIL(EMF_ERR_STATE, "dt not set");
+    for (int c = 0; c < 3; ++c) {
+        if (s->ceps && !s->aE[c]) CHECK(dalloc(&s->aE[c], s->n));
+        hipLaunchKernelGGL(k_eps_coeffs, dim3(grid1d(s->n, 256)), dim3(256), 0, s->stream,
+                           s->eps[c], s->bE[c], s->aE[c], s->n, s->dt, s->odt, s->ceps);
+        if (!s->mu_uniform) {
+            if (!s->mH[c]) CHECK(dalloc(&s->mH[c], s->n));
+            if (!s->muR[c]) CHECK(dalloc(&s->muR[c], s->n));
+            hipLaunchKernelGGL(k_mu_coeffs, dim3(grid1d(s->n, 256)), dim3(256), 0, s->stream,
+                               s->mu[c], s->mH[c], s->muR[c], s->n, s->dt);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    s->coef_dirty = false;
+    return EMF_OK;
+}
+
+// CPML tables of compute_pml_params (src/fdtd.cpp:950-1181) into one device
+// buffer: for axis a, six arrays (kH bH cH kE bE cE) of w_min + w_max entries.
+int refresh_tables(emf_solver *s, PassTab *th, PassTab *te)
+{
+    size_t off[3], len[3], total = 0;
+    for (int a = 0; a < 3; ++a) {
+        len[a] = std::max(1, s->w[2 * a] + s->w[2 * a + 1]);
+        off[a] = total;
+        total += 6 * len[a];
+    }
+    if (s->tab_dirty) {
+        std::vector<double> h(total, 0.0);
+        for (int a = 0; a < 3; ++a) {
+            double *kH = &h[off[a]], *bH = kH + len[a], *cH = bH + len[a];
+            double *kE = cH + len[a], *bE = kE + len[a], *cE = bE + len[a];
+            for (int side = 0; side < 2; ++side) {
+                int wd = s->w[2 * a + side];
+                for (int l = 0; l < wd; ++l) {
+                    double dH, dE;
+                    bool clampH = false, clampE = false;
+                    if (a < 2) {   // x, y faces
+                        dH = side == 0 ? double(l - 0.5) / wd : double(l + 0.5) / wd;
+                        dE = double(l) / wd;
+                        clampH = side == 0;
+                    } else {       // z faces
+                        dH = double(l) / wd;
+                        dE = side == 0 ? double(l + 0.5) / wd : double(l - 0.5) / wd;
+                        clampE = side == 1;
+                    }
+                    double fH = std::pow(dH, s->pw), fE = std::pow(dE, s->pw);
+                    if (clampH && fH < 0) fH = 0;
+                    if (clampE && fE < 0) fE = 0;
+                    int q = side == 0 ? l : s->w[2 * a] + l;
+                    pml_layer(s->sigma, s->alpha, s->kappa, s->dt, fH, &kH[q], &bH[q], &cH[q]);
+                    pml_layer(s->sigma, s->alpha, s->kappa, s->dt, fE, &kE[q], &bE[q], &cE[q]);
+                }
+            }
+        }
+        dfree(s->tab);
+        CHECK(dalloc(&s->tab, total));
+        HIPCHK(hipMemcpyAsync(s->tab, h.data(), total * sizeof(double), hipMemcpyHostToDevice,
+                              s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));   // h is a temporary
+        s->tab_dirty = false;
+    }
+    for (int a = 0; a < 3; ++a) {
+        double *base = s->tab + off[a];
+        th->kap[a] = base; th->b[a] = base + len[a]; th->c[a] = base + 2 * len[a];
+        te->kap[a] = base + 3 * len[a]; te->b[a] = base + 4 * len[a]; te->c[a] = base + 5 * len[a];
+    }
+    return EMF_OK;
+}
+
+void free_psi(emf_solver *s)
+{
+    for (int ps = 0; ps < 2; ++ps)
+        for (int f = 0; f < 6; ++f)
+            for (int c = 0; c < 2; ++c) dfree(s->psi[ps][f][c]);
+    s->pml_built = false;
+}
+
+int build_psi(emf_solver *s)
+{
+    CHECK(ensure_fields(s));
+    free_psi(s);
+    long long nx = s->Nx, ny = s->Ny, nz = s->Nz;
+    for (int f = 0; f < 6; ++f) {
+        int wd = s->w[f];
+        long long cells = f < 2 ? nz * ny * wd : (f < 4 ? nz * nx * wd : ny * nx * wd);
+        s->psi_n[f] = cells;
+        for (int ps = 0; ps < 2; ++ps)
+            for (int c = 0; c < 2; ++c) {
+                CHECK(dalloc(&s->psi[ps][f][c], (size_t)cells));
+                if (cells)
+                    HIPCHK(hipMemsetAsync(s->psi[ps][f][c], 0, cells * sizeof(double), s->stream));
+            }
+    }
+    s->pml_built = true;
+    s->tab_dirty = true;
+    return EMF_OK;
+}
+
+int reset_psi(emf_solver *s)
+{
+    for (int ps = 0; ps < 2; ++ps)
+        for (int f = 0; f < 6; ++f)
+            for (int c = 0; c < 2; ++c)
+                if (s->psi[ps][f][c])
+                    HIPCHK(hipMemsetAsync(s->psi[ps][f][c], 0, s->psi_n[f] * sizeof(double),
+                                          s->stream));
+    return EMF_OK;
+}
+
+int check_ready(emf_solver *s)
+{
+    CHECK(ensure_fields(s));
+    if (!s->pml_built) EMF_FAIL(EMF_ERR_STATE, "CPML not built (emf_build_pml)");
+    for (int f = 0; f < 6; ++f) {
+        int lim = f < 2 ? s->Nx : (f < 4 ? s->Ny : s->Nz);
+        if (s->w[f] < 0 || s->w[f] > lim)
+            EMF_FAIL(EMF_ERR_ARG, "pml width %d out of range for face %d", s->w[f], f);
+    }
+    if (s->w[0] + s->w[1] > s->Nx || s->w[2] + s->w[3] > s->Ny || s->w[4] + s->w[5] > s->Nz)
+        EMF_FAIL(EMF_ERR_ARG, "opposite pml slabs overlap");
+    if (s->R <= 0 || s->dx <= 0 || s->dy <= 0 || s->dz <= 0)
+        EMF_FAIL(EMF_ERR_STATE, "wavelength / physical dims not set");
+    CHECK(refresh_coeffs(s));
+    return EMF_OK;
+}
+
+double env_at(const emf_solver *s, double t)
+{
+    // ramp envelope of src_func_t (src/fdtd.cpp:1595-1602)
+    if (t <= s->src_T)
+        return (1 + s->src_min) * exp(-(t - s->src_T) * (t - s->src_T) / s->src_k) - s->src_min;
+    return 1.0;
+}
+
+PassSrc inline_sources(const emf_solver *s, int first_comp, double t)
+{
+    PassSrc ps;
+    memset(&ps, 0, sizeof(ps));
+    ps.t = t;
+    ps.env = env_at(s, t);
+    ps.n = (int)std::min<size_t>(s->srcs.size(), MAX_INLINE_SRC);
+    for (int q = 0; q < ps.n; ++q) {
+        const SourceDev &d = s->srcs[q];
+        SrcBlk &b = ps.b[q];
+        b.i0 = d.i0; b.j0 = d.j0; b.k0 = d.k0; b.I = d.I; b.J = d.J; b.K = d.K;
+        for (int c = 0; c < 3; ++c) b.f[c] = d.f[first_comp + c];
+    }
+    return ps;
+}
+
+dim3 step_grid(const emf_solver *s)
+{
+    return dim3((s->Nx + TX - 1) / TX, (s->Ny + TY - 1) / TY, (s->Nz + s->zc - 1) / s->zc);
+}
+
+int launch_h(emf_solver *s, double t, const int *done)
+{
+    PassTab th, te;
+    CHECK(refresh_tables(s, &th, &te));
+    PassPsi pp;
+    for (int f = 0; f < 6; ++f)
+        for (int c = 0; c < 2; ++c) pp.p[f][c] = s->psi[0][f][c];
+    PassSrc ps = inline_sources(s, 3, t);
+    Geo g = s->geo();
+    double m_s = s->dt / s->mu_re;
+    if (s->mu_uniform)
+        hipLaunchKernelGGL(k_update_h<false>, step_grid(s), dim3(TX, TY), 0, s->stream,
+                           s->E[0], s->E[1], s->E[2], s->H[0], s->H[1], s->H[2], nullptr, nullptr,
+                           nullptr, nullptr, nullptr, nullptr, m_s, s->mu_re, g, th, pp, ps, done);
+    else
+        hipLaunchKernelGGL(k_update_h<true>, step_grid(s), dim3(TX, TY), 0, s->stream,
+                           s->E[0], s->E[1], s->E[2], s->H[0], s->H[1], s->H[2], s->mH[0],
+                           s->mH[1], s->mH[2], s->muR[0], s->muR[1], s->muR[2], m_s, s->mu_re, g,
+                           th, pp, ps, done);
+    for (size_t q = MAX_INLINE_SRC; q < s->srcs.size(); ++q) {
+        const SourceDev &d = s->srcs[q];
+        SrcBlk b = { d.i0, d.j0, d.k0, d.I, d.J, d.K, { d.f[3], d.f[4], d.f[5] } };
+        long long cnt = (long long)d.I * d.J * d.K;
+        hipLaunchKernelGGL(k_src_h, dim3(grid1d(cnt, 256)), dim3(256), 0, s->stream, s->H[0],
+                           s->H[1], s->H[2], s->mu_uniform ? nullptr : s->muR[0],
+                           s->mu_uniform ? nullptr : s->muR[1], s->mu_uniform ? nullptr : s->muR[2],
+                           s->mu_re, g, b, t, ps.env, done);
+    }
+    HIPCHK(hipGetLastError());
+    return EMF_OK;
+}
+
+int launch_e(emf_solver *s, double t, const int *done)
+{
+    PassTab th, te;
+    CHECK(refresh_tables(s, &th, &te));
+    PassPsi pp;
+    for (int f = 0; f < 6; ++f)
+        for (int c = 0; c < 2; ++c) pp.p[f][c] = s->psi[1][f][c];
+    PassSrc ps = inline_sources(s, 0, t);
+    Geo g = s->geo();
+    if (s->ceps)
+        hipLaunchKernelGGL(k_update_e<true>, step_grid(s), dim3(TX, TY), 0, s->stream, s->E[0],
+                           s->E[1], s->E[2], s->H[0], s->H[1], s->H[2], s->bE[0], s->bE[1],
+                           s->bE[2], s->aE[0], s->aE[1], s->aE[2], g, te, pp, ps, done);
+    else
+        hipLaunchKernelGGL(k_update_e<false>, step_grid(s), dim3(TX, TY), 0, s->stream, s->E[0],
+                           s->E[1], s->E[2], s->H[0], s->H[1], s->H[2], s->bE[0], s->bE[1],
+                           s->bE[2], nullptr, nullptr, nullptr, g, te, pp, ps, done);
+    for (size_t q = MAX_INLINE_SRC; q < s->srcs.size(); ++q) {
+        const SourceDev &d = s->srcs[q];
+        SrcBlk b = { d.i0, d.j0, d.k0, d.I, d.J, d.K, { d.f[0], d.f[1], d.f[2] } };
+        long long cnt = (long long)d.I * d.J * d.K;
+        hipLaunchKernelGGL(k_src_e, dim3(grid1d(cnt, 256)), dim3(256), 0, s->stream, s->E[0],
+                           s->E[1], s->E[2], s->bE[0], s->bE[1], s->bE[2], g, b, t, ps.env, done);
+    }
+    HIPCHK(hipGetLastError());
+    return EMF_OK;
+}
+
+int step(emf_solver *s, long long nidx, const int *done)
+{
+    CHECK(launch_h(s, (double)nidx * s->dt, done));
+    CHECK(launch_e(s, ((double)nidx + 0.5) * s->dt, done));
+    return EMF_OK;
+}
+
+int capture(emf_solver *s, int slot, int which)
+{
+    CHECK(ensure_fields(s));
+    CHECK(ensure_slot(s, slot));
+    double2 **r = s->res[slot];
+    hipLaunchKernelGGL(k_capture, dim3(grid1d(s->n, 256)), dim3(256), 0, s->stream, s->E[0],
+                       s->E[1], s->E[2], s->H[0], s->H[1], s->H[2], r[0], r[1], r[2], r[3], r[4],
+                       r[5], s->n, which);
+    HIPCHK(hipGetLastError());
+    return EMF_OK;
+}
+
+int phasor3(emf_solver *s, int slot, double t0, double t1, double t2)
+{
+    CHECK(ensure_fields(s));
+    CHECK(ensure_slot(s, slot));
+    PhTimes te, thh;
+    double h0 = t0 - 0.5 * s->dt, h1 = t1 - 0.5 * s->dt, h2 = t2 - 0.5 * s->dt;
+    te.s21 = sin(t2) - sin(t1); te.s10 = sin(t1) - sin(t0);
+    te.c10 = cos(t1) - cos(t0); te.c21 = cos(t2) - cos(t1);
+    thh.s21 = sin(h2) - sin(h1); thh.s10 = sin(h1) - sin(h0);
+    thh.c10 = cos(h1) - cos(h0); thh.c21 = cos(h2) - cos(h1);
+    double2 **r = s->res[slot];
+    hipLaunchKernelGGL(k_phasor3, dim3(grid1d(s->n, 256)), dim3(256), 0, s->stream, s->E[0],
+                       s->E[1], s->E[2], s->H[0], s->H[1], s->H[2], r[0], r[1], r[2], r[3], r[4],
+                       r[5], s->n, te, thh);
+    HIPCHK(hipGetLastError());
+    return EMF_OK;
+}
+
+// Convergence sample points: arange(0, I*J*K, int(I*J*K/nconv)) over the
+// padded array (emopt/fdtd.py:411-413).  On one rank a ghost-face sample reads
+// the opposite face's outermost layer (the periodic self-exchange of
+// emopt/fdtd.py:1733-1873) and edge/corner ghosts stay 0.
+int build_samples(emf_solver *s)
+{
+    long long I = s->Nz, J = s->Ny, K = s->Nx, IJK = I * J * K;
+    long long stepv = IJK / std::max(1, s->nconv);
+    if (stepv <= 0) EMF_FAIL(EMF_ERR_ARG, "nconv larger than the grid");
+    std::vector<long long> cells;
+    for (long long p = 0; p < IJK; p += stepv) {
+        long long pk = p % (K + 2), pj = (p / (K + 2)) % (J + 2), pi = p / ((K + 2) * (J + 2));
+        int ghosts = 0;
+        auto wrap = [&](long long v, long long N) -> long long {
+            if (v == 0) { ++ghosts; return N - 1; }
+            if (v == N + 1) { ++ghosts; return 0; }
+            return v - 1;
+        };
+        long long k = wrap(pk, K), j = wrap(pj, J), i = wrap(pi, I);
+        cells.push_back(ghosts > 1 ? -1 : (i * J + j) * K + k);
+    }
+    if ((int)cells.size() > NCONV_MAX)
+        EMF_FAIL(EMF_ERR_ARG, "%zu convergence points exceed the limit %d", cells.size(), NCONV_MAX);
+    dfree(s->samples);
+    CHECK(dalloc(&s->samples, cells.size()));
+    HIPCHK(hipMemcpyAsync(s->samples, cells.data(), cells.size() * sizeof(long long),
+                          hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));   // cells is a temporary
+    s->nsamples = (int)cells.size();
+    return EMF_OK;
+}
+
+}  // namespace
+
+// =========================================================================
+// C ABI
+// =========================================================================
+
+#define LOCK(s)                                                     \
+    if (!(s)) EMF_FAIL(EMF_ERR_ARG, "null solver handle");         \
+    std::lock_guard<std::mutex> lk_((s)->mtx);                      \
+    if (hipSetDevice((s)->dev) != hipSuccess)                       \
+        EMF_FAIL(EMF_ERR_HIP, "hipSetDevice(%d) failed", (s)->dev);
+
+extern "C" {
+
+const char *emf_last_error(void) { return g_err.c_str(); }
+
+const char *emf_version(void)
+{
+    return "emfdtd 0.1 gfx950 fp64 (hipcc -O3, packed-fp32-ops disabled)";
+}
+
+int emf_device_count(void)
+{
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    return n;
+}
+
+int emf_new(int device, emf_solver **out)
+{
+    if (!out) EMF_FAIL(EMF_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int nd = 0;
+    HIPCHK(hipGetDeviceCount(&nd));
+    if (device < 0 || device >= nd) EMF_FAIL(EMF_ERR_ARG, "device %d not present (%d visible)", device, nd);
+    HIPCHK(hipSetDevice(device));
+    emf_solver *s = new emf_solver();
+    s->dev = device;
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        EMF_FAIL(EMF_ERR_HIP, "hipStreamCreate failed");
+    }
+    if (hipHostMalloc((void **)&s->done_host, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        (void)hipStreamDestroy(s->stream);
+        delete s;
+        EMF_FAIL(EMF_ERR_HIP, "hipHostMalloc failed");
+    }
+    if (dalloc(&s->conv, 1) != EMF_OK) {
+        (void)hipHostFree(s->done_host);
+        (void)hipStreamDestroy(s->stream);
+        delete s;
+        return EMF_ERR_NOMEM;
+    }
+    *out = s;
+    return EMF_OK;
+}
+
+int emf_delete(emf_solver *s)
+{
+    if (!s) return EMF_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mtx);
+        (void)hipSetDevice(s->dev);
+        (void)hipStreamSynchronize(s->stream);
+        free_fields(s);
+        free_psi(s);
+        dfree(s->tab);
+        for (auto &d : s->srcs)
+            for (int c = 0; c < 6; ++c) dfree(d.f[c]);
+        s->srcs.clear();
+        dfree(s->conv);
+        dfree(s->samples);
+        dfree(s->hist);
+        dfree(s->scratch);
+        if (s->done_host) (void)hipHostFree(s->done_host);
+        (void)hipStreamDestroy(s->stream);
+    }
+    delete s;
+    return EMF_OK;
+}
+
+int emf_set_wavelength(emf_solver *s, double wl)
+{
+    LOCK(s);
+    if (!(wl > 0)) EMF_FAIL(EMF_ERR_ARG, "wavelength must be > 0");
+    s->wl = wl;
+    s->R = wl / (2 * M_PI);
+    return EMF_OK;
+}
+
+int emf_set_physical_dims(emf_solver *s, double X, double Y, double Z, double dx, double dy,
+                          double dz)
+{
+    LOCK(s);
+    if (!(dx > 0 && dy > 0 && dz > 0)) EMF_FAIL(EMF_ERR_ARG, "grid spacings must be > 0");
+    s->X = X; s->Y = Y; s->Z = Z; s->dx = dx; s->dy = dy; s->dz = dz;
+    return EMF_OK;
+}
+
+int emf_set_grid_dims(emf_solver *s, int Nx, int Ny, int Nz)
+{
+    LOCK(s);
+    if (Nx <= 0 || Ny <= 0 || Nz <= 0) EMF_FAIL(EMF_ERR_ARG, "grid dims must be positive");
+    if ((long long)Nx * Ny * Nz > (1LL << 40)) EMF_FAIL(EMF_ERR_ARG, "grid too large");
+    if (Nx != s->Nx || Ny != s->Ny || Nz != s->Nz) {
+        (void)hipStreamSynchronize(s->stream);
+        free_fields(s);
+        free_psi(s);
+        s->Nx = Nx; s->Ny = Ny; s->Nz = Nz;
+        s->n = (long long)Nx * Ny * Nz;
+    }
+    return ensure_fields(s);
+}
+
+int emf_set_local_grid(emf_solver *s, int k0, int j0, int i0, int K, int J, int I)
+{
+    LOCK(s);
+    if (k0 != 0 || j0 != 0 || i0 != 0 || K != s->Nx || J != s->Ny || I != s->Nz)
+        EMF_FAIL(EMF_ERR_ARG, "one GPU owns the whole grid: local grid must be (0,0,0,Nx,Ny,Nz)");
+    return EMF_OK;
+}
+
+int emf_set_dt(emf_solver *s, double dt)
+{
+    LOCK(s);
+    if (!(dt > 0)) EMF_FAIL(EMF_ERR_ARG, "dt must be > 0");
+    s->dt = dt;
+    s->odt = 1.0 / dt;
+    s->coef_dirty = true;
+    s->tab_dirty = true;
+    return EMF_OK;
+}
+
+int emf_set_complex_eps(emf_solver *s, int c)
+{
+    LOCK(s);
+    s->ceps = c ? 1 : 0;
+    s->coef_dirty = true;
+    return EMF_OK;
+}
+
+int emf_set_mat_arrays(emf_solver *s, const double *ex, const double *ey, const double *ez,
+                       const double *mx, const double *my, const double *mz)
+{
+    LOCK(s);
+    if (!ex || !ey || !ez || !mx || !my || !mz) EMF_FAIL(EMF_ERR_ARG, "null material array");
+    CHECK(ensure_fields(s));
+    const double *e[3] = { ex, ey, ez }, *m[3] = { mx, my, mz };
+    for (int c = 0; c < 3; ++c)
+        HIPCHK(hipMemcpyAsync(s->eps[c], e[c], s->n * sizeof(double2), hipMemcpyHostToDevice,
+                              s->stream));
+    // mu enters only through mu.re (src/fdtd.cpp:186-188); a uniform value
+    // becomes a scalar and no per-cell mu traffic is generated.
+    double v = m[0][0];
+    bool uni = true;
+    for (int c = 0; c < 3 && uni; ++c)
+        for (long long g = 0; g < s->n; ++g)
+            if (m[c][2 * g] != v) { uni = false; break; }
+    s->mu_uniform = uni;
+    s->mu_re = v;
+    if (!uni)
+        for (int c = 0; c < 3; ++c) {
+            if (!s->mu[c]) CHECK(dalloc(&s->mu[c], s->n));
+            HIPCHK(hipMemcpyAsync(s->mu[c], m[c], s->n * sizeof(double2), hipMemcpyHostToDevice,
+                                  s->stream));
+        }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->mat_set = true;
+    s->coef_dirty = true;
+    return EMF_OK;
+}
+
+int emf_update_eps_box(emf_solver *s, int c, const double *vals, int k1, int k2, int j1, int j2,
+                       int i1, int i2)
+{
+    LOCK(s);
+    if (c < 0 || c > 2 || !vals) EMF_FAIL(EMF_ERR_ARG, "bad component or null array");
+    if (!s->mat_set) EMF_FAIL(EMF_ERR_STATE, "materials not set");
+    if (k1 < 0 || j1 < 0 || i1 < 0 || k2 > s->Nx || j2 > s->Ny || i2 > s->Nz || k1 >= k2 ||
+        j1 >= j2 || i1 >= i2)
+        EMF_FAIL(EMF_ERR_ARG, "box out of range");
+    long long cnt = (long long)(k2 - k1) * (j2 - j1) * (i2 - i1);
+    CHECK(ensure_scratch(s, cnt));
+    HIPCHK(hipMemcpyAsync(s->scratch, vals, cnt * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+    Geo g = s->geo();
+    hipLaunchKernelGGL(k_scatter_box, dim3(grid1d(cnt, 256)), dim3(256), 0, s->stream, s->eps[c],
+                       s->scratch, g, k1, k2 - k1, j1, j2 - j1, i1, i2 - i1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->coef_dirty = true;
+    return EMF_OK;
+}
+
+int emf_get_mat_arrays(emf_solver *s, double *ex, double *ey, double *ez, double *mx, double *my,
+                       double *mz)
+{
+    LOCK(s);
+    if (!s->mat_set) EMF_FAIL(EMF_ERR_STATE, "materials not set");
+    double *e[3] = { ex, ey, ez }, *m[3] = { mx, my, mz };
+    for (int c = 0; c < 3; ++c) {
+        if (e[c])
+            HIPCHK(hipMemcpyAsync(e[c], s->eps[c], s->n * sizeof(double2), hipMemcpyDeviceToHost,
+                                  s->stream));
+        if (m[c]) {
+            if (s->mu_uniform) {
+                HIPCHK(hipStreamSynchronize(s->stream));
+                for (long long g = 0; g < s->n; ++g) { m[c][2 * g] = s->mu_re; m[c][2 * g + 1] = 0.0; }
+            } else {
+                HIPCHK(hipMemcpyAsync(m[c], s->mu[c], s->n * sizeof(double2),
+                                      hipMemcpyDeviceToHost, s->stream));
+            }
+        }
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EMF_OK;
+}
+
+int emf_set_bc(emf_solver *s, const char *bc)
+{
+    LOCK(s);
+    if (!bc || strlen(bc) < 3) EMF_FAIL(EMF_ERR_ARG, "bc must have 3 characters");
+    for (int a = 0; a < 3; ++a)
+        if (bc[a] != '0' && bc[a] != 'E' && bc[a] != 'H')
+            EMF_FAIL(EMF_ERR_ARG, "unsupported boundary condition '%c' (use 0, E or H)", bc[a]);
+    for (int a = 0; a < 3; ++a) s->bc[a] = bc[a];
+    return EMF_OK;
+}
+
+int emf_set_pml_widths(emf_solver *s, int xmin, int xmax, int ymin, int ymax, int zmin, int zmax)
+{
+    LOCK(s);
+    int v[6] = { xmin, xmax, ymin, ymax, zmin, zmax };
+    for (int q = 0; q < 6; ++q)
+        if (v[q] < 0) EMF_FAIL(EMF_ERR_ARG, "negative pml width");
+    memcpy(s->w, v, sizeof(v));
+    s->pml_built = false;
+    s->tab_dirty = true;
+    return EMF_OK;
+}
+
+int emf_set_pml_properties(emf_solver *s, double sigma, double alpha, double kappa, double pw)
+{
+    LOCK(s);
+    s->sigma = sigma; s->alpha = alpha; s->kappa = kappa; s->pw = pw;
+    s->tab_dirty = true;
+    return EMF_OK;
+}
+
+int emf_build_pml(emf_solver *s)
+{
+    LOCK(s);
+    return build_psi(s);
+}
+
+int emf_reset_pml(emf_solver *s)
+{
+    LOCK(s);
+    return reset_psi(s);
+}
+
+int emf_add_source(emf_solver *s, const double *Jx, const double *Jy, const double *Jz,
+                   const double *Mx, const double *My, const double *Mz, int i0, int j0, int k0,
+                   int I, int J, int K, int calc_phase)
+{
+    LOCK(s);
+    const double *f[6] = { Jx, Jy, Jz, Mx, My, Mz };
+    for (int c = 0; c < 6; ++c)
+        if (!f[c]) EMF_FAIL(EMF_ERR_ARG, "null source array");
+    if (I <= 0 || J <= 0 || K <= 0) EMF_FAIL(EMF_ERR_ARG, "empty source block");
+    if (i0 < 0 || j0 < 0 || k0 < 0 || i0 + I > s->Nz || j0 + J > s->Ny || k0 + K > s->Nx)
+        EMF_FAIL(EMF_ERR_ARG, "source block outside the grid");
+    long long cnt = (long long)I * J * K;
+    SourceDev d;
+    d.i0 = i0; d.j0 = j0; d.k0 = k0; d.I = I; d.J = J; d.K = K;
+    for (int c = 0; c < 6; ++c) d.f[c] = nullptr;
+    std::vector<double> buf(2 * cnt);
+    for (int c = 0; c < 6; ++c) {
+        for (long long m = 0; m < cnt; ++m) {
+            double re = f[c][2 * m], im = f[c][2 * m + 1];
+            if (calc_phase) {   // (|v|, -atan2(im, re)), src/fdtd.cpp:1517-1576
+                buf[2 * m] = sqrt(re * re + im * im);
+                buf[2 * m + 1] = (im == 0 && re == 0) ? 0.0 : -1 * atan2(im, re);
+            } else {
+                buf[2 * m] = re;
+                buf[2 * m + 1] = im;
+            }
+        }
+        int rc = dalloc(&d.f[c], cnt);
+        if (rc != EMF_OK) {
+            for (int q = 0; q < c; ++q) dfree(d.f[q]);
+            return rc;
+        }
+        HIPCHK(hipMemcpy(d.f[c], buf.data(), cnt * sizeof(double2), hipMemcpyHostToDevice));
+    }
+    s->srcs.push_back(d);
+    return EMF_OK;
+}
+
+int emf_clear_sources(emf_solver *s)
+{
+    LOCK(s);
+    (void)hipStreamSynchronize(s->stream);
+    for (auto &d : s->srcs)
+        for (int c = 0; c < 6; ++c) dfree(d.f[c]);
+    s->srcs.clear();
+    return EMF_OK;
+}
+
+int emf_set_source_properties(emf_solver *s, double T, double m)
+{
+    LOCK(s);
+    if (!(T > 0) || !(m > 0)) EMF_FAIL(EMF_ERR_ARG, "ramp time and minimum must be > 0");
+    s->src_T = T;
+    s->src_min = m;
+    s->src_k = T * T / log((1 + m) / m);
+    return EMF_OK;
+}
+
+double emf_src_func_t(emf_solver *s, int n, double t, double phase)
+{
+    (void)n;
+    if (!s) return NAN;
+    if (t <= s->src_T)
+        return sin(t + phase) * ((1 + s->src_min) * exp(-(t - s->src_T) * (t - s->src_T) / s->src_k) -
+                                 s->src_min);
+    return sin(t + phase);
+}
+
+int emf_update_H(emf_solver *s, int n, double t)
+{
+    LOCK(s);
+    (void)n;
+    CHECK(check_ready(s));
+    return launch_h(s, t, nullptr);
+}
+
+int emf_update_E(emf_solver *s, int n, double t)
+{
+    LOCK(s);
+    (void)n;
+    CHECK(check_ready(s));
+    return launch_e(s, t, nullptr);
+}
+
+int emf_zero_fields(emf_solver *s)
+{
+    LOCK(s);
+    CHECK(ensure_fields(s));
+    for (int c = 0; c < 3; ++c) {
+        HIPCHK(hipMemsetAsync(s->E[c], 0, s->n * sizeof(double), s->stream));
+        HIPCHK(hipMemsetAsync(s->H[c], 0, s->n * sizeof(double), s->stream));
+    }
+    return EMF_OK;
+}
+
+int emf_get_time_fields(emf_solver *s, double *ex, double *ey, double *ez, double *hx, double *hy,
+                        double *hz)
+{
+    LOCK(s);
+    CHECK(ensure_fields(s));
+    double *o[6] = { ex, ey, ez, hx, hy, hz };
+    for (int c = 0; c < 6; ++c)
+        if (o[c])
+            HIPCHK(hipMemcpyAsync(o[c], c < 3 ? s->E[c] : s->H[c - 3], s->n * sizeof(double),
+                                  hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EMF_OK;
+}
+
+int emf_set_time_fields(emf_solver *s, const double *ex, const double *ey, const double *ez,
+                        const double *hx, const double *hy, const double *hz)
+{
+    LOCK(s);
+    CHECK(ensure_fields(s));
+    const double *o[6] = { ex, ey, ez, hx, hy, hz };
+    for (int c = 0; c < 6; ++c)
+        if (o[c])
+            HIPCHK(hipMemcpyAsync(c < 3 ? s->E[c] : s->H[c - 3], o[c], s->n * sizeof(double),
+                                  hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EMF_OK;
+}
+
+int emf_capture_t0_fields(emf_solver *s, int slot)
+{
+    LOCK(s);
+    return capture(s, slot, 0);
+}
+
+int emf_capture_t1_fields(emf_solver *s, int slot)
+{
+    LOCK(s);
+    return capture(s, slot, 1);
+}
+
+int emf_calc_complex_fields_3T(emf_solver *s, int slot, double t0, double t1, double t2)
+{
+    LOCK(s);
+    return phasor3(s, slot, t0, t1, t2);
+}
+
+int emf_calc_complex_fields_2T(emf_solver *s, int slot, double t0, double t1)
+{
+    LOCK(s);
+    CHECK(ensure_fields(s));
+    CHECK(ensure_slot(s, slot));
+    PhTimes2 te, thh;
+    double h0 = t0 - 0.5 * s->dt, h1 = t1 - 0.5 * s->dt;
+    te.st0 = sin(t0); te.st1 = sin(t1); te.ct0 = cos(t0); te.ct1 = cos(t1);
+    thh.st0 = sin(h0); thh.st1 = sin(h1); thh.ct0 = cos(h0); thh.ct1 = cos(h1);
+    double2 **r = s->res[slot];
+    hipLaunchKernelGGL(k_phasor2, dim3(grid1d(s->n, 256)), dim3(256), 0, s->stream, s->E[0],
+                       s->E[1], s->E[2], s->H[0], s->H[1], s->H[2], r[0], r[1], r[2], r[3], r[4],
+                       r[5], s->n, te, thh);
+    HIPCHK(hipGetLastError());
+    return EMF_OK;
+}
+
+double emf_calc_phase_2T(double t0, double t1, double f0, double f1)
+{
+    return phase_2t(t0, t1, f0, f1);
+}
+
+double emf_calc_amplitude_2T(double t0, double t1, double f0, double f1, double ph)
+{
+    return amplitude_2t(t0, t1, f0, f1, ph);
+}
+
+double emf_calc_phase_3T(double t0, double t1, double t2, double f0, double f1, double f2)
+{
+    return phase_3t(t0, t1, t2, f0, f1, f2);
+}
+
+double emf_calc_amplitude_3T(double t0, double t1, double t2, double f0, double f1, double f2,
+                             double ph)
+{
+    return amplitude_3t(t0, t1, t2, f0, f1, f2, ph);
+}
+
+int emf_set_nconv(emf_solver *s, int nconv)
+{
+    LOCK(s);
+    if (nconv <= 0) EMF_FAIL(EMF_ERR_ARG, "nconv must be > 0");
+    s->nconv = nconv;
+    return EMF_OK;
+}
+
+int emf_solve(emf_solver *s, int slot, double rtol, double nmax, int64_t Tn, int64_t fixed_n0,
+              emf_solve_info *info)
+{
+    LOCK(s);
+    if (!(rtol > 0)) EMF_FAIL(EMF_ERR_ARG, "rtol must be > 0");
+    if (Tn < 0) EMF_FAIL(EMF_ERR_ARG, "Tn must be >= 0");
+    CHECK(check_ready(s));
+    CHECK(ensure_slot(s, slot));
+    CHECK(build_samples(s));
+    long long cap = (Tn > 0 ? (long long)(std::min(std::max(nmax, 0.0), 1e15) / (double)Tn) : 0) + 4;
+    cap = std::min<long long>(cap, 1LL << 20);
+    if (cap > s->hist_cap) {
+        dfree(s->hist);
+        CHECK(dalloc(&s->hist, 3 * (size_t)cap));
+        s->hist_cap = (int)std::min<long long>(cap, 1 << 30);
+    }
+    auto t_start = std::chrono::steady_clock::now();
+    CHECK(reset_psi(s));
+    for (int c = 0; c < 3; ++c) {
+        HIPCHK(hipMemsetAsync(s->E[c], 0, s->n * sizeof(double), s->stream));
+        HIPCHK(hipMemsetAsync(s->H[c], 0, s->n * sizeof(double), s->stream));
+    }
+    HIPCHK(hipMemsetAsync(s->conv, 0, sizeof(ConvState), s->stream));
+    const double amp_rtol = rtol, phi_rtol = sqrt(rtol);
+    int *dflag = &s->conv->done;
+
+    long long n = 0;
+    int status = 0;
+    if (fixed_n0 >= 0) {
+        for (long long q = 0; q < fixed_n0; ++q) CHECK(step(s, q, nullptr));
+        n = fixed_n0;
+        status = 2;
+    } else {
+        // The loop breaks at the first step index n > nmax (emopt/fdtd.py:1054).
+        long long n_stop = nmax < 0 ? 0 : (long long)floor(nmax) + 1;
+        const int LOOK = 2;               // chunks in flight before the first flag is read
+        std::vector<std::pair<long long, hipEvent_t>> pend;
+        int ring = 0;
+        bool converged = false;
+        while (true) {
+            while ((int)pend.size() < LOOK && Tn > 0 && n + Tn <= n_stop) {
+                for (long long q = 0; q < Tn; ++q) CHECK(step(s, n + q, dflag));
+                long long nchk = n + Tn - 1;
+                int slotf = ring++ % 64;
+                hipLaunchKernelGGL(k_conv_check, dim3(1), dim3(64), 0, s->stream, s->E[0],
+                                   s->E[1], s->E[2], s->samples, s->nsamples,
+                                   (double)nchk * s->dt, amp_rtol, phi_rtol, nchk, s->conv,
+                                   s->hist, s->hist_cap, (int *)nullptr);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(&s->done_host[slotf], dflag, sizeof(int),
+                                      hipMemcpyDeviceToHost, s->stream));
+                hipEvent_t ev;
+                HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                HIPCHK(hipEventRecord(ev, s->stream));
+                pend.push_back({ n, ev });
+                n += Tn;
+            }
+            if (pend.empty()) {
+                // fewer than Tn steps remain before nmax: run them and stop
+                for (; n < n_stop; ++n) CHECK(step(s, n, dflag));
+                status = 1;
+                break;
+            }
+            auto front = pend.front();
+            pend.erase(pend.begin());
+            HIPCHK(hipEventSynchronize(front.second));
+            (void)hipEventDestroy(front.second);
+            int slotf = (ring - 1 - (int)pend.size()) % 64;
+            if (s->done_host[slotf]) {
+                converged = true;
+                n = front.first + Tn;
+                break;
+            }
+        }
+        for (auto &pe : pend) {
+            (void)hipEventSynchronize(pe.second);
+            (void)hipEventDestroy(pe.second);
+        }
+        (void)converged;
+    }
+    // capture / two more Tn-step runs / phasor (emopt/fdtd.py:1156-1194)
+    long long n0 = n;
+    CHECK(capture(s, slot, 0));
+    for (long long q = 0; q < Tn; ++q) CHECK(step(s, n0 + q, nullptr));
+    CHECK(capture(s, slot, 1));
+    for (long long q = 0; q < Tn; ++q) CHECK(step(s, n0 + Tn + q, nullptr));
+    CHECK(phasor3(s, slot, (double)n0 * s->dt, (double)(n0 + Tn) * s->dt,
+                  (double)(n0 + 2 * Tn) * s->dt));
+    ConvState cs;
+    HIPCHK(hipMemcpyAsync(&cs, s->conv, sizeof(ConvState), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    auto t_end = std::chrono::steady_clock::now();
+    s->nhist = std::min(cs.nchecks, s->hist_cap);
+    if (info) {
+        info->n0 = n0;
+        info->Tn = Tn;
+        info->steps = n0 + 2 * Tn;
+        info->status = status;
+        info->nchecks = cs.nchecks;
+        info->a_change = cs.a_change;
+        info->phi_change = cs.phi_change;
+        info->seconds = std::chrono::duration<double>(t_end - t_start).count();
+    }
+    return EMF_OK;
+}
+
+int emf_get_history(emf_solver *s, double *out, int cap)
+{
+    LOCK(s);
+    int nrec = s->nhist;
+    if (out && cap > 0 && nrec > 0) {
+        int m = std::min(cap, nrec);
+        HIPCHK(hipMemcpy(out, s->hist, 3 * (size_t)m * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return nrec;
+}
+
+int emf_get_field(emf_solver *s, int slot, int comp, int k1, int k2, int j1, int j2, int i1,
+                  int i2, double *out)
+{
+    LOCK(s);
+    if (slot < 0 || slot > 1 || !s->res[slot][0]) EMF_FAIL(EMF_ERR_STATE, "result slot %d empty", slot);
+    if (comp < 0 || comp > 5 || !out) EMF_FAIL(EMF_ERR_ARG, "bad component or null output");
+    if (k1 < 0 || j1 < 0 || i1 < 0 || k2 > s->Nx || j2 > s->Ny || i2 > s->Nz || k1 > k2 ||
+        j1 > j2 || i1 > i2)
+        EMF_FAIL(EMF_ERR_ARG, "box out of range");
+    long long cnt = (long long)(k2 - k1) * (j2 - j1) * (i2 - i1);
+    if (cnt == 0) return EMF_OK;
+    if (cnt == s->n) {
+        HIPCHK(hipMemcpyAsync(out, s->res[slot][comp], cnt * sizeof(double2),
+                              hipMemcpyDeviceToHost, s->stream));
+    } else {
+        CHECK(ensure_scratch(s, cnt));
+        Geo g = s->geo();
+        hipLaunchKernelGGL(k_gather_box, dim3(grid1d(cnt, 256)), dim3(256), 0, s->stream,
+                           s->res[slot][comp], s->scratch, g, k1, k2 - k1, j1, j2 - j1, i1, i2 - i1);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, s->scratch, cnt * sizeof(double2), hipMemcpyDeviceToHost,
+                              s->stream));
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EMF_OK;
+}
+
+int emf_calc_ydAx(emf_solver *s, const double *eps0_x, const double *eps0_y,
+                  const double *eps0_z, int k1, int k2, int j1, int j2, int i1, int i2, double *out)
+{
+    LOCK(s);
+    if (!out || !eps0_x || !eps0_y || !eps0_z) EMF_FAIL(EMF_ERR_ARG, "null argument");
+    if (!s->res[0][0] || !s->res[1][0]) EMF_FAIL(EMF_ERR_STATE, "forward and adjoint results needed");
+    if (k1 < 0 || j1 < 0 || i1 < 0 || k2 > s->Nx || j2 > s->Ny || i2 > s->Nz || k1 > k2 ||
+        j1 > j2 || i1 > i2)
+        EMF_FAIL(EMF_ERR_ARG, "box out of range");
+    long long cnt = (long long)(k2 - k1) * (j2 - j1) * (i2 - i1);
+    out[0] = out[1] = 0.0;
+    if (cnt == 0) return EMF_OK;
+    const int nblk = 1024;
+    CHECK(ensure_scratch(s, 3 * cnt + nblk));
+    double2 *z = s->scratch, *part = s->scratch + 3 * cnt;
+    const double *e0[3] = { eps0_x, eps0_y, eps0_z };
+    for (int c = 0; c < 3; ++c)
+        HIPCHK(hipMemcpyAsync(z + c * cnt, e0[c], cnt * sizeof(double2), hipMemcpyHostToDevice,
+                              s->stream));
+    Geo g = s->geo();
+    double2 **A = s->res[1], **F = s->res[0];
+    hipLaunchKernelGGL(k_ydAx, dim3(nblk), dim3(RED_THREADS), 0, s->stream, A[0], A[1], A[2], F[0],
+                       F[1], F[2], s->eps[0], s->eps[1], s->eps[2], z, z + cnt, z + 2 * cnt, g, k1,
+                       k2 - k1, j1, j2 - j1, i1, i2 - i1, part);
+    HIPCHK(hipGetLastError());
+    std::vector<double2> h(nblk);
+    HIPCHK(hipMemcpyAsync(h.data(), part, nblk * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    double re = 0, im = 0;
+    for (int b = 0; b < nblk; ++b) { re += h[b].x; im += h[b].y; }
+    out[0] = re;
+    out[1] = im;
+    return EMF_OK;
+}
+
+int emf_grad_assemble(emf_solver *s, int64_t n, const int32_t *pidx, const int32_t *comp,
+                      const int64_t *cell, const double *deps, int nparam, double step,
+                      double *grad)
+{
+    LOCK(s);
+    if (nparam <= 0 || !grad) EMF_FAIL(EMF_ERR_ARG, "nparam must be > 0");
+    if (n < 0 || (n > 0 && (!pidx || !comp || !cell || !deps))) EMF_FAIL(EMF_ERR_ARG, "null entries");
+    if (!(step != 0)) EMF_FAIL(EMF_ERR_ARG, "step must be non-zero");
+    if (!s->res[0][0] || !s->res[1][0]) EMF_FAIL(EMF_ERR_STATE, "forward and adjoint results needed");
+    std::vector<long long> seg(nparam + 1, 0);
+    for (int64_t e = 0; e < n; ++e) {
+        if (pidx[e] < 0 || pidx[e] >= nparam) EMF_FAIL(EMF_ERR_ARG, "parameter index out of range");
+        if (e > 0 && pidx[e] < pidx[e - 1]) EMF_FAIL(EMF_ERR_ARG, "entries must be grouped by parameter");
+        if (comp[e] < 0 || comp[e] > 2) EMF_FAIL(EMF_ERR_ARG, "component must be 0..2");
+        if (cell[e] < 0 || cell[e] >= s->n) EMF_FAIL(EMF_ERR_ARG, "cell index out of range");
+        seg[pidx[e] + 1]++;
+    }
+    for (int p = 0; p < nparam; ++p) seg[p + 1] += seg[p];
+    long long *dseg = nullptr, *dcell = nullptr;
+    int *dcomp = nullptr;
+    double2 *ddeps = nullptr;
+    double *dgrad = nullptr;
+    int rc = EMF_OK;
+    do {
+        if ((rc = dalloc(&dseg, seg.size())) != EMF_OK) break;
+        if ((rc = dalloc(&dgrad, (size_t)nparam)) != EMF_OK) break;
+        if (n > 0) {
+            if ((rc = dalloc(&dcell, (size_t)n)) != EMF_OK) break;
+            if ((rc = dalloc(&dcomp, (size_t)n)) != EMF_OK) break;
+            if ((rc = dalloc(&ddeps, (size_t)n)) != EMF_OK) break;
+        }
+        hipError_t e = hipMemcpy(dseg, seg.data(), seg.size() * sizeof(long long), hipMemcpyHostToDevice);
+        if (e == hipSuccess && n > 0) e = hipMemcpy(dcell, cell, n * sizeof(long long), hipMemcpyHostToDevice);
+        if (e == hipSuccess && n > 0) e = hipMemcpy(dcomp, comp, n * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess && n > 0) e = hipMemcpy(ddeps, deps, n * sizeof(double2), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { set_err("hipMemcpy failed: %s", hipGetErrorString(e)); rc = EMF_ERR_HIP; break; }
+        double2 **A = s->res[1], **F = s->res[0];
+        hipLaunchKernelGGL(k_grad, dim3(nparam), dim3(RED_THREADS), 0, s->stream, A[0], A[1], A[2],
+                           F[0], F[1], F[2], dseg, dcomp, dcell, ddeps, step, dgrad);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(grad, dgrad, nparam * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) { set_err("gradient kernel failed: %s", hipGetErrorString(e)); rc = EMF_ERR_HIP; }
+    } while (0);
+    dfree(dseg); dfree(dcell); dfree(dcomp); dfree(ddeps); dfree(dgrad);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,678 @@
+// Device kernels of the CW-FDTD hot path for gfx950 (MI355X), FP64.
+//
+// Layout in HBM: every array is unpadded, [Nz][Ny][Nx] with x fastest, cell
+// g = (i*Ny + j)*Nx + k, the reference's "global" index (src/fdtd.cpp:183).
+// There are no ghost layers: the boundary values the reference keeps in its
+// padded arrays are produced inside the kernels:
+//   * H pass: E beyond a max face reads as 0 (PEC ghosts, src/fdtd.cpp:142-173)
+//   * E pass: H beyond a min face reads as 0 / -H(first) / +H(first) for
+//     bc '0' / 'E' / 'H' (src/fdtd.cpp:397-501)
+//
+// Thread mapping: a 64x4 (x, y) work-group owns a tile of columns and sweeps
+// a chunk of z planes, keeping the z-neighbour it needs in registers (the H
+// pass carries E(i+1) -> E(i); the E pass carries H(i) -> H(i-1)), so every
+// HBM byte of the interior stencil is loaded once per pass.  In-plane
+// neighbours (x+1/x-1, y+1/y-1) are re-read through L1/L2.  Work-group ids are
+// remapped so that each XCD (own L2) receives a contiguous range of tiles.
+//
+// Arithmetic follows the reference's operation order term by term, so the
+// only differences to the CPU are FMA contraction and libm last-bit effects.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace emf {
+
+constexpr int TX = 64;          // lanes along x: one wavefront row
+constexpr int TY = 4;           // rows along y per work-group
+constexpr int MAX_INLINE_SRC = 4;
+constexpr int NCONV_MAX = 64;
+constexpr int NUM_XCD = 8;
+
+struct Geo {
+    int Nx, Ny, Nz;
+    long long nxy;
+    double rx, ry, rz;   // R/dx, R/dy, R/dz
+    double dt;
+    int w[6];            // CPML widths xmin xmax ymin ymax zmin zmax
+    int bc[3];           // '0', 'E', 'H' per axis x, y, z
+    int zc;              // z planes per work-group
+};
+
+// CPML coefficient tables of one pass (H uses the *_H_* set, E the *_E_*).
+struct PassTab {
+    const double *kap[3], *b[3], *c[3];
+};
+
+// psi slabs of one pass: [face][transverse component], NULL where width 0.
+struct PassPsi {
+    double *p[6][2];
+};
+
+struct SrcBlk {
+    int i0, j0, k0, I, J, K;
+    const double2 *f[3];   // (amplitude, phase) per cell, one per component
+};
+
+struct PassSrc {
+    int n;
+    double t;     // time argument of src_func_t
+    double env;   // ramp envelope at t (exactly 1 after the ramp)
+    SrcBlk b[MAX_INLINE_SRC];
+};
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ unsigned xcd_remap(unsigned lin, unsigned nb)
+{
+    // Consecutive work-group ids are dispatched round-robin over the XCDs;
+    // give XCD x the contiguous logical range of tiles [start_x, start_x+count_x).
+    unsigned q = nb / NUM_XCD, r = nb % NUM_XCD;
+    unsigned x = lin % NUM_XCD, slot = lin / NUM_XCD;
+    return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
+__device__ __forceinline__ void tile_of(const Geo &g, int &k, int &j, int &z0, int &z1)
+{
+    unsigned nbx = gridDim.x, nby = gridDim.y, nbz = gridDim.z;
+    unsigned lin = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    unsigned l = xcd_remap(lin, nbx * nby * nbz);
+    unsigned bx = l % nbx, by = (l / nbx) % nby, bz = l / (nbx * nby);
+    k = (int)(bx * TX + threadIdx.x);
+    j = (int)(by * TY + threadIdx.y);
+    z0 = (int)bz * g.zc;
+    z1 = min(z0 + g.zc, g.Nz);
+}
+
+// psi <- c*d + b*psi ; returns the new psi
+__device__ __forceinline__ double psi_step(double *psi, long long ip, double c, double b, double d)
+{
+    double v = c * d + b * psi[ip];
+    psi[ip] = v;
+    return v;
+}
+
+__device__ __forceinline__ bool in_box(const SrcBlk &s, int i, int j, int k, long long &m)
+{
+    const int a = i - s.i0, b = j - s.j0, c = k - s.k0;
+    if ((unsigned)a < (unsigned)s.I && (unsigned)b < (unsigned)s.J && (unsigned)c < (unsigned)s.K) {
+        m = ((long long)a * s.J + b) * s.K + c;
+        return true;
+    }
+    return false;
+}
+
+// --------------------------------------------------------------- H pass
+//
+// update_H of src/fdtd.cpp:121-367 for one z-chunk of one (x, y) column.
+// mH* = dt/mu.re per cell (MU_ARRAY) or the scalar m_s; muR* = mu.re per cell
+// (or mu_s) for the source term ((src_t*|M|)*dt)/mu.re.
+template <bool MU_ARRAY>
+__global__ __launch_bounds__(TX * TY) void k_update_h(
+    const double *__restrict__ Ex, const double *__restrict__ Ey, const double *__restrict__ Ez,
+    double *__restrict__ Hx, double *__restrict__ Hy, double *__restrict__ Hz,
+    const double *__restrict__ mHx, const double *__restrict__ mHy, const double *__restrict__ mHz,
+    const double *__restrict__ muRx, const double *__restrict__ muRy, const double *__restrict__ muRz,
+    double m_s, double mu_s, Geo g, PassTab tab, PassPsi psi, PassSrc src,
+    const int *__restrict__ done)
+{
+    if (done && *done) return;
+    int k, j, z0, z1;
+    tile_of(g, k, j, z0, z1);
+    if (k >= g.Nx || j >= g.Ny) return;
+    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    const long long nxy = g.nxy;
+    const bool jn = j + 1 < Ny, kn = k + 1 < Nx;
+    const int xlo = g.w[0], xhi = Nx - g.w[1], ylo = g.w[2], yhi = Ny - g.w[3];
+    const int zlo = g.w[4], zhi = Nz - g.w[5];
+
+    long long p = (long long)z0 * nxy + (long long)j * Nx + k;
+    double ex = Ex[p], ey = Ey[p];
+    for (int i = z0; i < z1; ++i, p += nxy) {
+        const bool in_ = i + 1 < Nz;
+        const double ex_i = in_ ? Ex[p + nxy] : 0.0;
+        const double ey_i = in_ ? Ey[p + nxy] : 0.0;
+        const double ez = Ez[p];
+        const double ez_j = jn ? Ez[p + Nx] : 0.0;
+        const double ex_j = jn ? Ex[p + Nx] : 0.0;
+        const double ez_k = kn ? Ez[p + 1] : 0.0;
+        const double ey_k = kn ? Ey[p + 1] : 0.0;
+        const double mx = MU_ARRAY ? mHx[p] : m_s;
+        const double my = MU_ARRAY ? mHy[p] : m_s;
+        const double mz = MU_ARRAY ? mHz[p] : m_s;
+
+        const double dEzdy = g.ry * (ez_j - ez);
+        const double dEydz = g.rz * (ey_i - ey);
+        double hx = Hx[p] + mx * (dEydz - dEzdy);
+        const double dExdz = g.rz * (ex_i - ex);
+        const double dEzdx = g.rx * (ez_k - ez);
+        double hy = Hy[p] + my * (dEzdx - dExdz);
+        const double dEydx = g.rx * (ey_k - ey);
+        const double dExdy = g.ry * (ex_j - ex);
+        double hz = Hz[p] + mz * (dExdy - dEydx);
+
+        // CPML slabs in the reference's x, y, z order (src/fdtd.cpp:212-306)
+        if (k < xlo || k >= xhi) {
+            const bool lo = k < xlo;
+            double *const q0 = lo ? psi.p[0][0] : psi.p[1][0];
+            double *const q1 = lo ? psi.p[0][1] : psi.p[1][1];
+            const int wl = lo ? g.w[0] : g.w[1];
+            const int kk = lo ? k : k - xhi;
+            const int par = lo ? xlo - k - 1 : k - xhi + g.w[0];
+            const long long ip = ((long long)i * Ny + j) * wl + kk;
+            const double kap = tab.kap[0][par], b = tab.b[0][par], c = tab.c[0][par];
+            const double p0 = psi_step(q0, ip, c, b, dEydx);
+            const double p1 = psi_step(q1, ip, c, b, dEzdx);
+            hz = hz - mz * (p0 - dEydx + dEydx / kap);
+            hy = hy + my * (p1 - dEzdx + dEzdx / kap);
+        }
+        if (j < ylo || j >= yhi) {
+            const bool lo = j < ylo;
+            double *const q0 = lo ? psi.p[2][0] : psi.p[3][0];
+            double *const q1 = lo ? psi.p[2][1] : psi.p[3][1];
+            const int wl = lo ? g.w[2] : g.w[3];
+            const int jj = lo ? j : j - yhi;
+            const int par = lo ? ylo - j - 1 : j - yhi + g.w[2];
+            const long long ip = ((long long)i * wl + jj) * Nx + k;
+            const double kap = tab.kap[1][par], b = tab.b[1][par], c = tab.c[1][par];
+            const double p0 = psi_step(q0, ip, c, b, dExdy);
+            const double p1 = psi_step(q1, ip, c, b, dEzdy);
+            hz = hz + mz * (p0 - dExdy + dExdy / kap);
+            hx = hx - mx * (p1 - dEzdy + dEzdy / kap);
+        }
+        if (i < zlo || i > zhi) {   // zmax test is strict: its first layer is skipped
+            const bool lo = i < zlo;
+            double *const q0 = lo ? psi.p[4][0] : psi.p[5][0];
+            double *const q1 = lo ? psi.p[4][1] : psi.p[5][1];
+            const int ii = lo ? i : i - zhi;
+            const int par = lo ? zlo - i - 1 : i - zhi + g.w[4];
+            const long long ip = ((long long)ii * Ny + j) * Nx + k;
+            const double kap = tab.kap[2][par], b = tab.b[2][par], c = tab.c[2][par];
+            const double p0 = psi_step(q0, ip, c, b, dExdz);
+            const double p1 = psi_step(q1, ip, c, b, dEydz);
+            hx = hx + mx * (p1 - dEydz + dEydz / kap);
+            hy = hy - my * (p0 - dExdz + dExdz / kap);
+        }
+
+        // magnetic current sources (src/fdtd.cpp:313-366), block by block
+#pragma unroll
+        for (int q = 0; q < MAX_INLINE_SRC; ++q) {
+            long long m;
+            if (q < src.n && in_box(src.b[q], i, j, k, m)) {
+                const double2 vx = src.b[q].f[0][m], vy = src.b[q].f[1][m], vz = src.b[q].f[2][m];
+                const double ux = MU_ARRAY ? muRx[p] : mu_s;
+                const double uy = MU_ARRAY ? muRy[p] : mu_s;
+                const double uz = MU_ARRAY ? muRz[p] : mu_s;
+                hx = hx + sin(src.t + vx.y) * src.env * vx.x * g.dt / ux;
+                hy = hy + sin(src.t + vy.y) * src.env * vy.x * g.dt / uy;
+                hz = hz + sin(src.t + vz.y) * src.env * vz.x * g.dt / uz;
+            }
+        }
+
+        Hx[p] = hx; Hy[p] = hy; Hz[p] = hz;
+        ex = ex_i; ey = ey_i;
+    }
+}
+
+// --------------------------------------------------------------- E pass
+//
+// update_E of src/fdtd.cpp:369-755.  bE* = dt/eps.re (real eps) or
+// 1/(eps.re/dt + eps.im/2) (complex eps); aE* only for complex eps.
+template <bool CEPS>
+__global__ __launch_bounds__(TX * TY) void k_update_e(
+    double *__restrict__ Ex, double *__restrict__ Ey, double *__restrict__ Ez,
+    const double *__restrict__ Hx, const double *__restrict__ Hy, const double *__restrict__ Hz,
+    const double *__restrict__ bEx, const double *__restrict__ bEy, const double *__restrict__ bEz,
+    const double *__restrict__ aEx, const double *__restrict__ aEy, const double *__restrict__ aEz,
+    Geo g, PassTab tab, PassPsi psi, PassSrc src, const int *__restrict__ done)
+{
+    if (done && *done) return;
+    int k, j, z0, z1;
+    tile_of(g, k, j, z0, z1);
+    if (k >= g.Nx || j >= g.Ny) return;
+    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    const long long nxy = g.nxy;
+    const int xlo = g.w[0], xhi = Nx - g.w[1], ylo = g.w[2], yhi = Ny - g.w[3];
+    const int zlo = g.w[4], zhi = Nz - g.w[5];
+    // min-face ghost rule: 0 -> 0, 'E' -> -H(first), 'H' -> +H(first)
+    const double sx = g.bc[0] == 'E' ? -1.0 : (g.bc[0] == 'H' ? 1.0 : 0.0);
+    const double sy = g.bc[1] == 'E' ? -1.0 : (g.bc[1] == 'H' ? 1.0 : 0.0);
+    const double sz = g.bc[2] == 'E' ? -1.0 : (g.bc[2] == 'H' ? 1.0 : 0.0);
+    const bool bx0 = g.bc[0] == '0', by0 = g.bc[1] == '0', bz0 = g.bc[2] == '0';
+
+    long long p = (long long)z0 * nxy + (long long)j * Nx + k;
+    double hx_m, hy_m;   // H at i-1
+    if (z0 > 0) {
+        hx_m = Hx[p - nxy];
+        hy_m = Hy[p - nxy];
+    } else {
+        hx_m = bz0 ? 0.0 : sz * Hx[p];
+        hy_m = bz0 ? 0.0 : sz * Hy[p];
+    }
+    for (int i = z0; i < z1; ++i, p += nxy) {
+        const double hx = Hx[p], hy = Hy[p], hz = Hz[p];
+        double hz_j, hx_j, hz_k, hy_k;
+        if (j > 0) { hz_j = Hz[p - Nx]; hx_j = Hx[p - Nx]; }
+        else { hz_j = by0 ? 0.0 : sy * hz; hx_j = by0 ? 0.0 : sy * hx; }
+        if (k > 0) { hz_k = Hz[p - 1]; hy_k = Hy[p - 1]; }
+        else { hz_k = bx0 ? 0.0 : sx * hz; hy_k = bx0 ? 0.0 : sx * hy; }
+        const double bx = bEx[p], by = bEy[p], bz = bEz[p];
+
+        const double dHzdy = g.ry * (hz - hz_j);
+        const double dHydz = g.rz * (hy - hy_m);
+        double ex, ey, ez;
+        if (CEPS) ex = aEx[p] * Ex[p] + (dHzdy - dHydz) * bx;
+        else ex = Ex[p] + (dHzdy - dHydz) * bx;
+        const double dHxdz = g.rz * (hx - hx_m);
+        const double dHzdx = g.rx * (hz - hz_k);
+        if (CEPS) ey = aEy[p] * Ey[p] + (dHxdz - dHzdx) * by;
+        else ey = Ey[p] + (dHxdz - dHzdx) * by;
+        const double dHydx = g.rx * (hy - hy_k);
+        const double dHxdy = g.ry * (hx - hx_j);
+        if (CEPS) ez = aEz[p] * Ez[p] + (dHydx - dHxdy) * bz;
+        else ez = Ez[p] + (dHydx - dHxdy) * bz;
+
+        // CPML slabs (src/fdtd.cpp:573-666)
+        if (k < xlo || k >= xhi) {
+            const bool lo = k < xlo;
+            double *const q0 = lo ? psi.p[0][0] : psi.p[1][0];
+            double *const q1 = lo ? psi.p[0][1] : psi.p[1][1];
+            const int wl = lo ? g.w[0] : g.w[1];
+            const int kk = lo ? k : k - xhi;
+            const int par = lo ? xlo - k - 1 : k - xhi + g.w[0];
+            const long long ip = ((long long)i * Ny + j) * wl + kk;
+            const double kap = tab.kap[0][par], b = tab.b[0][par], c = tab.c[0][par];
+            const double p0 = psi_step(q0, ip, c, b, dHydx);
+            const double p1 = psi_step(q1, ip, c, b, dHzdx);
+            ez = ez + (p0 - dHydx + dHydx / kap) * bz;
+            ey = ey - (p1 - dHzdx + dHzdx / kap) * by;
+        }
+        if (j < ylo || j >= yhi) {
+            const bool lo = j < ylo;
+            double *const q0 = lo ? psi.p[2][0] : psi.p[3][0];
+            double *const q1 = lo ? psi.p[2][1] : psi.p[3][1];
+            const int wl = lo ? g.w[2] : g.w[3];
+            const int jj = lo ? j : j - yhi;
+            const int par = lo ? ylo - j - 1 : j - yhi + g.w[2];
+            const long long ip = ((long long)i * wl + jj) * Nx + k;
+            const double kap = tab.kap[1][par], b = tab.b[1][par], c = tab.c[1][par];
+            const double p0 = psi_step(q0, ip, c, b, dHxdy);
+            const double p1 = psi_step(q1, ip, c, b, dHzdy);
+            ez = ez - (p0 - dHxdy + dHxdy / kap) * bz;
+            ex = ex + (p1 - dHzdy + dHzdy / kap) * bx;
+        }
+        if (i < zlo || i > zhi) {
+            const bool lo = i < zlo;
+            double *const q0 = lo ? psi.p[4][0] : psi.p[5][0];
+            double *const q1 = lo ? psi.p[4][1] : psi.p[5][1];
+            const int ii = lo ? i : i - zhi;
+            const int par = lo ? zlo - i - 1 : i - zhi + g.w[4];
+            const long long ip = ((long long)ii * Ny + j) * Nx + k;
+            const double kap = tab.kap[2][par], b = tab.b[2][par], c = tab.c[2][par];
+            const double p0 = psi_step(q0, ip, c, b, dHxdz);
+            const double p1 = psi_step(q1, ip, c, b, dHydz);
+            ex = ex - (p1 - dHydz + dHydz / kap) * bx;
+            ey = ey + (p0 - dHxdz + dHxdz / kap) * by;
+        }
+
+        // electric current sources (src/fdtd.cpp:672-754)
+#pragma unroll
+        for (int q = 0; q < MAX_INLINE_SRC; ++q) {
+            long long m;
+            if (q < src.n && in_box(src.b[q], i, j, k, m)) {
+                const double2 vx = src.b[q].f[0][m], vy = src.b[q].f[1][m], vz = src.b[q].f[2][m];
+                ex = ex - sin(src.t + vx.y) * src.env * vx.x * bx;
+                ey = ey - sin(src.t + vy.y) * src.env * vy.x * by;
+                ez = ez - sin(src.t + vz.y) * src.env * vz.x * bz;
+            }
+        }
+
+        Ex[p] = ex; Ey[p] = ey; Ez[p] = ez;
+        hx_m = hx; hy_m = hy;
+    }
+}
+
+// Sources beyond the inline ones: one thread per source cell, applied after
+// the main pass in block order (same per-cell order as the reference loops).
+__global__ void k_src_h(double *__restrict__ Hx, double *__restrict__ Hy, double *__restrict__ Hz,
+                        const double *__restrict__ muRx, const double *__restrict__ muRy,
+                        const double *__restrict__ muRz, double mu_s, Geo g, SrcBlk s,
+                        double t, double env, const int *__restrict__ done)
+{
+    if (done && *done) return;
+    long long m = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    long long n = (long long)s.I * s.J * s.K;
+    if (m >= n) return;
+    int k = (int)(m % s.K), j = (int)((m / s.K) % s.J), i = (int)(m / ((long long)s.K * s.J));
+    long long p = (long long)(i + s.i0) * g.nxy + (long long)(j + s.j0) * g.Nx + (k + s.k0);
+    const double2 vx = s.f[0][m], vy = s.f[1][m], vz = s.f[2][m];
+    const double ux = muRx ? muRx[p] : mu_s, uy = muRy ? muRy[p] : mu_s, uz = muRz ? muRz[p] : mu_s;
+    Hx[p] = Hx[p] + sin(t + vx.y) * env * vx.x * g.dt / ux;
+    Hy[p] = Hy[p] + sin(t + vy.y) * env * vy.x * g.dt / uy;
+    Hz[p] = Hz[p] + sin(t + vz.y) * env * vz.x * g.dt / uz;
+}
+
+__global__ void k_src_e(double *__restrict__ Ex, double *__restrict__ Ey, double *__restrict__ Ez,
+                        const double *__restrict__ bEx, const double *__restrict__ bEy,
+                        const double *__restrict__ bEz, Geo g, SrcBlk s, double t, double env,
+                        const int *__restrict__ done)
+{
+    if (done && *done) return;
+    long long m = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    long long n = (long long)s.I * s.J * s.K;
+    if (m >= n) return;
+    int k = (int)(m % s.K), j = (int)((m / s.K) % s.J), i = (int)(m / ((long long)s.K * s.J));
+    long long p = (long long)(i + s.i0) * g.nxy + (long long)(j + s.j0) * g.Nx + (k + s.k0);
+    const double2 vx = s.f[0][m], vy = s.f[1][m], vz = s.f[2][m];
+    Ex[p] = Ex[p] - sin(t + vx.y) * env * vx.x * bEx[p];
+    Ey[p] = Ey[p] - sin(t + vy.y) * env * vy.x * bEy[p];
+    Ez[p] = Ez[p] - sin(t + vz.y) * env * vz.x * bEz[p];
+}
+
+// ------------------------------------------------------ convergence check
+//
+// One check of emopt/fdtd.py:1072-1144 for all nconv sample points, run by a
+// single thread so that the norms are accumulated in a fixed order.
+struct ConvState {
+    double eh[3][3][NCONV_MAX];   // [sample slot 0..2][component][point]
+    double phi1[NCONV_MAX], a1[NCONV_MAX];
+    double t0, t1, t2;
+    double a_change, phi_change;
+    int done;
+    int nchecks;
+};
+
+__device__ inline double phase3(double t0, double t1, double t2, double f0, double f1, double f2)
+{
+    double f10 = f1 - f0, f21 = f2 - f1;
+    if (f10 == 0 && f21 == 0) return 0.0;
+    return atan2(f10 * (sin(t2) - sin(t1)) - f21 * (sin(t1) - sin(t0)),
+                 f21 * (cos(t1) - cos(t0)) - f10 * (cos(t2) - cos(t1)));
+}
+
+__device__ inline double amp3(double t0, double t1, double t2, double f0, double f1, double f2,
+                              double ph)
+{
+    double f21 = f2 - f1, f10 = f1 - f0;
+    if (f21 == 0 && f10 == 0) return 0.0;
+    if (f21 * f21 >= f10 * f10)
+        return f21 / (cos(ph) * (sin(t2) - sin(t1)) + sin(ph) * (cos(t2) - cos(t1)));
+    return f10 / (cos(ph) * (sin(t1) - sin(t0)) + sin(ph) * (cos(t1) - cos(t0)));
+}
+
+__global__ void k_conv_check(const double *__restrict__ Ex, const double *__restrict__ Ey,
+                             const double *__restrict__ Ez, const long long *__restrict__ sample,
+                             int nc, double tnow, double amp_rtol, double phi_rtol,
+                             long long nstep, ConvState *st, double *hist, int hist_cap,
+                             int *done_out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (st->done) return;
+    st->t0 = st->t1; st->t1 = st->t2; st->t2 = tnow;
+    const double t0 = st->t0, t1 = st->t1, t2 = st->t2;
+    const double *E[3] = { Ex, Ey, Ez };
+    double sa = 0, s0 = 0, sp = 0, sp0 = 0;
+    for (int q = 0; q < nc; ++q) {
+        double ph[3], am[3];
+        for (int c = 0; c < 3; ++c) {
+            st->eh[0][c][q] = st->eh[1][c][q];
+            st->eh[1][c][q] = st->eh[2][c][q];
+            st->eh[2][c][q] = sample[q] >= 0 ? E[c][sample[q]] : 0.0;
+            const double f0 = st->eh[0][c][q], f1 = st->eh[1][c][q], f2 = st->eh[2][c][q];
+            ph[c] = phase3(t0, t1, t2, f0, f1, f2);
+            am[c] = amp3(t0, t1, t2, f0, f1, f2, ph[c]);
+        }
+        for (int c = 0; c < 3; ++c)
+            if (am[c] < 0) { am[c] *= -1; ph[c] += M_PI; }
+        const double phi0 = st->phi1[q], a0 = st->a1[q];
+        const double phi1 = ph[0] + ph[1] + ph[2];
+        const double a1 = am[0] + am[1] + am[2];
+        st->phi1[q] = phi1; st->a1[q] = a1;
+        sa += (a1 - a0) * (a1 - a0); s0 += a0 * a0;
+        sp += (phi1 - phi0) * (phi1 - phi0); sp0 += phi0 * phi0;
+    }
+    const double ac = sqrt(sa) / sqrt(s0), pc = sqrt(sp) / sqrt(sp0);
+    st->a_change = ac; st->phi_change = pc;
+    const int h = st->nchecks++;
+    if (hist && h < hist_cap) {
+        hist[3 * h] = (double)nstep; hist[3 * h + 1] = ac; hist[3 * h + 2] = pc;
+    }
+    const int conv = (ac <= amp_rtol) && (pc <= phi_rtol);
+    st->done = conv;
+    if (done_out) *done_out = conv;
+}
+
+// -------------------------------------------------------- capture / phasor
+
+// t0 snapshot -> .x, t1 snapshot -> .y of the result slot (src/fdtd.cpp:1205-1251)
+__global__ void k_capture(const double *__restrict__ f0, const double *__restrict__ f1,
+                          const double *__restrict__ f2, const double *__restrict__ f3,
+                          const double *__restrict__ f4, const double *__restrict__ f5,
+                          double2 *r0, double2 *r1, double2 *r2, double2 *r3, double2 *r4,
+                          double2 *r5, long long n, int which)
+{
+    long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; g < n; g += (long long)gridDim.x * blockDim.x) {
+        double *d0 = which ? &r0[g].y : &r0[g].x;
+        double *d1 = which ? &r1[g].y : &r1[g].x;
+        double *d2 = which ? &r2[g].y : &r2[g].x;
+        double *d3 = which ? &r3[g].y : &r3[g].x;
+        double *d4 = which ? &r4[g].y : &r4[g].x;
+        double *d5 = which ? &r5[g].y : &r5[g].x;
+        *d0 = f0[g]; *d1 = f1[g]; *d2 = f2[g]; *d3 = f3[g]; *d4 = f4[g]; *d5 = f5[g];
+    }
+}
+
+// Differences of sin/cos of the three sample times, precomputed on the host
+// with the same expressions the reference evaluates per cell.
+struct PhTimes {
+    double s21, s10, c10, c21;   // 3T: sin(t2)-sin(t1), sin(t1)-sin(t0), cos(t1)-cos(t0), cos(t2)-cos(t1)
+};
+
+__device__ __forceinline__ double2 phasor3(double f0, double f1, double f2, const PhTimes &T)
+{
+    const double f10 = f1 - f0, f21 = f2 - f1;
+    double ph, A;
+    if (f10 == 0 && f21 == 0) {
+        ph = 0.0; A = 0.0;
+    } else {
+        ph = atan2(f10 * T.s21 - f21 * T.s10, f21 * T.c10 - f10 * T.c21);
+        if (f21 * f21 >= f10 * f10) A = f21 / (cos(ph) * T.s21 + sin(ph) * T.c21);
+        else A = f10 / (cos(ph) * T.s10 + sin(ph) * T.c10);
+    }
+    if (A < 0) { A *= -1; ph += M_PI; }
+    return make_double2(A * cos(ph), -A * sin(ph));
+}
+
+// calc_complex_fields 3T (src/fdtd.cpp:1346-1443): slot holds (t0, t1) per cell
+__global__ void k_phasor3(const double *__restrict__ f0, const double *__restrict__ f1,
+                          const double *__restrict__ f2, const double *__restrict__ f3,
+                          const double *__restrict__ f4, const double *__restrict__ f5,
+                          double2 *r0, double2 *r1, double2 *r2, double2 *r3, double2 *r4,
+                          double2 *r5, long long n, PhTimes TE, PhTimes TH)
+{
+    long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; g < n; g += (long long)gridDim.x * blockDim.x) {
+        double2 v;
+        v = r0[g]; r0[g] = phasor3(v.x, v.y, f0[g], TE);
+        v = r1[g]; r1[g] = phasor3(v.x, v.y, f1[g], TE);
+        v = r2[g]; r2[g] = phasor3(v.x, v.y, f2[g], TE);
+        v = r3[g]; r3[g] = phasor3(v.x, v.y, f3[g], TH);
+        v = r4[g]; r4[g] = phasor3(v.x, v.y, f4[g], TH);
+        v = r5[g]; r5[g] = phasor3(v.x, v.y, f5[g], TH);
+    }
+}
+
+// 2T variant (src/fdtd.cpp:1253-1343,1445-1463)
+struct PhTimes2 {
+    double st0, st1, ct0, ct1;
+};
+
+__device__ __forceinline__ double2 phasor2(double f0, double f1, const PhTimes2 &T)
+{
+    double ph, A;
+    if (f0 == 0.0 && f1 == 0) ph = 0.0;
+    else ph = atan((f1 * T.st0 - f0 * T.st1) / (f0 * T.ct1 - f1 * T.ct0));
+    if (f0 * f0 > f1 * f1) A = f1 / (T.st1 * cos(ph) + T.ct1 * sin(ph));
+    else A = f0 / (T.st0 * cos(ph) + T.ct0 * sin(ph));
+    if (A < 0) { A *= -1; ph += M_PI; }
+    return make_double2(A * cos(ph), -A * sin(ph));
+}
+
+__global__ void k_phasor2(const double *__restrict__ f0, const double *__restrict__ f1,
+                          const double *__restrict__ f2, const double *__restrict__ f3,
+                          const double *__restrict__ f4, const double *__restrict__ f5,
+                          double2 *r0, double2 *r1, double2 *r2, double2 *r3, double2 *r4,
+                          double2 *r5, long long n, PhTimes2 TE, PhTimes2 TH)
+{
+    long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; g < n; g += (long long)gridDim.x * blockDim.x) {
+        r0[g] = phasor2(r0[g].x, f0[g], TE);
+        r1[g] = phasor2(r1[g].x, f1[g], TE);
+        r2[g] = phasor2(r2[g].x, f2[g], TE);
+        r3[g] = phasor2(r3[g].x, f3[g], TH);
+        r4[g] = phasor2(r4[g].x, f4[g], TH);
+        r5[g] = phasor2(r5[g].x, f5[g], TH);
+    }
+}
+
+// ------------------------------------------------------------- utilities
+
+__global__ void k_fill(double *a, long long n, double v)
+{
+    long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; g < n; g += (long long)gridDim.x * blockDim.x) a[g] = v;
+}
+
+// coefficient arrays from raw complex eps (real: b = dt/eps.re;
+// complex: a = (er - ei)/(er + ei), b = 1/(er + ei), er = eps.re/dt, ei = eps.im/2)
+__global__ void k_eps_coeffs(const double2 *__restrict__ eps, double *__restrict__ b,
+                             double *__restrict__ a, long long n, double dt, double odt, int ceps)
+{
+    long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; g < n; g += (long long)gridDim.x * blockDim.x) {
+        const double2 e = eps[g];
+        if (!ceps) {
+            b[g] = dt / e.x;
+        } else {
+            const double er = e.x * odt, ei = e.y * 0.5;
+            a[g] = (er - ei) / (er + ei);
+            b[g] = 1.0 / (er + ei);
+        }
+    }
+}
+
+// dt/mu.re and mu.re from raw complex mu
+__global__ void k_mu_coeffs(const double2 *__restrict__ mu, double *__restrict__ m,
+                            double *__restrict__ mr, long long n, double dt)
+{
+    long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; g < n; g += (long long)gridDim.x * blockDim.x) {
+        m[g] = dt / mu[g].x;
+        mr[g] = mu[g].x;
+    }
+}
+
+// gather a box of a complex [Nz][Ny][Nx] array into a dense box array
+__global__ void k_gather_box(const double2 *__restrict__ src, double2 *__restrict__ dst, Geo g,
+                             int k1, int nk, int j1, int nj, int i1, int ni)
+{
+    long long m = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    long long n = (long long)nk * nj * ni;
+    for (; m < n; m += (long long)gridDim.x * blockDim.x) {
+        int k = (int)(m % nk), j = (int)((m / nk) % nj), i = (int)(m / ((long long)nk * nj));
+        dst[m] = src[(long long)(i + i1) * g.nxy + (long long)(j + j1) * g.Nx + (k + k1)];
+    }
+}
+
+__global__ void k_scatter_box(double2 *__restrict__ dst, const double2 *__restrict__ src, Geo g,
+                              int k1, int nk, int j1, int nj, int i1, int ni)
+{
+    long long m = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    long long n = (long long)nk * nj * ni;
+    for (; m < n; m += (long long)gridDim.x * blockDim.x) {
+        int k = (int)(m % nk), j = (int)((m / nk) % nj), i = (int)(m / ((long long)nk * nj));
+        dst[(long long)(i + i1) * g.nxy + (long long)(j + j1) * g.Nx + (k + k1)] = src[m];
+    }
+}
+
+// ------------------------------------------------------ gradient assembly
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// adj * 1j * d * fwd evaluated left to right like numpy (emopt/fdtd.py:1710-1712)
+__device__ __forceinline__ double2 ydx_term(double2 adj, double2 d, double2 fwd)
+{
+    return cmul(cmul(cmul(adj, make_double2(0.0, 1.0)), d), fwd);
+}
+
+constexpr int RED_THREADS = 256;
+
+__device__ __forceinline__ double2 block_sum2(double2 v)
+{
+    __shared__ double2 sh[RED_THREADS];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[threadIdx.x].x += sh[threadIdx.x + w].x;
+            sh[threadIdx.x].y += sh[threadIdx.x + w].y;
+        }
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// partial sums of sum_c adj_c * 1j * (eps_c - eps0_c) * fwd_c over a box; one
+// partial per work-group (fixed grid => fixed summation order)
+__global__ __launch_bounds__(RED_THREADS) void k_ydAx(
+    const double2 *__restrict__ a0, const double2 *__restrict__ a1, const double2 *__restrict__ a2,
+    const double2 *__restrict__ f0, const double2 *__restrict__ f1, const double2 *__restrict__ f2,
+    const double2 *__restrict__ e0, const double2 *__restrict__ e1, const double2 *__restrict__ e2,
+    const double2 *__restrict__ z0, const double2 *__restrict__ z1, const double2 *__restrict__ z2,
+    Geo g, int k1, int nk, int j1, int nj, int i1, int ni, double2 *partial)
+{
+    long long n = (long long)nk * nj * ni;
+    double2 acc = make_double2(0.0, 0.0);
+    for (long long m = blockIdx.x * (long long)blockDim.x + threadIdx.x; m < n;
+         m += (long long)gridDim.x * blockDim.x) {
+        int k = (int)(m % nk), j = (int)((m / nk) % nj), i = (int)(m / ((long long)nk * nj));
+        long long c = (long long)(i + i1) * g.nxy + (long long)(j + j1) * g.Nx + (k + k1);
+        double2 d, t;
+        d = make_double2(e0[c].x - z0[m].x, e0[c].y - z0[m].y);
+        t = ydx_term(a0[c], d, f0[c]); acc.x += t.x; acc.y += t.y;
+        d = make_double2(e1[c].x - z1[m].x, e1[c].y - z1[m].y);
+        t = ydx_term(a1[c], d, f1[c]); acc.x += t.x; acc.y += t.y;
+        d = make_double2(e2[c].x - z2[m].x, e2[c].y - z2[m].y);
+        t = ydx_term(a2[c], d, f2[c]); acc.x += t.x; acc.y += t.y;
+    }
+    double2 tot = block_sum2(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// grad[p] = -2 Re(sum over the entries of parameter p of adj*1j*deps*fwd)/step;
+// entries are grouped by parameter, seg[p]..seg[p+1] (one work-group each)
+__global__ __launch_bounds__(RED_THREADS) void k_grad(
+    const double2 *__restrict__ a0, const double2 *__restrict__ a1, const double2 *__restrict__ a2,
+    const double2 *__restrict__ f0, const double2 *__restrict__ f1, const double2 *__restrict__ f2,
+    const long long *__restrict__ seg, const int *__restrict__ comp,
+    const long long *__restrict__ cell, const double2 *__restrict__ deps, double step,
+    double *grad)
+{
+    const int p = blockIdx.x;
+    double2 acc = make_double2(0.0, 0.0);
+    for (long long e = seg[p] + threadIdx.x; e < seg[p + 1]; e += blockDim.x) {
+        const int c = comp[e];
+        const long long g = cell[e];
+        const double2 adj = c == 0 ? a0[g] : (c == 1 ? a1[g] : a2[g]);
+        const double2 fwd = c == 0 ? f0[g] : (c == 1 ? f1[g] : f2[g]);
+        const double2 t = ydx_term(adj, deps[e], fwd);
+        acc.x += t.x; acc.y += t.y;
+    }
+    double2 tot = block_sum2(acc);
+    if (threadIdx.x == 0) grad[p] = -2 * (tot.x / step);
+}
+
+}  // namespace emf

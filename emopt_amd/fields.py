@@ -1,0 +1,128 @@
+"""Field post-processing on the host: interpolation of every component onto
+the Ez grid and the Poynting source power.
+
+Semantics of emopt/fdtd.py:1401-1553 (``get_field_interp``) and
+emopt/fdtd.py:1555-1667 (``get_source_power``).  The solver copies back only
+the (one-cell expanded) domain that is asked for, not the whole array.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .misc import DomainCoordinates
+
+# neighbour offsets (di, dj, dk) averaged onto the Ez grid, per component;
+# the stencils of emopt/fdtd.py:1481-1547 written as offset lists
+_STENCIL = {
+    "Ex": ((0, 0, 0), (0, 0, -1), (1, 0, 0), (1, 0, -1)),
+    "Ey": ((0, 0, 0), (0, -1, 0), (1, 0, 0), (1, -1, 0)),
+    "Hx": ((0, 0, 0), (0, -1, 0)),
+    "Hy": ((0, 0, 0), (0, 0, -1)),
+    "Hz": ((0, 0, 0), (0, 0, -1), (0, -1, 0), (0, -1, -1),
+           (1, 0, 0), (1, 0, -1), (1, -1, 0), (1, -1, -1)),
+}
+# which min faces get the 'E'/'H' mirror before averaging (x: axis 2, y: axis 1)
+_MIRROR = {"Ex": (2,), "Ey": (1,), "Hx": (1,), "Hy": (2,), "Hz": (2, 1)}
+# region of the padded array that is updated (others keep the raw value):
+# slices along (i, j, k) as (start, stop) offsets from (0, n)
+_REGION = {
+    "Ex": ((1, -1), (0, 0), (1, -1)),
+    "Ey": ((1, -1), (1, -1), (0, 0)),
+    "Hx": ((0, 0), (1, 0), (0, 0)),
+    "Hy": ((0, 0), (0, 0), (1, 0)),
+    "Hz": ((0, -1), (1, 0), (1, 0)),
+}
+
+
+def _avg(fr, comp):
+    """Average `fr` (padded by one zero cell on every side) over the stencil
+    of `comp`, inside the component's update region."""
+    (i0, i1), (j0, j1), (k0, k1) = _REGION[comp]
+    ni, nj, nk = fr.shape
+    si, sj, sk = slice(i0, ni + i1), slice(j0, nj + j1), slice(k0, nk + k1)
+    out = np.copy(fr)
+    part = out[si, sj, sk]   # neighbours are added one by one, in stencil order
+    for (di, dj, dk) in _STENCIL[comp][1:]:
+        part += fr[slice(si.start + di, si.stop + di), slice(sj.start + dj, sj.stop + dj),
+                   slice(sk.start + dk, sk.stop + dk)]
+    return out / float(len(_STENCIL[comp]))
+
+
+def interp_to_ez(sim, component, domain=None, squeeze=False, adjoint=False):
+    get = sim.get_adjoint_field if adjoint else sim.get_field
+    if component == "Ez":
+        f = get(component, domain)
+        return np.squeeze(f) if squeeze else f
+    Nx, Ny, Nz = sim.Nx, sim.Ny, sim.Nz
+    dx, dy, dz = sim.dx, sim.dy, sim.dz
+    if domain is None:
+        dint = DomainCoordinates(0, sim.X, 0, sim.Y, 0, sim.Z, dx, dy, dz)
+        domain = dint
+    else:
+        k1, k2, j1, j2, i1, i2 = domain.k1, domain.k2, domain.j1, domain.j2, domain.i1, domain.i2
+        if k1 > 0: k1 -= 1
+        if k2 < Nx - 1: k2 += 1
+        if j1 > 0: j1 -= 1
+        if j2 < Ny - 1: j2 += 1
+        if i1 > 0: i1 -= 1
+        if i2 < Nz - 1: i2 += 1
+        dint = DomainCoordinates(k1 * dx, k2 * dx, j1 * dy, j2 * dy, i1 * dz, i2 * dz, dx, dy, dz)
+    k1, k2, j1, j2, i1, i2 = dint.k1, dint.k2, dint.j1, dint.j2, dint.i1, dint.i2
+    fr = np.pad(get(component, dint), 1, "constant", constant_values=0)
+    bc = sim.bc
+    for ax in _MIRROR[component]:
+        lo = k1 if ax == 2 else j1
+        b = bc[0] if ax == 2 else bc[1]
+        if lo == 0 and b in "EH":
+            sgn = -1 if b == "E" else 1
+            if ax == 2:
+                fr[:, :, 0] = sgn * fr[:, :, 1]
+            else:
+                fr[:, 0, :] = sgn * fr[:, 1, :]
+    f = _avg(fr, component)
+    f = f[1 + domain.i1 - i1:-1 - (i2 - domain.i2),
+          1 + domain.j1 - j1:-1 - (j2 - domain.j2),
+          1 + domain.k1 - k1:-1 - (k2 - domain.k2)]
+    return np.squeeze(f) if squeeze else f
+
+
+def source_power(sim):
+    dx, dy, dz = sim.dx, sim.dy, sim.dz
+    w = sim.w_pml
+    X, Y, Z = sim.X, sim.Y, sim.Z
+    xmin = w[0] + dx if w[0] > 0 else 0.0
+    xmax = X - w[1] - dx if w[1] > 0 else X - dx
+    ymin = w[2] + dy if w[2] > 0 else 0.0
+    ymax = Y - w[3] - dy if w[3] > 0 else Y - dy
+    zmin = w[4] + dz if w[4] > 0 else 0.0
+    zmax = Z - w[5] - dz if w[5] > 0 else Z - dz
+    D = lambda *a: DomainCoordinates(*a, dx, dy, dz)
+    g = sim.get_field_interp
+    bc = sim.bc
+    re = np.real
+    P = 0.0
+
+    x1 = D(xmin, xmin, ymin, ymax, zmin, zmax)
+    Ey, Ez, Hy, Hz = g("Ey", x1), g("Ez", x1), g("Hy", x1), g("Hz", x1)
+    if bc[0] not in "EH":
+        P += -0.5 * dy * dz * np.sum(re(Ey * np.conj(Hz) - Ez * np.conj(Hy)))
+    x2 = D(xmax, xmax, ymin, ymax, zmin, zmax)
+    Ey, Ez, Hy, Hz = g("Ey", x2), g("Ez", x2), g("Hy", x2), g("Hz", x2)
+    P += 0.5 * dy * dz * np.sum(re(Ey * np.conj(Hz) - Ez * np.conj(Hy)))
+
+    y1 = D(xmin, xmax, ymin, ymin, zmin, zmax)
+    Ex, Ez, Hx, Hz = g("Ex", y1), g("Ez", y1), g("Hx", y1), g("Hz", y1)
+    if bc[1] not in "EH":
+        P += 0.5 * dx * dz * np.sum(re(Ex * np.conj(Hz) - Ez * np.conj(Hx)))
+    y2 = D(xmin, xmax, ymax, ymax, zmin, zmax)
+    Ex, Ez, Hx, Hz = g("Ex", y2), g("Ez", y2), g("Hx", y2), g("Hz", y2)
+    P += -0.5 * dx * dz * np.sum(re(Ex * np.conj(Hz) - Ez * np.conj(Hx)))
+
+    z1 = D(xmin, xmax, ymin, ymax, zmin, zmin)
+    Ex, Ey, Hx, Hy = g("Ex", z1), g("Ey", z1), g("Hx", z1), g("Hy", z1)
+    if bc[2] not in "EH":
+        P += -0.5 * dx * dy * np.sum(re(Ex * np.conj(Hy) - Ey * np.conj(Hx)))
+    z2 = D(xmin, xmax, ymin, ymax, zmax, zmax)
+    Ex, Ey, Hx, Hy = g("Ex", z2), g("Ey", z2), g("Hx", z2), g("Hy", z2)
+    P += 0.5 * dx * dy * np.sum(re(Ex * np.conj(Hy) - Ey * np.conj(Hx)))
+    return P

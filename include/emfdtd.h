@@ -1,0 +1,252 @@
+/*
+ * emfdtd — C ABI of the MI355X (gfx950) CW-FDTD library `libemfdtd.so`.
+ *
+ * This header is the drop-in boundary that replaces EMopt's FDTD.so ABI
+ * (src/fdtd.hpp:571-643, bound by emopt/fdtd_ctypes.py:26-149).  Every entry
+ * point below names the reference function it replaces.  Differences that
+ * follow from device residency:
+ *
+ *  - Arrays are COPIED into device-owned buffers instead of being borrowed by
+ *    pointer (the reference keeps numpy/PETSc pointers, src/fdtd.cpp:107-119).
+ *    Field state never leaves HBM during a solve.
+ *  - Every call returns an int status (EMF_OK = 0, negative on error) instead
+ *    of undefined behaviour on bad input; emf_last_error() gives the message.
+ *  - Complex arrays are numpy complex128 ({re, im} interleaved doubles),
+ *    C-contiguous, shape [Nz][Ny][Nx] (x fastest), exactly the reference's
+ *    unpadded "global" layout (src/fdtd.cpp:183).
+ *  - The grid is never padded with ghost layers: boundary conditions are
+ *    applied inside the kernels.
+ *  - One handle = one solver on one GPU; calls on one handle are serialised
+ *    by a per-handle lock.  Different handles may be used from different
+ *    threads.
+ *
+ * Component order everywhere: 0..5 = Ex, Ey, Ez, Hx, Hy, Hz (and Jx..Mz).
+ * Result slots: 0 = forward phasors, 1 = adjoint phasors.
+ */
+#ifndef EMFDTD_H
+#define EMFDTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Symbols are exported explicitly; the library builds with -fvisibility=hidden. */
+#if defined(__GNUC__)
+#define EMF_API __attribute__((visibility("default")))
+#else
+#define EMF_API
+#endif
+
+#define EMF_OK 0
+#define EMF_ERR_ARG (-1)    /* invalid argument or call order */
+#define EMF_ERR_HIP (-2)    /* a HIP runtime call failed */
+#define EMF_ERR_NOMEM (-3)  /* device allocation failed */
+#define EMF_ERR_STATE (-4)  /* handle not ready (dims/materials/pml missing) */
+
+typedef struct emf_solver emf_solver;
+
+/* Outcome of one emf_solve call (the loop of emopt/fdtd.py:1049-1194). */
+typedef struct {
+    int64_t n0;         /* step index at which the t0 snapshot was taken */
+    int64_t Tn;         /* steps between snapshots, int(0.75 * Ncycle) */
+    int64_t steps;      /* total time steps executed (n0 + 2*Tn) */
+    int32_t status;     /* 0 converged, 1 hit Nmax, 2 fixed step count */
+    int32_t nchecks;    /* convergence checks performed */
+    double a_change;    /* last relative amplitude change */
+    double phi_change;  /* last relative phase change */
+    double seconds;     /* wall time of the stepping on the device */
+} emf_solve_info;
+
+/* Message of the last failing call on this thread ("" if none). */
+EMF_API const char *emf_last_error(void);
+
+/* Library build string (target, compile flags). */
+EMF_API const char *emf_version(void);
+
+/* Number of visible HIP devices, or a negative status. */
+EMF_API int emf_device_count(void);
+
+/* ---- lifecycle -------------------------------------------------------- */
+
+/* Replaces FDTD_new (src/fdtd.hpp:572).  Binds the handle to `device`. */
+EMF_API int emf_new(int device, emf_solver **out);
+
+/* No counterpart: the reference never frees its object.  Releases every
+ * device buffer of the handle. */
+EMF_API int emf_delete(emf_solver *s);
+
+/* ---- geometry and time ------------------------------------------------ */
+
+/* Replaces FDTD_set_wavelength (src/fdtd.hpp:575): R = wavelength / 2pi. */
+EMF_API int emf_set_wavelength(emf_solver *s, double wavelength);
+
+/* Replaces FDTD_set_physical_dims (src/fdtd.hpp:576-578). */
+EMF_API int emf_set_physical_dims(emf_solver *s, double X, double Y, double Z,
+                          double dx, double dy, double dz);
+
+/* Replaces FDTD_set_grid_dims (src/fdtd.hpp:579).  (Re)allocates the
+ * device field state for Nx*Ny*Nz cells. */
+EMF_API int emf_set_grid_dims(emf_solver *s, int Nx, int Ny, int Nz);
+
+/* Replaces FDTD_set_local_grid (src/fdtd.hpp:580-582).  One GPU owns the
+ * whole grid, so the chunk must be (0,0,0,Nx,Ny,Nz). */
+EMF_API int emf_set_local_grid(emf_solver *s, int k0, int j0, int i0, int K, int J, int I);
+
+/* Replaces FDTD_set_dt (src/fdtd.hpp:583). */
+EMF_API int emf_set_dt(emf_solver *s, double dt);
+
+/* Replaces FDTD_set_complex_eps (src/fdtd.hpp:584). */
+EMF_API int emf_set_complex_eps(emf_solver *s, int complex_eps);
+
+/* Replaces FDTD_set_mat_arrays (src/fdtd.hpp:589-591).  Copies six complex
+ * [Nz][Ny][Nx] arrays (eps_x, eps_y, eps_z, mu_x, mu_y, mu_z) to the device;
+ * only the real parts of mu are used, as in the reference. */
+EMF_API int emf_set_mat_arrays(emf_solver *s, const double *eps_x, const double *eps_y,
+                       const double *eps_z, const double *mu_x, const double *mu_y,
+                       const double *mu_z);
+
+/* Overwrites eps_c (c = 0,1,2) on the sub-box [k1,k2)x[j1,j2)x[i1,i2) with a
+ * complex array of that box's shape.  Used by FDTD.update(bbox)
+ * (emopt/fdtd.py:882-992). */
+EMF_API int emf_update_eps_box(emf_solver *s, int c, const double *vals, int k1, int k2,
+                       int j1, int j2, int i1, int i2);
+
+/* Copies the device materials back (six complex arrays, any may be NULL).
+ * Supports get_A_diag (emopt/fdtd.py:1669-1692). */
+EMF_API int emf_get_mat_arrays(emf_solver *s, double *eps_x, double *eps_y, double *eps_z,
+                       double *mu_x, double *mu_y, double *mu_z);
+
+/* ---- boundary conditions and CPML -------------------------------------- */
+
+/* Replaces FDTD_set_bc (src/fdtd.hpp:637): three chars from "0EH". */
+EMF_API int emf_set_bc(emf_solver *s, const char *bc);
+
+/* Replaces FDTD_set_pml_widths (src/fdtd.hpp:597-599). */
+EMF_API int emf_set_pml_widths(emf_solver *s, int xmin, int xmax, int ymin, int ymax,
+                       int zmin, int zmax);
+
+/* Replaces FDTD_set_pml_properties (src/fdtd.hpp:600-601). */
+EMF_API int emf_set_pml_properties(emf_solver *s, double sigma, double alpha, double kappa,
+                           double pow);
+
+/* Replaces FDTD_build_pml (src/fdtd.hpp:602): allocates the psi slabs. */
+EMF_API int emf_build_pml(emf_solver *s);
+
+/* Replaces FDTD_reset_pml (src/fdtd.hpp:603): zeroes the psi slabs. */
+EMF_API int emf_reset_pml(emf_solver *s);
+
+/* ---- sources ---------------------------------------------------------- */
+
+/* Replaces FDTD_add_source (src/fdtd.hpp:625-629).  Six complex arrays of
+ * shape [I][J][K] placed at (i0,j0,k0).  With calc_phase != 0 each value is
+ * converted to (|v|, -atan2(im, re)) as the reference does in place
+ * (src/fdtd.cpp:1517-1576); the caller's arrays are never modified.  With
+ * calc_phase == 0 the arrays already hold (amplitude, phase). */
+EMF_API int emf_add_source(emf_solver *s, const double *Jx, const double *Jy, const double *Jz,
+                   const double *Mx, const double *My, const double *Mz,
+                   int i0, int j0, int k0, int I, int J, int K, int calc_phase);
+
+/* Replaces FDTD_clear_sources (src/fdtd.hpp:630). */
+EMF_API int emf_clear_sources(emf_solver *s);
+
+/* Replaces FDTD_set_source_properties (src/fdtd.hpp:632). */
+EMF_API int emf_set_source_properties(emf_solver *s, double src_T, double src_min);
+
+/* Replaces FDTD_src_func_t (src/fdtd.hpp:634). Host evaluation. */
+EMF_API double emf_src_func_t(emf_solver *s, int n, double t, double phase);
+
+/* ---- stepping --------------------------------------------------------- */
+
+/* Replaces FDTD_update_H / FDTD_update_E (src/fdtd.hpp:593-594): one half
+ * step on the device (asynchronous on the handle's stream). */
+EMF_API int emf_update_H(emf_solver *s, int n, double t);
+EMF_API int emf_update_E(emf_solver *s, int n, double t);
+
+/* Zeroes the six time-domain field arrays (the reference's numpy fill(0),
+ * emopt/fdtd.py:1017-1018). */
+EMF_API int emf_zero_fields(emf_solver *s);
+
+/* Copies the six real time-domain fields ([Nz][Ny][Nx] doubles each, any may
+ * be NULL) to / from the host.  For step-level parity checks. */
+EMF_API int emf_get_time_fields(emf_solver *s, double *Ex, double *Ey, double *Ez,
+                        double *Hx, double *Hy, double *Hz);
+EMF_API int emf_set_time_fields(emf_solver *s, const double *Ex, const double *Ey,
+                        const double *Ez, const double *Hx, const double *Hy,
+                        const double *Hz);
+
+/* ---- snapshots and phasors -------------------------------------------- */
+
+/* Replace FDTD_capture_t0_fields / FDTD_capture_t1_fields (src/fdtd.hpp:619-620)
+ * for result slot `slot`. */
+EMF_API int emf_capture_t0_fields(emf_solver *s, int slot);
+EMF_API int emf_capture_t1_fields(emf_solver *s, int slot);
+
+/* Replace FDTD_calc_complex_fields_2T / _3T (src/fdtd.hpp:621-622): recovers
+ * the complex phasors of all six components into result slot `slot`. */
+EMF_API int emf_calc_complex_fields_2T(emf_solver *s, int slot, double t0, double t1);
+EMF_API int emf_calc_complex_fields_3T(emf_solver *s, int slot, double t0, double t1, double t2);
+
+/* Replace FDTD_calc_phase_2T/3T and FDTD_calc_amplitude_2T/3T
+ * (src/fdtd.hpp:613-617): scalar closed forms evaluated on the host. */
+EMF_API double emf_calc_phase_2T(double t0, double t1, double f0, double f1);
+EMF_API double emf_calc_amplitude_2T(double t0, double t1, double f0, double f1, double phase);
+EMF_API double emf_calc_phase_3T(double t0, double t1, double t2, double f0, double f1, double f2);
+EMF_API double emf_calc_amplitude_3T(double t0, double t1, double t2, double f0, double f1,
+                             double f2, double phase);
+
+/* ---- whole solve ------------------------------------------------------- */
+
+/* Number of convergence sample points (nconv, emopt/fdtd.py:240-241,411-413). */
+EMF_API int emf_set_nconv(emf_solver *s, int nconv);
+
+/* Runs the complete CW solve of emopt/fdtd.py:994-1194 on the device:
+ * reset CPML and fields, step with a convergence check every Tn steps until
+ * the amplitude change <= rtol and the phase change <= sqrt(rtol) (or the step
+ * index exceeds nmax), capture t0, Tn steps, capture t1, Tn steps, and recover
+ * the phasors into `slot`.  With fixed_n0 >= 0 the convergence loop is
+ * replaced by exactly fixed_n0 steps.  Tn = int(0.75 * Ncycle) is computed by
+ * the caller (emopt/fdtd.py:1021).  Sources currently registered are used.
+ * Replaces the per-step Python loop plus FDTD_update_H/E, the 6 ghost
+ * exchanges per step and the 6*nconv ctypes calls per check. */
+EMF_API int emf_solve(emf_solver *s, int slot, double rtol, double nmax, int64_t Tn,
+              int64_t fixed_n0, emf_solve_info *info);
+
+/* Convergence history of the last emf_solve: up to `cap` records of
+ * (n, a_change, phi_change).  Returns the number of records available. */
+EMF_API int emf_get_history(emf_solver *s, double *out, int cap);
+
+/* Copies the complex phasor of component `comp` in result slot `slot` on the
+ * box [k1,k2)x[j1,j2)x[i1,i2) into `out` (complex [i2-i1][j2-j1][k2-k1]).
+ * Replaces __get_field's gather (emopt/fdtd.py:1322-1359). */
+EMF_API int emf_get_field(emf_solver *s, int slot, int comp, int k1, int k2, int j1, int j2,
+                  int i1, int i2, double *out);
+
+/* ---- gradient assembly ------------------------------------------------- */
+
+/* y^T dA x of emopt/fdtd.py:1694-1717 on the device:
+ *   sum_c sum_cells E_adj_c * 1j * (eps_c - eps0_c) * E_c
+ * over the box [k1,k2)x[j1,j2)x[i1,i2), where eps0 is given as three complex
+ * arrays of the box's shape.  Result written to out[0] (re), out[1] (im).
+ * The mu term of the reference is omitted: mu is not a design variable in any
+ * supported workload, so mu - mu0 = 0. */
+EMF_API int emf_calc_ydAx(emf_solver *s, const double *eps0_x, const double *eps0_y,
+                  const double *eps0_z, int k1, int k2, int j1, int j2, int i1, int i2,
+                  double *out);
+
+/* Hand-written gradient assembly for many design parameters at once:
+ *   grad[p] = -2 Re( sum_c sum_cells E_adj_c * 1j * deps[p,c,cell] * E_c ) / step
+ * where deps holds, per parameter, sparse (cell, component, d_eps) triples
+ * (d_eps complex).  This is the reference's per-parameter loop
+ * (emopt/adjoint_method.py:504-533) collapsed into one device pass.
+ * entries: n triples; cell index g = (i*Ny + j)*Nx + k; param ids in pidx. */
+EMF_API int emf_grad_assemble(emf_solver *s, int64_t n, const int32_t *pidx,
+                      const int32_t *comp, const int64_t *cell, const double *deps,
+                      int nparam, double step, double *grad);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMFDTD_H */

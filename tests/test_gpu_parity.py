@@ -1,0 +1,139 @@
+"""GPU parity: the HIP path (through libemfdtd.so) against the CPU oracle.
+
+Bar (from the workload definition): relative L2 error <= 1e-6 per recovered
+complex component at an identical step count; in practice the two agree to
+~1e-12, the residue of FMA contraction and libm last-bit differences.
+"""
+import numpy as np
+import pytest
+
+from tests.cases import random_case, rel_l2, rel_l2_set, run_gpu, run_oracle, strip_case
+
+TOL = 1e-6          # per-component relative L2, the workload's bar
+STEP_TOL = 1e-11    # time-domain fields after a few steps
+
+pytestmark = pytest.mark.gpu
+
+
+def _step_gpu(case, nsteps):
+    """Run nsteps raw H/E half steps through the ABI; return Ex..Hz."""
+    import warnings
+    from emopt_amd.fdtd import FDTD
+    from emopt_amd.misc import DomainCoordinates
+    from tests.cases import _configure
+    d = case["d"]
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim = FDTD(case["X"], case["Y"], case["Z"], d, d, d, case["wl"],
+                   complex_eps=case.get("ceps", False))
+        _configure(sim, case, DomainCoordinates)
+    sim.set_material_arrays(case["mats"][:3], case["mats"][3:])
+    sim._register(sim._sources)
+    L, h = sim._lib, sim._h
+    L.emf_reset_pml(h)
+    L.emf_zero_fields(h)
+    dt = sim.dt
+    for n in range(nsteps):
+        L.emf_update_H(h, n, n * dt)
+        L.emf_update_E(h, n, (n + 0.5) * dt)
+    return sim.time_fields()
+
+
+def _step_oracle(case, nsteps):
+    import warnings
+    from oracle import driver as D
+    from tests.cases import _configure
+    d = case["d"]
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        o = D.OracleFDTD(D.load(D.ORACLE_LIB), case["X"], case["Y"], case["Z"], d, d, d,
+                         case["wl"], complex_eps=case.get("ceps", False))
+        _configure(o, case, D.Domain)
+    o.set_material_arrays(case["mats"][:3], case["mats"][3:])
+    o._bind(o.fwd)
+    o.lib.FDTD_clear_sources(o.h)
+    for arrs, i0, j0, k0, sI, sJ, sK in o.sources:
+        o.lib.FDTD_add_source(o.h, *arrs, i0, j0, k0, sI, sJ, sK, False)
+    o.lib.FDTD_reset_pml(o.h)
+    for n in range(nsteps):
+        o._step(n)
+    return [o.padded(q)[1:-1, 1:-1, 1:-1] for q in range(6)]
+
+
+@pytest.mark.parametrize("bc", ["000", "0H0", "E00", "HE0", "EEH"])
+@pytest.mark.parametrize("ceps", [False, True])
+def test_step_level(bc, ceps):
+    case = random_case(13, 11, 9, seed=sum(map(ord, bc)) + int(ceps), ceps=ceps, bc=bc, nsrc=2)
+    g = _step_gpu(case, 20)
+    o = _step_oracle(case, 20)
+    errs = rel_l2_set(g, o)
+    assert max(errs) < STEP_TOL, (bc, ceps, errs)
+
+
+def test_step_level_many_sources_and_no_pml():
+    # more source blocks than the kernel carries inline, and zero CPML widths
+    case = random_case(12, 10, 8, w=(0, 0, 0, 0, 0, 0), seed=7, nsrc=6)
+    g = _step_gpu(case, 15)
+    o = _step_oracle(case, 15)
+    errs = rel_l2_set(g, o)
+    assert max(errs) < STEP_TOL, errs
+
+
+def test_step_level_wide_pml_uniform_mu():
+    case = random_case(40, 24, 20, w=(6, 5, 4, 7, 5, 6), seed=3, mu_rand=False, nsrc=1)
+    g = _step_gpu(case, 40)
+    o = _step_oracle(case, 40)
+    errs = rel_l2_set(g, o)
+    assert max(errs) < STEP_TOL, errs
+
+
+def test_strip_solve_converged_matches_oracle():
+    case = strip_case(24, 20, 16, w=4)
+    g = run_gpu(case)
+    o = run_oracle(case)
+    assert g["status"] == "converged"
+    assert g["n0"] == o["n0"], (g["n0"], o["n0"])
+    np.testing.assert_allclose(g["history"][:, 0], o["history"][:, 0])
+    np.testing.assert_allclose(g["history"][:, 1:], o["history"][:, 1:], rtol=1e-8, atol=1e-300)
+    errs = [rel_l2(a, b) for a, b in zip(g["fields"], o["fields"])]
+    assert max(errs) <= TOL, errs
+    print("rel L2 Ex..Hz:", " ".join("%.2e" % e for e in errs))
+
+
+def test_strip_solve_fixed_n0_bc_symmetry():
+    case = strip_case(30, 22, 18, w=4, bc="0H0")
+    g = run_gpu(case, fixed_n0=600)
+    o = run_oracle(case, fixed_n0=600)
+    errs = [rel_l2(a, b) for a, b in zip(g["fields"], o["fields"])]
+    assert max(errs) <= TOL, errs
+
+
+def _golden(name):
+    import os
+    p = os.path.join(os.path.dirname(__file__), "golden", name)
+    return np.load(p, allow_pickle=False)
+
+
+def test_gpu_matches_reference_golden_steps():
+    """Time-domain fields against EMopt's own kernel (fixtures written by the
+    compiled src/fdtd.cpp)."""
+    g = _golden("steps.npz")
+    for key in sorted({k.rsplit("_", 1)[0] for k in g.files if k.endswith("_meta")}):
+        meta = g[key + "_meta"]
+        Nx, Ny, Nz, seed, ceps, nsrc, w0, w1, w2, w3, w4, w5 = [int(v) for v in meta[:12]]
+        bc = "".join(chr(int(c)) for c in meta[12:15])
+        case = random_case(Nx, Ny, Nz, w=(w0, w1, w2, w3, w4, w5), seed=seed, ceps=bool(ceps),
+                           bc=bc, nsrc=nsrc)
+        for n_target, fld in zip(g[key + "_steps"], g[key + "_fields"]):
+            errs = rel_l2_set(_step_gpu(case, int(n_target)), fld)
+            assert max(errs) < STEP_TOL, (key, int(n_target), errs)
+
+
+def test_gpu_matches_reference_golden_solve():
+    g = _golden("solve_strip_24x20x16.npz")
+    r = run_gpu(strip_case(24, 20, 16, w=4))
+    assert r["n0"] == int(g["n0"]) and r["Tn"] == int(g["Tn"])
+    np.testing.assert_allclose(r["history"], g["history"], rtol=1e-8)
+    errs = [rel_l2(a, b) for a, b in zip(r["fields"], g["fields"])]
+    print("vs reference: rel L2 Ex..Hz:", " ".join("%.2e" % e for e in errs))
+    assert max(errs) <= TOL, errs
