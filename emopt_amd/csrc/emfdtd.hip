@@ -1211,4 +1211,80 @@ int emf_grad_assemble(emf_solver *s, int64_t n, const int32_t *pidx, const int32
     return rc;
 }
 
+int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check_every,
+                   double *out)
+{
+    LOCK(s);
+    if (!out || nsteps < 0 || n_start < 0) EMF_FAIL(EMF_ERR_ARG, "bad arguments");
+    CHECK(check_ready(s));
+    if (check_every > 0) CHECK(build_samples(s));
+    std::vector<hipEvent_t> ev(2 * nsteps + 1);
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipMemsetAsync(s->conv, 0, sizeof(ConvState), s->stream));
+    HIPCHK(hipEventRecord(ev[0], s->stream));
+    int rc = EMF_OK;
+    for (int64_t q = 0; q < nsteps && rc == EMF_OK; ++q) {
+        long long n = n_start + q;
+        rc = launch_h(s, (double)n * s->dt, nullptr);
+        if (rc == EMF_OK) rc = hipEventRecord(ev[2 * q + 1], s->stream) == hipSuccess ? EMF_OK : EMF_ERR_HIP;
+        if (rc == EMF_OK) rc = launch_e(s, ((double)n + 0.5) * s->dt, nullptr);
+        if (rc == EMF_OK && check_every > 0 && (q + 1) % check_every == 0) {
+            hipLaunchKernelGGL(k_conv_check, dim3(1), dim3(64), 0, s->stream, s->E[0], s->E[1],
+                               s->E[2], s->samples, s->nsamples, (double)n * s->dt, 1e-300, 1e-300,
+                               n, s->conv, (double *)nullptr, 0, (int *)nullptr);
+        }
+        if (rc == EMF_OK) rc = hipEventRecord(ev[2 * q + 2], s->stream) == hipSuccess ? EMF_OK : EMF_ERR_HIP;
+    }
+    if (rc == EMF_OK && hipEventSynchronize(ev[2 * nsteps]) != hipSuccess) rc = EMF_ERR_HIP;
+    double tot = 0, th = 0, te = 0;
+    if (rc == EMF_OK) {
+        float ms;
+        hipEventElapsedTime(&ms, ev[0], ev[2 * nsteps]);
+        tot = ms;
+        for (int64_t q = 0; q < nsteps; ++q) {
+            hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]); th += ms;
+            hipEventElapsedTime(&ms, ev[2 * q + 1], ev[2 * q + 2]); te += ms;
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (rc != EMF_OK) EMF_FAIL(rc, "timed stepping failed");
+    out[0] = tot; out[1] = th; out[2] = te;
+    return EMF_OK;
+}
+
+int emf_copy_bandwidth(int device, int64_t bytes, int reps, double *gbps)
+{
+    if (!gbps || bytes < 1024 || reps < 1) EMF_FAIL(EMF_ERR_ARG, "bad arguments");
+    HIPCHK(hipSetDevice(device));
+    long long n = bytes / (long long)sizeof(double2);
+    double2 *a = nullptr, *b = nullptr;
+    CHECK(dalloc(&a, n));
+    if (dalloc(&b, n) != EMF_OK) { dfree(a); return EMF_ERR_NOMEM; }
+    hipStream_t st;
+    hipEvent_t e0, e1;
+    int rc = EMF_OK;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) { dfree(a); dfree(b); EMF_FAIL(EMF_ERR_HIP, "stream"); }
+    hipEventCreate(&e0); hipEventCreate(&e1);
+    hipMemsetAsync(a, 0, n * sizeof(double2), st);
+    std::vector<float> t;
+    unsigned grid = (unsigned)std::min<long long>((n + 255) / 256, 256LL * 64);
+    for (int r = 0; r < reps + 1 && rc == EMF_OK; ++r) {
+        hipEventRecord(e0, st);
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, st, a, b, n);
+        hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = EMF_ERR_HIP;
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (r > 0) t.push_back(ms);   // first launch is a warm-up
+    }
+    hipEventDestroy(e0); hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    dfree(a); dfree(b);
+    if (rc != EMF_OK) EMF_FAIL(rc, "copy kernel failed");
+    std::sort(t.begin(), t.end());
+    double ms = t[t.size() / 2];
+    *gbps = 2.0 * n * sizeof(double2) / (ms * 1e-3) / 1e9;
+    return EMF_OK;
+}
+
 }  // extern "C"

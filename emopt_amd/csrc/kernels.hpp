@@ -595,6 +595,13 @@ __global__ void k_scatter_box(double2 *__restrict__ dst, const double2 *__restri
     }
 }
 
+// plain vector copy (bandwidth reference)
+__global__ void k_copy(const double2 *__restrict__ a, double2 *__restrict__ b, long long n)
+{
+    long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; g < n; g += (long long)gridDim.x * blockDim.x) b[g] = a[g];
+}
+
 // ------------------------------------------------------ gradient assembly
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)

@@ -143,21 +143,30 @@ class _Sampled(MaterialPrimitive):
 
 
 class Ellipse(_Sampled):
-    """Ellipse centred at (x0, y0) with semi-axes (a, b), rotated by angle."""
+    """Ellipse centred at (x0, y0) with semi-axes (a, b), rotated by angle,
+    optionally clipped to the box clip = (xmin, xmax, ymin, ymax)."""
 
-    def __init__(self, x0, y0, a, b, angle=0.0, material_value=1.0, layer=1):
+    def __init__(self, x0, y0, a, b, angle=0.0, material_value=1.0, layer=1, clip=None):
         super().__init__(material_value, layer)
         self.x0, self.y0, self.a, self.b, self.angle = x0, y0, a, b, angle
+        self.clip = clip
 
     def bbox(self):
         r = max(self.a, self.b)
-        return self.x0 - r, self.x0 + r, self.y0 - r, self.y0 + r
+        x0, x1, y0, y1 = self.x0 - r, self.x0 + r, self.y0 - r, self.y0 + r
+        if self.clip is not None:
+            x0, x1 = max(x0, self.clip[0]), min(x1, self.clip[1])
+            y0, y1 = max(y0, self.clip[2]), min(y1, self.clip[3])
+        return x0, x1, y0, y1
 
     def inside(self, x, y):
         c, s = np.cos(self.angle), np.sin(self.angle)
         u = (x - self.x0) * c + (y - self.y0) * s
         v = -(x - self.x0) * s + (y - self.y0) * c
-        return (u / self.a) ** 2 + (v / self.b) ** 2 <= 1.0
+        ins = (u / self.a) ** 2 + (v / self.b) ** 2 <= 1.0
+        if self.clip is not None:
+            ins &= (x >= self.clip[0]) & (x < self.clip[1]) & (y >= self.clip[2]) & (y < self.clip[3])
+        return ins
 
 
 class Polygon(_Sampled):

@@ -223,6 +223,21 @@ EMF_API int emf_get_history(emf_solver *s, double *out, int cap);
 EMF_API int emf_get_field(emf_solver *s, int slot, int comp, int k1, int k2, int j1, int j2,
                   int i1, int i2, double *out);
 
+/* ---- measurement (no counterpart in the reference) --------------------- */
+
+/* Runs `nsteps` time steps starting at step index n_start on the current
+ * state (sources, CPML and fields as they are), with a convergence check
+ * every `check_every` steps (0 = none), exactly as emf_solve's loop does, and
+ * times them with HIP events on the handle's stream.  out[0] = total device
+ * ms of the region, out[1] = summed ms of the H-pass launches, out[2] =
+ * summed ms of the E-pass launches. */
+EMF_API int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check_every,
+                           double *out);
+
+/* Device copy bandwidth (GB/s, read + write bytes) of a plain vector copy of
+ * `bytes` bytes, median of `reps` launches; a roofline reference point. */
+EMF_API int emf_copy_bandwidth(int device, int64_t bytes, int reps, double *gbps);
+
 /* ---- gradient assembly ------------------------------------------------- */
 
 /* y^T dA x of emopt/fdtd.py:1694-1717 on the device:

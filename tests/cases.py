@@ -77,6 +77,46 @@ def strip_case(Nx, Ny, Nz, w=10, d=0.04, wl=1.55, core=(0.5, 0.22), rtol=1e-4,
                 ramp=ramp, bc=bc, mats=mats, sources=[(src, (ks, ks + 1, 0, Ny, iz, iz + 1))])
 
 
+def ellipse_design_eps(Nx, Ny, Nz, d, design=(240, 120, 10), n_ell=40, seed=2,
+                       axes=(0.1, 0.6)):
+    """Cladding (1.444^2) everywhere; a design slab of design[0] x design[1]
+    x design[2] cells centred in the grid filled with n_ell seeded ellipses
+    (semi-axes uniform in `axes` um, eps 3.48^2 inside), boundary cells
+    area-averaged."""
+    from emopt_amd.grid import StructuredMaterial3D, Rectangle, Ellipse
+    X, Y, Z = span(Nx, d), span(Ny, d), span(Nz, d)
+    rng = np.random.default_rng(seed)
+    eps = StructuredMaterial3D(X, Y, Z, d, d, d)
+    eps.add_primitive(Rectangle(X / 2, Y / 2, 2 * X, 2 * Y, material_value=EPS_OX, layer=10), -Z, 2 * Z)
+    dxw, dyw, dzw = design[0] * d, design[1] * d, design[2] * d
+    box = (X / 2 - dxw / 2, X / 2 + dxw / 2, Y / 2 - dyw / 2, Y / 2 + dyw / 2)
+    z0 = Z / 2 - dzw / 2
+    for _ in range(n_ell):
+        cx = rng.uniform(box[0], box[1]); cy = rng.uniform(box[2], box[3])
+        a, b = rng.uniform(*axes, size=2)
+        ang = rng.uniform(0, np.pi)
+        eps.add_primitive(Ellipse(cx, cy, a, b, ang, material_value=EPS_SI, layer=1, clip=box),
+                          z0, z0 + dzw)
+    return eps
+
+
+def large_case(Nx=400, Ny=200, Nz=120, d=0.03, w=12, wl=1.31, seed=2, rtol=1e-4):
+    """Single large FP64 solve: 400x200x120 at dx=0.03 um, 12-cell CPML,
+    240x120x10 design slab of 40 seeded ellipses, lambda 1.31 um, planar Ey
+    source across the input face (x index w + 2, interior y-z)."""
+    from emopt_amd.grid import ConstantMaterial3D
+    X, Y, Z = span(Nx, d), span(Ny, d), span(Nz, d)
+    eps = ellipse_design_eps(Nx, Ny, Nz, d, seed=seed)
+    mu = ConstantMaterial3D(1.0)
+    mats = [sample(eps if q < 3 else mu, Nx, Ny, Nz, q) for q in range(6)]
+    ks = w + 2
+    J0, J1, I0, I1 = w, Ny - w, w, Nz - w
+    src = [np.zeros((I1 - I0, J1 - J0, 1), dtype=np.complex128) for _ in range(6)]
+    src[1][:] = 1.0
+    return dict(Nx=Nx, Ny=Ny, Nz=Nz, d=d, wl=wl, X=X, Y=Y, Z=Z, w=[w] * 6, rtol=rtol,
+                ramp=None, bc="000", mats=mats, sources=[(src, (ks, ks + 1, J0, J1, I0, I1))])
+
+
 def random_case(Nx, Ny, Nz, w=(3, 3, 4, 3, 3, 4), d=0.05, wl=1.0, seed=0, ceps=False,
                 mu_rand=True, bc="000", nsrc=1, ramp=5.0):
     """Random materials and random complex J/M blocks (step-level fixtures)."""
@@ -102,7 +142,8 @@ def _configure(sim, case, dom_cls):
     d = case["d"]
     sim.w_pml = [wc * d + 1e-9 * d for wc in case["w"]]
     sim.bc = case["bc"]
-    sim.src_ramp_time = case["ramp"]
+    if case.get("ramp") is not None:
+        sim.src_ramp_time = case["ramp"]
     for arrs, (k1, k2, j1, j2, i1, i2) in case["sources"]:
         sim.set_sources(arrs, dom_cls.from_indices(k1, k2, j1, j2, i1, i2, *([d] * 3))
                         if dom_cls.__name__ == "DomainCoordinates" else
@@ -126,7 +167,8 @@ def run_gpu(case, fixed_n0=None, device=0):
         warnings.simplefilter("ignore")
         sim._register(sim._sources)
         sim._solve(0)
-    fields = [sim.get_field(c) for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")]
+    full = DomainCoordinates.from_indices(0, case["Nx"], 0, case["Ny"], 0, case["Nz"], d, d, d)
+    fields = [sim.get_field(c, full) for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")]
     out = dict(sim.last_solve)
     out["fields"] = fields
     out["history"] = sim.convergence_history
