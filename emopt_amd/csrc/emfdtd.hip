@@ -19,7 +19,7 @@
 #include <vector>
 
 #include "../../include/emfdtd.h"
-#include "kernels.hpp"
+#include "twopass.hpp"
 
 using namespace emf;
 
@@ -144,10 +144,18 @@ struct emf_solver {
     double sigma = 3.0, alpha = 0.0, kappa = 1.0, pw = 3.0;
     int bc[3] = { '0', '0', '0' };
     int nconv = 10;
-    int zc = 8;
+    int zc = 2;
 
-    // device state
+    // device state.  E/H (and the H-pass psi set) exist twice for the fused
+    // step, which reads one set and writes the other; E, H and psi[0] always
+    // point at the set holding the current state (index cur).
+    // Every 3-component quantity is one allocation of 3n doubles (components
+    // at a fixed stride), so kernels take one base pointer per quantity; the
+    // per-component pointers below are views into those allocations.
     double *E[3] = {}, *H[3] = {};
+    double *Ev[2] = {}, *Hv[2] = {};
+    double *Eb[2][3] = {}, *Hb[2][3] = {};
+    int cur = 0;
     double2 *eps[3] = {};
     double2 *mu[3] = {};           // only when mu is not uniform
     bool mat_set = false;
@@ -155,17 +163,28 @@ struct emf_solver {
     double mu_re = 1.0;            // uniform mu.re
     double *bE[3] = {}, *aE[3] = {};
     double *mH[3] = {}, *muR[3] = {};
+    double *bEv = nullptr, *aEv = nullptr, *mHv = nullptr, *muRv = nullptr;
     bool coef_dirty = true;
 
     double *tab = nullptr;         // CPML tables: per axis kH bH cH kE bE cE
     bool tab_dirty = true;
-    double *psi[2][6][2] = {};     // [pass H/E][face][component]
+    double *psi[2][6][2] = {};     // [pass H/E][face][component]; psi[0] = psiH[cur]
+    double *psiH[2][6][2] = {};    // the two H-pass sets
+    double *psiHv[2] = {}, *psiEv = nullptr;   // one allocation per psi set
+    long long psi_off[6][2] = {};  // element offset of each slab array in a set
+    long long psi_tot = 0;
     long long psi_n[6] = {};
     bool pml_built = false;
 
     double2 *res[2][6] = {};       // result slots: fwd, adj
 
     std::vector<SourceDev> srcs;
+    SrcBlk *src_desc = nullptr;    // device copy of the source descriptors
+    size_t src_desc_cap = 0;
+    bool src_dirty = true;
+
+    int kind = 1;                  // 0 = fused step kernel, 1 = separate H / E passes
+    int tyg = 4;                   // fused: waves (rows) per work-group
 
     ConvState *conv = nullptr;
     long long *samples = nullptr;
@@ -198,27 +217,70 @@ int ensure_fields(emf_solver *s)
 {
     if (s->Nx <= 0 || s->Ny <= 0 || s->Nz <= 0) EMF_FAIL(EMF_ERR_STATE, "grid dims not set");
     if (s->E[0]) return EMF_OK;
+    CHECK(dalloc(&s->Ev[0], 3 * s->n));
+    CHECK(dalloc(&s->Hv[0], 3 * s->n));
+    CHECK(dalloc(&s->bEv, 3 * s->n));
+    HIPCHK(hipMemsetAsync(s->Ev[0], 0, 3 * s->n * sizeof(double), s->stream));
+    HIPCHK(hipMemsetAsync(s->Hv[0], 0, 3 * s->n * sizeof(double), s->stream));
     for (int c = 0; c < 3; ++c) {
-        CHECK(dalloc(&s->E[c], s->n));
-        CHECK(dalloc(&s->H[c], s->n));
         CHECK(dalloc(&s->eps[c], s->n));
-        CHECK(dalloc(&s->bE[c], s->n));
-        HIPCHK(hipMemsetAsync(s->E[c], 0, s->n * sizeof(double), s->stream));
-        HIPCHK(hipMemsetAsync(s->H[c], 0, s->n * sizeof(double), s->stream));
+        s->Eb[0][c] = s->Ev[0] + c * s->n;
+        s->Hb[0][c] = s->Hv[0] + c * s->n;
+        s->bE[c] = s->bEv + c * s->n;
+        s->E[c] = s->Eb[0][c];
+        s->H[c] = s->Hb[0][c];
+    }
+    s->cur = 0;
+    return EMF_OK;
+}
+
+// Make buffer set `par` the current state.
+void set_current(emf_solver *s, int par)
+{
+    s->cur = par;
+    for (int c = 0; c < 3; ++c) {
+        s->E[c] = s->Eb[par][c];
+        s->H[c] = s->Hb[par][c];
+    }
+    for (int f = 0; f < 6; ++f)
+        for (int c = 0; c < 2; ++c) s->psi[0][f][c] = s->psiH[par][f][c];
+}
+
+// Second buffer set for the fused step (allocated on first use).
+int ensure_alt(emf_solver *s)
+{
+    if (!s->Ev[1]) {
+        CHECK(dalloc(&s->Ev[1], 3 * s->n));
+        CHECK(dalloc(&s->Hv[1], 3 * s->n));
+        for (int c = 0; c < 3; ++c) {
+            s->Eb[1][c] = s->Ev[1] + c * s->n;
+            s->Hb[1][c] = s->Hv[1] + c * s->n;
+        }
+    }
+    if (s->pml_built && !s->psiHv[1] && s->psi_tot > 0) {
+        CHECK(dalloc(&s->psiHv[1], (size_t)s->psi_tot));
+        for (int f = 0; f < 6; ++f)
+            for (int c = 0; c < 2; ++c)
+                s->psiH[1][f][c] = s->psi_n[f] ? s->psiHv[1] + s->psi_off[f][c] : nullptr;
     }
     return EMF_OK;
 }
 
 void free_fields(emf_solver *s)
 {
+    for (int q = 0; q < 2; ++q) { dfree(s->Ev[q]); dfree(s->Hv[q]); }
+    dfree(s->bEv); dfree(s->aEv); dfree(s->mHv); dfree(s->muRv);
     for (int c = 0; c < 3; ++c) {
-        dfree(s->E[c]); dfree(s->H[c]); dfree(s->eps[c]); dfree(s->mu[c]);
-        dfree(s->bE[c]); dfree(s->aE[c]); dfree(s->mH[c]); dfree(s->muR[c]);
+        s->E[c] = s->H[c] = nullptr;
+        s->Eb[0][c] = s->Eb[1][c] = s->Hb[0][c] = s->Hb[1][c] = nullptr;
+        s->bE[c] = s->aE[c] = s->mH[c] = s->muR[c] = nullptr;
+        dfree(s->eps[c]); dfree(s->mu[c]);
     }
     for (int sl = 0; sl < 2; ++sl)
         for (int c = 0; c < 6; ++c) dfree(s->res[sl][c]);
     s->mat_set = false;
     s->coef_dirty = true;
+    s->cur = 0;
 }
 
 int ensure_slot(emf_solver *s, int slot)
@@ -253,13 +315,19 @@ int refresh_coeffs(emf_solver *s)
     if (!s->mat_set) EMF_FAIL(EMF_ERR_STATE, "materials not set (emf_set_mat_arrays)");
     if (!s->coef_dirty) return EMF_OK;
     if (s->dt <= 0) EMF_FAIL(EMF_ERR_STATE, "dt not set");
+    if (s->ceps && !s->aEv) {
+        CHECK(dalloc(&s->aEv, 3 * s->n));
+        for (int c = 0; c < 3; ++c) s->aE[c] = s->aEv + c * s->n;
+    }
+    if (!s->mu_uniform && !s->mHv) {
+        CHECK(dalloc(&s->mHv, 3 * s->n));
+        CHECK(dalloc(&s->muRv, 3 * s->n));
+        for (int c = 0; c < 3; ++c) { s->mH[c] = s->mHv + c * s->n; s->muR[c] = s->muRv + c * s->n; }
+    }
     for (int c = 0; c < 3; ++c) {
-        if (s->ceps && !s->aE[c]) CHECK(dalloc(&s->aE[c], s->n));
         hipLaunchKernelGGL(k_eps_coeffs, dim3(grid1d(s->n, 256)), dim3(256), 0, s->stream,
                            s->eps[c], s->bE[c], s->aE[c], s->n, s->dt, s->odt, s->ceps);
         if (!s->mu_uniform) {
-            if (!s->mH[c]) CHECK(dalloc(&s->mH[c], s->n));
-            if (!s->muR[c]) CHECK(dalloc(&s->muR[c], s->n));
             hipLaunchKernelGGL(k_mu_coeffs, dim3(grid1d(s->n, 256)), dim3(256), 0, s->stream,
                                s->mu[c], s->mH[c], s->muR[c], s->n, s->dt);
         }
@@ -324,9 +392,11 @@ int refresh_tables(emf_solver *s, PassTab *th, PassTab *te)
 
 void free_psi(emf_solver *s)
 {
-    for (int ps = 0; ps < 2; ++ps)
-        for (int f = 0; f < 6; ++f)
-            for (int c = 0; c < 2; ++c) dfree(s->psi[ps][f][c]);
+    dfree(s->psiHv[0]); dfree(s->psiHv[1]); dfree(s->psiEv);
+    for (int f = 0; f < 6; ++f)
+        for (int c = 0; c < 2; ++c)
+            s->psi[0][f][c] = s->psi[1][f][c] = s->psiH[0][f][c] = s->psiH[1][f][c] = nullptr;
+    s->psi_tot = 0;
     s->pml_built = false;
 }
 
@@ -339,26 +409,65 @@ int build_psi(emf_solver *s)
         int wd = s->w[f];
         long long cells = f < 2 ? nz * ny * wd : (f < 4 ? nz * nx * wd : ny * nx * wd);
         s->psi_n[f] = cells;
-        for (int ps = 0; ps < 2; ++ps)
-            for (int c = 0; c < 2; ++c) {
-                CHECK(dalloc(&s->psi[ps][f][c], (size_t)cells));
-                if (cells)
-                    HIPCHK(hipMemsetAsync(s->psi[ps][f][c], 0, cells * sizeof(double), s->stream));
-            }
     }
-    s->pml_built = true;
+    long long tot = 0;
+    for (int f = 0; f < 6; ++f)
+        for (int c = 0; c < 2; ++c) { s->psi_off[f][c] = tot; tot += s->psi_n[f]; }
+    s->psi_tot = tot;
+    if (tot > 0) {
+        CHECK(dalloc(&s->psiHv[0], (size_t)tot));
+        CHECK(dalloc(&s->psiEv, (size_t)tot));
+        if (s->Ev[1]) CHECK(dalloc(&s->psiHv[1], (size_t)tot));
+        HIPCHK(hipMemsetAsync(s->psiHv[0], 0, tot * sizeof(double), s->stream));
+        HIPCHK(hipMemsetAsync(s->psiEv, 0, tot * sizeof(double), s->stream));
+    }
+    for (int f = 0; f < 6; ++f)
+        for (int c = 0; c < 2; ++c) {
+            const bool has = s->psi_n[f] > 0;
+            s->psiH[0][f][c] = has ? s->psiHv[0] + s->psi_off[f][c] : nullptr;
+            s->psiH[1][f][c] = has && s->psiHv[1] ? s->psiHv[1] + s->psi_off[f][c] : nullptr;
+            s->psi[1][f][c] = has ? s->psiEv + s->psi_off[f][c] : nullptr;
+        }
+    s->pml_built = true;   // set_current below needs the views
+    set_current(s, s->cur == 1 && !s->psiHv[1] && tot > 0 ? 0 : s->cur);
     s->tab_dirty = true;
     return EMF_OK;
 }
 
 int reset_psi(emf_solver *s)
 {
-    for (int ps = 0; ps < 2; ++ps)
-        for (int f = 0; f < 6; ++f)
-            for (int c = 0; c < 2; ++c)
-                if (s->psi[ps][f][c])
-                    HIPCHK(hipMemsetAsync(s->psi[ps][f][c], 0, s->psi_n[f] * sizeof(double),
-                                          s->stream));
+    // zero the current H-pass set and the E-pass set
+    if (s->psi_tot > 0) {
+        HIPCHK(hipMemsetAsync(s->psiHv[s->cur] ? s->psiHv[s->cur] : s->psiHv[0], 0,
+                              s->psi_tot * sizeof(double), s->stream));
+        HIPCHK(hipMemsetAsync(s->psiEv, 0, s->psi_tot * sizeof(double), s->stream));
+    }
+    return EMF_OK;
+}
+
+// Device descriptors of the registered sources (rebuilt after changes).
+int refresh_sources(emf_solver *s)
+{
+    if (!s->src_dirty) return EMF_OK;
+    size_t n = s->srcs.size();
+    if (n > s->src_desc_cap) {
+        dfree(s->src_desc);
+        s->src_desc_cap = 0;
+        CHECK(dalloc(&s->src_desc, n));
+        s->src_desc_cap = n;
+    }
+    if (n) {
+        std::vector<SrcBlk> h(n);
+        for (size_t q = 0; q < n; ++q) {
+            const SourceDev &d = s->srcs[q];
+            h[q].i0 = d.i0; h[q].j0 = d.j0; h[q].k0 = d.k0; h[q].I = d.I; h[q].J = d.J; h[q].K = d.K;
+            for (int c = 0; c < 6; ++c) h[q].f[c] = d.f[c];
+        }
+        HIPCHK(hipMemcpyAsync(s->src_desc, h.data(), n * sizeof(SrcBlk), hipMemcpyHostToDevice,
+                              s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));   // h is a temporary
+    }
+    s->src_dirty = false;
     return EMF_OK;
 }
 
@@ -387,92 +496,143 @@ double env_at(const emf_solver *s, double t)
     return 1.0;
 }
 
-PassSrc inline_sources(const emf_solver *s, int first_comp, double t)
+PassSrc pass_sources(const emf_solver *s, int comp0, double t)
 {
     PassSrc ps;
-    memset(&ps, 0, sizeof(ps));
-    ps.t = t;
+    ps.n = (int)s->srcs.size();
+    ps.comp0 = comp0;
+    ps.st = sin(t);
+    ps.ct = cos(t);
     ps.env = env_at(s, t);
-    ps.n = (int)std::min<size_t>(s->srcs.size(), MAX_INLINE_SRC);
-    for (int q = 0; q < ps.n; ++q) {
-        const SourceDev &d = s->srcs[q];
-        SrcBlk &b = ps.b[q];
-        b.i0 = d.i0; b.j0 = d.j0; b.k0 = d.k0; b.I = d.I; b.J = d.J; b.K = d.K;
-        for (int c = 0; c < 3; ++c) b.f[c] = d.f[first_comp + c];
-    }
+    ps.blk = s->src_desc;
     return ps;
 }
 
-dim3 step_grid(const emf_solver *s)
+
+
+// Kernel arguments for the state in buffer set `from` (written to `to`).
+FusedArgs make_args(const emf_solver *s, int from, int to)
 {
-    return dim3((s->Nx + TX - 1) / TX, (s->Ny + TY - 1) / TY, (s->Nz + s->zc - 1) / s->zc);
+    FusedArgs A;
+    A.Eo = s->Ev[from]; A.Ho = s->Hv[from];
+    A.En = s->Ev[to]; A.Hn = s->Hv[to];
+    A.bE = s->bEv; A.aE = s->aEv;
+    A.mH = s->mu_uniform ? nullptr : s->mHv;
+    A.muR = s->mu_uniform ? nullptr : s->muRv;
+    A.pho = s->psiHv[from]; A.phn = s->psiHv[to]; A.pe = s->psiEv;
+    A.tab = s->tab;
+    A.cs = (unsigned)(s->n * 8);
+    A.m_s = s->dt / s->mu_re;
+    A.mu_s = s->mu_re;
+    return A;
 }
 
+dim3 pass_grid(const emf_solver *s)
+{
+    return dim3((s->Nx + PX - 1) / PX, (s->Ny + 3) / 4, (s->Nz + s->zc - 1) / s->zc);
+}
+
+// H pass in place on the current set (update_H, src/fdtd.cpp:121-367)
 int launch_h(emf_solver *s, double t, const int *done)
 {
     PassTab th, te;
     CHECK(refresh_tables(s, &th, &te));
-    PassPsi pp;
-    for (int f = 0; f < 6; ++f)
-        for (int c = 0; c < 2; ++c) pp.p[f][c] = s->psi[0][f][c];
-    PassSrc ps = inline_sources(s, 3, t);
+    CHECK(refresh_sources(s));
+    PassSrc ps = pass_sources(s, 3, t);
+    FusedArgs A = make_args(s, s->cur, s->cur);
     Geo g = s->geo();
-    double m_s = s->dt / s->mu_re;
     if (s->mu_uniform)
-        hipLaunchKernelGGL(k_update_h<false>, step_grid(s), dim3(TX, TY), 0, s->stream,
-                           s->E[0], s->E[1], s->E[2], s->H[0], s->H[1], s->H[2], nullptr, nullptr,
-                           nullptr, nullptr, nullptr, nullptr, m_s, s->mu_re, g, th, pp, ps, done);
+        hipLaunchKernelGGL((k_pass_h<false, 4>), pass_grid(s), dim3(PX, 4), 0, s->stream, A, g, ps, done);
     else
-        hipLaunchKernelGGL(k_update_h<true>, step_grid(s), dim3(TX, TY), 0, s->stream,
-                           s->E[0], s->E[1], s->E[2], s->H[0], s->H[1], s->H[2], s->mH[0],
-                           s->mH[1], s->mH[2], s->muR[0], s->muR[1], s->muR[2], m_s, s->mu_re, g,
-                           th, pp, ps, done);
-    for (size_t q = MAX_INLINE_SRC; q < s->srcs.size(); ++q) {
-        const SourceDev &d = s->srcs[q];
-        SrcBlk b = { d.i0, d.j0, d.k0, d.I, d.J, d.K, { d.f[3], d.f[4], d.f[5] } };
-        long long cnt = (long long)d.I * d.J * d.K;
-        hipLaunchKernelGGL(k_src_h, dim3(grid1d(cnt, 256)), dim3(256), 0, s->stream, s->H[0],
-                           s->H[1], s->H[2], s->mu_uniform ? nullptr : s->muR[0],
-                           s->mu_uniform ? nullptr : s->muR[1], s->mu_uniform ? nullptr : s->muR[2],
-                           s->mu_re, g, b, t, ps.env, done);
-    }
+        hipLaunchKernelGGL((k_pass_h<true, 4>), pass_grid(s), dim3(PX, 4), 0, s->stream, A, g, ps, done);
     HIPCHK(hipGetLastError());
     return EMF_OK;
 }
 
+// E pass in place on the current set (update_E, src/fdtd.cpp:369-755)
 int launch_e(emf_solver *s, double t, const int *done)
 {
     PassTab th, te;
     CHECK(refresh_tables(s, &th, &te));
-    PassPsi pp;
-    for (int f = 0; f < 6; ++f)
-        for (int c = 0; c < 2; ++c) pp.p[f][c] = s->psi[1][f][c];
-    PassSrc ps = inline_sources(s, 0, t);
+    CHECK(refresh_sources(s));
+    PassSrc ps = pass_sources(s, 0, t);
+    FusedArgs A = make_args(s, s->cur, s->cur);
     Geo g = s->geo();
     if (s->ceps)
-        hipLaunchKernelGGL(k_update_e<true>, step_grid(s), dim3(TX, TY), 0, s->stream, s->E[0],
-                           s->E[1], s->E[2], s->H[0], s->H[1], s->H[2], s->bE[0], s->bE[1],
-                           s->bE[2], s->aE[0], s->aE[1], s->aE[2], g, te, pp, ps, done);
+        hipLaunchKernelGGL((k_pass_e<true, 4>), pass_grid(s), dim3(PX, 4), 0, s->stream, A, g, ps, done);
     else
-        hipLaunchKernelGGL(k_update_e<false>, step_grid(s), dim3(TX, TY), 0, s->stream, s->E[0],
-                           s->E[1], s->E[2], s->H[0], s->H[1], s->H[2], s->bE[0], s->bE[1],
-                           s->bE[2], nullptr, nullptr, nullptr, g, te, pp, ps, done);
-    for (size_t q = MAX_INLINE_SRC; q < s->srcs.size(); ++q) {
-        const SourceDev &d = s->srcs[q];
-        SrcBlk b = { d.i0, d.j0, d.k0, d.I, d.J, d.K, { d.f[0], d.f[1], d.f[2] } };
-        long long cnt = (long long)d.I * d.J * d.K;
-        hipLaunchKernelGGL(k_src_e, dim3(grid1d(cnt, 256)), dim3(256), 0, s->stream, s->E[0],
-                           s->E[1], s->E[2], s->bE[0], s->bE[1], s->bE[2], g, b, t, ps.env, done);
+        hipLaunchKernelGGL((k_pass_e<false, 4>), pass_grid(s), dim3(PX, 4), 0, s->stream, A, g, ps, done);
+    HIPCHK(hipGetLastError());
+    return EMF_OK;
+}
+
+template <bool MU, bool CE>
+void launch_fused_t(const emf_solver *s, dim3 grid, const FusedArgs &A, const Geo &g,
+                    const PassSrc &sh, const PassSrc &se, const int *done)
+{
+    switch (s->tyg) {
+    case 1:
+        hipLaunchKernelGGL((k_step_fused<MU, CE, 1>), grid, dim3(FX, 1), 0, s->stream, A, g, sh, se, done);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_step_fused<MU, CE, 2>), grid, dim3(FX, 2), 0, s->stream, A, g, sh, se, done);
+        break;
+    case 8:
+        hipLaunchKernelGGL((k_step_fused<MU, CE, 8>), grid, dim3(FX, 8), 0, s->stream, A, g, sh, se, done);
+        break;
+    default:
+        hipLaunchKernelGGL((k_step_fused<MU, CE, 4>), grid, dim3(FX, 4), 0, s->stream, A, g, sh, se, done);
+    }
+}
+
+// One fused time step n reading buffer set `from` and writing set 1-from.
+int launch_fused(emf_solver *s, long long nidx, int from, const int *done)
+{
+    CHECK(ensure_alt(s));
+    PassTab th, te;
+    CHECK(refresh_tables(s, &th, &te));
+    CHECK(refresh_sources(s));
+    FusedArgs A = make_args(s, from, 1 - from);
+    PassSrc sh = pass_sources(s, 3, (double)nidx * s->dt);
+    PassSrc se = pass_sources(s, 0, ((double)nidx + 0.5) * s->dt);
+    Geo g = s->geo();
+    // x tiles overlap by one cell (63 owned lanes + one halo lane); one row per wave
+    dim3 grid((s->Nx + FXO - 1) / FXO, (s->Ny + s->tyg - 1) / s->tyg, (s->Nz + s->zc - 1) / s->zc);
+    if (s->mu_uniform) {
+        if (s->ceps) launch_fused_t<false, true>(s, grid, A, g, sh, se, done);
+        else launch_fused_t<false, false>(s, grid, A, g, sh, se, done);
+    } else {
+        if (s->ceps) launch_fused_t<true, true>(s, grid, A, g, sh, se, done);
+        else launch_fused_t<true, false>(s, grid, A, g, sh, se, done);
     }
     HIPCHK(hipGetLastError());
     return EMF_OK;
 }
 
-int step(emf_solver *s, long long nidx, const int *done)
+inline bool fused(const emf_solver *s);
+
+// One time step.  Fused: reads set `from`, writes 1-from (the caller tracks
+// which set is current).  Separate passes: in place on the current set.
+int step(emf_solver *s, long long nidx, const int *done, int from)
 {
+    if (fused(s)) return launch_fused(s, nidx, from, done);
     CHECK(launch_h(s, (double)nidx * s->dt, done));
     CHECK(launch_e(s, ((double)nidx + 0.5) * s->dt, done));
     return EMF_OK;
+}
+
+// Whether steps use the fused kernel: selected, and every 3-component buffer
+// and psi set below 4 GiB (the kernel addresses with 32-bit byte offsets).
+inline bool fused(const emf_solver *s)
+{
+    return s->kind != 1 && 3ULL * (unsigned long long)s->n * 8ULL < (1ULL << 32) &&
+           (unsigned long long)s->psi_tot * 8ULL < (1ULL << 32);
+}
+
+// Step n of a run that started at n_start with set `start_set` current.
+inline int from_set(const emf_solver *s, long long n, long long n_start, int start_set)
+{
+    return fused(s) ? (int)((start_set + (n - n_start)) & 1) : start_set;
 }
 
 int capture(emf_solver *s, int slot, int which)
@@ -607,6 +767,7 @@ int emf_delete(emf_solver *s)
         for (auto &d : s->srcs)
             for (int c = 0; c < 6; ++c) dfree(d.f[c]);
         s->srcs.clear();
+        dfree(s->src_desc);
         dfree(s->conv);
         dfree(s->samples);
         dfree(s->hist);
@@ -816,13 +977,14 @@ int emf_add_source(emf_solver *s, const double *Jx, const double *Jy, const doub
     for (int c = 0; c < 6; ++c) {
         for (long long m = 0; m < cnt; ++m) {
             double re = f[c][2 * m], im = f[c][2 * m + 1];
+            double amp = re, ph = im;
             if (calc_phase) {   // (|v|, -atan2(im, re)), src/fdtd.cpp:1517-1576
-                buf[2 * m] = sqrt(re * re + im * im);
-                buf[2 * m + 1] = (im == 0 && re == 0) ? 0.0 : -1 * atan2(im, re);
-            } else {
-                buf[2 * m] = re;
-                buf[2 * m + 1] = im;
+                amp = sqrt(re * re + im * im);
+                ph = (im == 0 && re == 0) ? 0.0 : -1 * atan2(im, re);
             }
+            // stored as (A cos(phi), A sin(phi)); see SrcBlk
+            buf[2 * m] = amp * cos(ph);
+            buf[2 * m + 1] = amp * sin(ph);
         }
         int rc = dalloc(&d.f[c], cnt);
         if (rc != EMF_OK) {
@@ -832,6 +994,7 @@ int emf_add_source(emf_solver *s, const double *Jx, const double *Jy, const doub
         HIPCHK(hipMemcpy(d.f[c], buf.data(), cnt * sizeof(double2), hipMemcpyHostToDevice));
     }
     s->srcs.push_back(d);
+    s->src_dirty = true;
     return EMF_OK;
 }
 
@@ -842,6 +1005,7 @@ int emf_clear_sources(emf_solver *s)
     for (auto &d : s->srcs)
         for (int c = 0; c < 6; ++c) dfree(d.f[c]);
     s->srcs.clear();
+    s->src_dirty = true;
     return EMF_OK;
 }
 
@@ -993,14 +1157,17 @@ int emf_solve(emf_solver *s, int slot, double rtol, double nmax, int64_t Tn, int
     CHECK(check_ready(s));
     CHECK(ensure_slot(s, slot));
     CHECK(build_samples(s));
+    if (fused(s)) CHECK(ensure_alt(s));
     long long cap = (Tn > 0 ? (long long)(std::min(std::max(nmax, 0.0), 1e15) / (double)Tn) : 0) + 4;
     cap = std::min<long long>(cap, 1LL << 20);
     if (cap > s->hist_cap) {
         dfree(s->hist);
         CHECK(dalloc(&s->hist, 3 * (size_t)cap));
-        s->hist_cap = (int)std::min<long long>(cap, 1 << 30);
+        s->hist_cap = (int)cap;
     }
     auto t_start = std::chrono::steady_clock::now();
+    // reset CPML and fields (emopt/fdtd.py:1016-1018); set 0 holds step 0
+    set_current(s, 0);
     CHECK(reset_psi(s));
     for (int c = 0; c < 3; ++c) {
         HIPCHK(hipMemsetAsync(s->E[c], 0, s->n * sizeof(double), s->stream));
@@ -1009,11 +1176,12 @@ int emf_solve(emf_solver *s, int slot, double rtol, double nmax, int64_t Tn, int
     HIPCHK(hipMemsetAsync(s->conv, 0, sizeof(ConvState), s->stream));
     const double amp_rtol = rtol, phi_rtol = sqrt(rtol);
     int *dflag = &s->conv->done;
+    // the state after n steps lives in set from_set(n) (fused) or set 0
 
     long long n = 0;
     int status = 0;
     if (fixed_n0 >= 0) {
-        for (long long q = 0; q < fixed_n0; ++q) CHECK(step(s, q, nullptr));
+        for (long long q = 0; q < fixed_n0; ++q) CHECK(step(s, q, nullptr, from_set(s, q, 0, 0)));
         n = fixed_n0;
         status = 2;
     } else {
@@ -1022,14 +1190,15 @@ int emf_solve(emf_solver *s, int slot, double rtol, double nmax, int64_t Tn, int
         const int LOOK = 2;               // chunks in flight before the first flag is read
         std::vector<std::pair<long long, hipEvent_t>> pend;
         int ring = 0;
-        bool converged = false;
         while (true) {
             while ((int)pend.size() < LOOK && Tn > 0 && n + Tn <= n_stop) {
-                for (long long q = 0; q < Tn; ++q) CHECK(step(s, n + q, dflag));
+                // steps after convergence are gated off by dflag and leave the state as is
+                for (long long q = 0; q < Tn; ++q) CHECK(step(s, n + q, dflag, from_set(s, n + q, 0, 0)));
                 long long nchk = n + Tn - 1;
                 int slotf = ring++ % 64;
-                hipLaunchKernelGGL(k_conv_check, dim3(1), dim3(64), 0, s->stream, s->E[0],
-                                   s->E[1], s->E[2], s->samples, s->nsamples,
+                const int ps = from_set(s, n + Tn, 0, 0);   // set holding E after step nchk
+                hipLaunchKernelGGL(k_conv_check, dim3(1), dim3(64), 0, s->stream, s->Eb[ps][0],
+                                   s->Eb[ps][1], s->Eb[ps][2], s->samples, s->nsamples,
                                    (double)nchk * s->dt, amp_rtol, phi_rtol, nchk, s->conv,
                                    s->hist, s->hist_cap, (int *)nullptr);
                 HIPCHK(hipGetLastError());
@@ -1043,7 +1212,7 @@ int emf_solve(emf_solver *s, int slot, double rtol, double nmax, int64_t Tn, int
             }
             if (pend.empty()) {
                 // fewer than Tn steps remain before nmax: run them and stop
-                for (; n < n_stop; ++n) CHECK(step(s, n, dflag));
+                for (; n < n_stop; ++n) CHECK(step(s, n, dflag, from_set(s, n, 0, 0)));
                 status = 1;
                 break;
             }
@@ -1053,7 +1222,6 @@ int emf_solve(emf_solver *s, int slot, double rtol, double nmax, int64_t Tn, int
             (void)hipEventDestroy(front.second);
             int slotf = (ring - 1 - (int)pend.size()) % 64;
             if (s->done_host[slotf]) {
-                converged = true;
                 n = front.first + Tn;
                 break;
             }
@@ -1062,14 +1230,16 @@ int emf_solve(emf_solver *s, int slot, double rtol, double nmax, int64_t Tn, int
             (void)hipEventSynchronize(pe.second);
             (void)hipEventDestroy(pe.second);
         }
-        (void)converged;
     }
     // capture / two more Tn-step runs / phasor (emopt/fdtd.py:1156-1194)
     long long n0 = n;
+    set_current(s, from_set(s, n0, 0, 0));
     CHECK(capture(s, slot, 0));
-    for (long long q = 0; q < Tn; ++q) CHECK(step(s, n0 + q, nullptr));
+    for (long long q = n0; q < n0 + Tn; ++q) CHECK(step(s, q, nullptr, from_set(s, q, 0, 0)));
+    set_current(s, from_set(s, n0 + Tn, 0, 0));
     CHECK(capture(s, slot, 1));
-    for (long long q = 0; q < Tn; ++q) CHECK(step(s, n0 + Tn + q, nullptr));
+    for (long long q = n0 + Tn; q < n0 + 2 * Tn; ++q) CHECK(step(s, q, nullptr, from_set(s, q, 0, 0)));
+    set_current(s, from_set(s, n0 + 2 * Tn, 0, 0));
     CHECK(phasor3(s, slot, (double)n0 * s->dt, (double)(n0 + Tn) * s->dt,
                   (double)(n0 + 2 * Tn) * s->dt));
     ConvState cs;
@@ -1218,37 +1388,59 @@ int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check
     if (!out || nsteps < 0 || n_start < 0) EMF_FAIL(EMF_ERR_ARG, "bad arguments");
     CHECK(check_ready(s));
     if (check_every > 0) CHECK(build_samples(s));
+    if (fused(s)) CHECK(ensure_alt(s));
     std::vector<hipEvent_t> ev(2 * nsteps + 1);
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipMemsetAsync(s->conv, 0, sizeof(ConvState), s->stream));
     HIPCHK(hipEventRecord(ev[0], s->stream));
+    const int start = s->cur;
     int rc = EMF_OK;
     for (int64_t q = 0; q < nsteps && rc == EMF_OK; ++q) {
         long long n = n_start + q;
-        rc = launch_h(s, (double)n * s->dt, nullptr);
-        if (rc == EMF_OK) rc = hipEventRecord(ev[2 * q + 1], s->stream) == hipSuccess ? EMF_OK : EMF_ERR_HIP;
-        if (rc == EMF_OK) rc = launch_e(s, ((double)n + 0.5) * s->dt, nullptr);
+        if (fused(s)) {
+            rc = launch_fused(s, n, from_set(s, n, n_start, start), nullptr);
+            if (rc == EMF_OK) rc = hipEventRecord(ev[2 * q + 1], s->stream) == hipSuccess ? EMF_OK : EMF_ERR_HIP;
+        } else {
+            rc = launch_h(s, (double)n * s->dt, nullptr);
+            if (rc == EMF_OK) rc = hipEventRecord(ev[2 * q + 1], s->stream) == hipSuccess ? EMF_OK : EMF_ERR_HIP;
+            if (rc == EMF_OK) rc = launch_e(s, ((double)n + 0.5) * s->dt, nullptr);
+        }
         if (rc == EMF_OK && check_every > 0 && (q + 1) % check_every == 0) {
-            hipLaunchKernelGGL(k_conv_check, dim3(1), dim3(64), 0, s->stream, s->E[0], s->E[1],
-                               s->E[2], s->samples, s->nsamples, (double)n * s->dt, 1e-300, 1e-300,
-                               n, s->conv, (double *)nullptr, 0, (int *)nullptr);
+            const int ps = from_set(s, n + 1, n_start, start);
+            hipLaunchKernelGGL(k_conv_check, dim3(1), dim3(64), 0, s->stream, s->Eb[ps][0],
+                               s->Eb[ps][1], s->Eb[ps][2], s->samples, s->nsamples,
+                               (double)n * s->dt, 1e-300, 1e-300, n, s->conv, (double *)nullptr, 0,
+                               (int *)nullptr);
         }
         if (rc == EMF_OK) rc = hipEventRecord(ev[2 * q + 2], s->stream) == hipSuccess ? EMF_OK : EMF_ERR_HIP;
     }
+    if (rc == EMF_OK) set_current(s, from_set(s, n_start + nsteps, n_start, start));
     if (rc == EMF_OK && hipEventSynchronize(ev[2 * nsteps]) != hipSuccess) rc = EMF_ERR_HIP;
     double tot = 0, th = 0, te = 0;
     if (rc == EMF_OK) {
         float ms;
-        hipEventElapsedTime(&ms, ev[0], ev[2 * nsteps]);
+        (void)hipEventElapsedTime(&ms, ev[0], ev[2 * nsteps]);
         tot = ms;
         for (int64_t q = 0; q < nsteps; ++q) {
-            hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]); th += ms;
-            hipEventElapsedTime(&ms, ev[2 * q + 1], ev[2 * q + 2]); te += ms;
+            (void)hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]); th += ms;
+            (void)hipEventElapsedTime(&ms, ev[2 * q + 1], ev[2 * q + 2]); te += ms;
         }
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     if (rc != EMF_OK) EMF_FAIL(rc, "timed stepping failed");
     out[0] = tot; out[1] = th; out[2] = te;
+    return EMF_OK;
+}
+
+int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc)
+{
+    LOCK(s);
+    if (kind != 0 && kind != 1) EMF_FAIL(EMF_ERR_ARG, "kind must be 0 (fused) or 1 (two passes)");
+    if (tyg != 1 && tyg != 2 && tyg != 4 && tyg != 8) EMF_FAIL(EMF_ERR_ARG, "tyg must be 1, 2, 4 or 8");
+    if (zc < 1) EMF_FAIL(EMF_ERR_ARG, "zc must be >= 1");
+    s->kind = kind;
+    s->tyg = tyg;
+    s->zc = zc;
     return EMF_OK;
 }
 

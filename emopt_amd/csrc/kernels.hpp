@@ -24,7 +24,6 @@ namespace emf {
 
 constexpr int TX = 64;          // lanes along x: one wavefront row
 constexpr int TY = 4;           // rows along y per work-group
-constexpr int MAX_INLINE_SRC = 4;
 constexpr int NCONV_MAX = 64;
 constexpr int NUM_XCD = 8;
 
@@ -43,22 +42,35 @@ struct PassTab {
     const double *kap[3], *b[3], *c[3];
 };
 
-// psi slabs of one pass: [face][transverse component], NULL where width 0.
-struct PassPsi {
-    double *p[6][2];
-};
 
+
+// One source block (the reference's SourceArray, src/fdtd.hpp:158-161) as a
+// device-resident descriptor.  Per cell and component it holds
+// (A cos(phi), A sin(phi)) of the reference's amplitude A and phase phi, so
+// that the time factor sin(t + phi) * A = sin(t) A cos(phi) + cos(t) A sin(phi)
+// needs no trigonometry on the device.
 struct SrcBlk {
     int i0, j0, k0, I, J, K;
-    const double2 *f[3];   // (amplitude, phase) per cell, one per component
+    const double2 *f[6];
 };
 
+// Sources of one pass: n descriptors in device memory evaluated at time t:
+// st = sin(t), ct = cos(t), env = ramp envelope (exactly 1 after the ramp)
+// of src_func_t (src/fdtd.cpp:1595-1602).  Blocks are applied in order, as
+// the reference loops over its source list.
 struct PassSrc {
     int n;
-    double t;     // time argument of src_func_t
-    double env;   // ramp envelope at t (exactly 1 after the ramp)
-    SrcBlk b[MAX_INLINE_SRC];
+    int comp0;
+    double st, ct;
+    double env;
+    const SrcBlk *blk;
 };
+
+// src_func_t(t, phi) * A for one cell
+__device__ __forceinline__ double src_val(const PassSrc &p, double2 v)
+{
+    return p.env * (p.st * v.x + p.ct * v.y);
+}
 
 // ---------------------------------------------------------------- helpers
 
@@ -69,18 +81,6 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned lin, unsigned nb)
     unsigned q = nb / NUM_XCD, r = nb % NUM_XCD;
     unsigned x = lin % NUM_XCD, slot = lin / NUM_XCD;
     return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
-}
-
-__device__ __forceinline__ void tile_of(const Geo &g, int &k, int &j, int &z0, int &z1)
-{
-    unsigned nbx = gridDim.x, nby = gridDim.y, nbz = gridDim.z;
-    unsigned lin = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
-    unsigned l = xcd_remap(lin, nbx * nby * nbz);
-    unsigned bx = l % nbx, by = (l / nbx) % nby, bz = l / (nbx * nby);
-    k = (int)(bx * TX + threadIdx.x);
-    j = (int)(by * TY + threadIdx.y);
-    z0 = (int)bz * g.zc;
-    z1 = min(z0 + g.zc, g.Nz);
 }
 
 // psi <- c*d + b*psi ; returns the new psi
@@ -99,273 +99,6 @@ __device__ __forceinline__ bool in_box(const SrcBlk &s, int i, int j, int k, lon
         return true;
     }
     return false;
-}
-
-// --------------------------------------------------------------- H pass
-//
-// update_H of src/fdtd.cpp:121-367 for one z-chunk of one (x, y) column.
-// mH* = dt/mu.re per cell (MU_ARRAY) or the scalar m_s; muR* = mu.re per cell
-// (or mu_s) for the source term ((src_t*|M|)*dt)/mu.re.
-template <bool MU_ARRAY>
-__global__ __launch_bounds__(TX * TY) void k_update_h(
-    const double *__restrict__ Ex, const double *__restrict__ Ey, const double *__restrict__ Ez,
-    double *__restrict__ Hx, double *__restrict__ Hy, double *__restrict__ Hz,
-    const double *__restrict__ mHx, const double *__restrict__ mHy, const double *__restrict__ mHz,
-    const double *__restrict__ muRx, const double *__restrict__ muRy, const double *__restrict__ muRz,
-    double m_s, double mu_s, Geo g, PassTab tab, PassPsi psi, PassSrc src,
-    const int *__restrict__ done)
-{
-    if (done && *done) return;
-    int k, j, z0, z1;
-    tile_of(g, k, j, z0, z1);
-    if (k >= g.Nx || j >= g.Ny) return;
-    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
-    const long long nxy = g.nxy;
-    const bool jn = j + 1 < Ny, kn = k + 1 < Nx;
-    const int xlo = g.w[0], xhi = Nx - g.w[1], ylo = g.w[2], yhi = Ny - g.w[3];
-    const int zlo = g.w[4], zhi = Nz - g.w[5];
-
-    long long p = (long long)z0 * nxy + (long long)j * Nx + k;
-    double ex = Ex[p], ey = Ey[p];
-    for (int i = z0; i < z1; ++i, p += nxy) {
-        const bool in_ = i + 1 < Nz;
-        const double ex_i = in_ ? Ex[p + nxy] : 0.0;
-        const double ey_i = in_ ? Ey[p + nxy] : 0.0;
-        const double ez = Ez[p];
-        const double ez_j = jn ? Ez[p + Nx] : 0.0;
-        const double ex_j = jn ? Ex[p + Nx] : 0.0;
-        const double ez_k = kn ? Ez[p + 1] : 0.0;
-        const double ey_k = kn ? Ey[p + 1] : 0.0;
-        const double mx = MU_ARRAY ? mHx[p] : m_s;
-        const double my = MU_ARRAY ? mHy[p] : m_s;
-        const double mz = MU_ARRAY ? mHz[p] : m_s;
-
-        const double dEzdy = g.ry * (ez_j - ez);
-        const double dEydz = g.rz * (ey_i - ey);
-        double hx = Hx[p] + mx * (dEydz - dEzdy);
-        const double dExdz = g.rz * (ex_i - ex);
-        const double dEzdx = g.rx * (ez_k - ez);
-        double hy = Hy[p] + my * (dEzdx - dExdz);
-        const double dEydx = g.rx * (ey_k - ey);
-        const double dExdy = g.ry * (ex_j - ex);
-        double hz = Hz[p] + mz * (dExdy - dEydx);
-
-        // CPML slabs in the reference's x, y, z order (src/fdtd.cpp:212-306)
-        if (k < xlo || k >= xhi) {
-            const bool lo = k < xlo;
-            double *const q0 = lo ? psi.p[0][0] : psi.p[1][0];
-            double *const q1 = lo ? psi.p[0][1] : psi.p[1][1];
-            const int wl = lo ? g.w[0] : g.w[1];
-            const int kk = lo ? k : k - xhi;
-            const int par = lo ? xlo - k - 1 : k - xhi + g.w[0];
-            const long long ip = ((long long)i * Ny + j) * wl + kk;
-            const double kap = tab.kap[0][par], b = tab.b[0][par], c = tab.c[0][par];
-            const double p0 = psi_step(q0, ip, c, b, dEydx);
-            const double p1 = psi_step(q1, ip, c, b, dEzdx);
-            hz = hz - mz * (p0 - dEydx + dEydx / kap);
-            hy = hy + my * (p1 - dEzdx + dEzdx / kap);
-        }
-        if (j < ylo || j >= yhi) {
-            const bool lo = j < ylo;
-            double *const q0 = lo ? psi.p[2][0] : psi.p[3][0];
-            double *const q1 = lo ? psi.p[2][1] : psi.p[3][1];
-            const int wl = lo ? g.w[2] : g.w[3];
-            const int jj = lo ? j : j - yhi;
-            const int par = lo ? ylo - j - 1 : j - yhi + g.w[2];
-            const long long ip = ((long long)i * wl + jj) * Nx + k;
-            const double kap = tab.kap[1][par], b = tab.b[1][par], c = tab.c[1][par];
-            const double p0 = psi_step(q0, ip, c, b, dExdy);
-            const double p1 = psi_step(q1, ip, c, b, dEzdy);
-            hz = hz + mz * (p0 - dExdy + dExdy / kap);
-            hx = hx - mx * (p1 - dEzdy + dEzdy / kap);
-        }
-        if (i < zlo || i > zhi) {   // zmax test is strict: its first layer is skipped
-            const bool lo = i < zlo;
-            double *const q0 = lo ? psi.p[4][0] : psi.p[5][0];
-            double *const q1 = lo ? psi.p[4][1] : psi.p[5][1];
-            const int ii = lo ? i : i - zhi;
-            const int par = lo ? zlo - i - 1 : i - zhi + g.w[4];
-            const long long ip = ((long long)ii * Ny + j) * Nx + k;
-            const double kap = tab.kap[2][par], b = tab.b[2][par], c = tab.c[2][par];
-            const double p0 = psi_step(q0, ip, c, b, dExdz);
-            const double p1 = psi_step(q1, ip, c, b, dEydz);
-            hx = hx + mx * (p1 - dEydz + dEydz / kap);
-            hy = hy - my * (p0 - dExdz + dExdz / kap);
-        }
-
-        // magnetic current sources (src/fdtd.cpp:313-366), block by block
-#pragma unroll
-        for (int q = 0; q < MAX_INLINE_SRC; ++q) {
-            long long m;
-            if (q < src.n && in_box(src.b[q], i, j, k, m)) {
-                const double2 vx = src.b[q].f[0][m], vy = src.b[q].f[1][m], vz = src.b[q].f[2][m];
-                const double ux = MU_ARRAY ? muRx[p] : mu_s;
-                const double uy = MU_ARRAY ? muRy[p] : mu_s;
-                const double uz = MU_ARRAY ? muRz[p] : mu_s;
-                hx = hx + sin(src.t + vx.y) * src.env * vx.x * g.dt / ux;
-                hy = hy + sin(src.t + vy.y) * src.env * vy.x * g.dt / uy;
-                hz = hz + sin(src.t + vz.y) * src.env * vz.x * g.dt / uz;
-            }
-        }
-
-        Hx[p] = hx; Hy[p] = hy; Hz[p] = hz;
-        ex = ex_i; ey = ey_i;
-    }
-}
-
-// --------------------------------------------------------------- E pass
-//
-// update_E of src/fdtd.cpp:369-755.  bE* = dt/eps.re (real eps) or
-// 1/(eps.re/dt + eps.im/2) (complex eps); aE* only for complex eps.
-template <bool CEPS>
-__global__ __launch_bounds__(TX * TY) void k_update_e(
-    double *__restrict__ Ex, double *__restrict__ Ey, double *__restrict__ Ez,
-    const double *__restrict__ Hx, const double *__restrict__ Hy, const double *__restrict__ Hz,
-    const double *__restrict__ bEx, const double *__restrict__ bEy, const double *__restrict__ bEz,
-    const double *__restrict__ aEx, const double *__restrict__ aEy, const double *__restrict__ aEz,
-    Geo g, PassTab tab, PassPsi psi, PassSrc src, const int *__restrict__ done)
-{
-    if (done && *done) return;
-    int k, j, z0, z1;
-    tile_of(g, k, j, z0, z1);
-    if (k >= g.Nx || j >= g.Ny) return;
-    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
-    const long long nxy = g.nxy;
-    const int xlo = g.w[0], xhi = Nx - g.w[1], ylo = g.w[2], yhi = Ny - g.w[3];
-    const int zlo = g.w[4], zhi = Nz - g.w[5];
-    // min-face ghost rule: 0 -> 0, 'E' -> -H(first), 'H' -> +H(first)
-    const double sx = g.bc[0] == 'E' ? -1.0 : (g.bc[0] == 'H' ? 1.0 : 0.0);
-    const double sy = g.bc[1] == 'E' ? -1.0 : (g.bc[1] == 'H' ? 1.0 : 0.0);
-    const double sz = g.bc[2] == 'E' ? -1.0 : (g.bc[2] == 'H' ? 1.0 : 0.0);
-    const bool bx0 = g.bc[0] == '0', by0 = g.bc[1] == '0', bz0 = g.bc[2] == '0';
-
-    long long p = (long long)z0 * nxy + (long long)j * Nx + k;
-    double hx_m, hy_m;   // H at i-1
-    if (z0 > 0) {
-        hx_m = Hx[p - nxy];
-        hy_m = Hy[p - nxy];
-    } else {
-        hx_m = bz0 ? 0.0 : sz * Hx[p];
-        hy_m = bz0 ? 0.0 : sz * Hy[p];
-    }
-    for (int i = z0; i < z1; ++i, p += nxy) {
-        const double hx = Hx[p], hy = Hy[p], hz = Hz[p];
-        double hz_j, hx_j, hz_k, hy_k;
-        if (j > 0) { hz_j = Hz[p - Nx]; hx_j = Hx[p - Nx]; }
-        else { hz_j = by0 ? 0.0 : sy * hz; hx_j = by0 ? 0.0 : sy * hx; }
-        if (k > 0) { hz_k = Hz[p - 1]; hy_k = Hy[p - 1]; }
-        else { hz_k = bx0 ? 0.0 : sx * hz; hy_k = bx0 ? 0.0 : sx * hy; }
-        const double bx = bEx[p], by = bEy[p], bz = bEz[p];
-
-        const double dHzdy = g.ry * (hz - hz_j);
-        const double dHydz = g.rz * (hy - hy_m);
-        double ex, ey, ez;
-        if (CEPS) ex = aEx[p] * Ex[p] + (dHzdy - dHydz) * bx;
-        else ex = Ex[p] + (dHzdy - dHydz) * bx;
-        const double dHxdz = g.rz * (hx - hx_m);
-        const double dHzdx = g.rx * (hz - hz_k);
-        if (CEPS) ey = aEy[p] * Ey[p] + (dHxdz - dHzdx) * by;
-        else ey = Ey[p] + (dHxdz - dHzdx) * by;
-        const double dHydx = g.rx * (hy - hy_k);
-        const double dHxdy = g.ry * (hx - hx_j);
-        if (CEPS) ez = aEz[p] * Ez[p] + (dHydx - dHxdy) * bz;
-        else ez = Ez[p] + (dHydx - dHxdy) * bz;
-
-        // CPML slabs (src/fdtd.cpp:573-666)
-        if (k < xlo || k >= xhi) {
-            const bool lo = k < xlo;
-            double *const q0 = lo ? psi.p[0][0] : psi.p[1][0];
-            double *const q1 = lo ? psi.p[0][1] : psi.p[1][1];
-            const int wl = lo ? g.w[0] : g.w[1];
-            const int kk = lo ? k : k - xhi;
-            const int par = lo ? xlo - k - 1 : k - xhi + g.w[0];
-            const long long ip = ((long long)i * Ny + j) * wl + kk;
-            const double kap = tab.kap[0][par], b = tab.b[0][par], c = tab.c[0][par];
-            const double p0 = psi_step(q0, ip, c, b, dHydx);
-            const double p1 = psi_step(q1, ip, c, b, dHzdx);
-            ez = ez + (p0 - dHydx + dHydx / kap) * bz;
-            ey = ey - (p1 - dHzdx + dHzdx / kap) * by;
-        }
-        if (j < ylo || j >= yhi) {
-            const bool lo = j < ylo;
-            double *const q0 = lo ? psi.p[2][0] : psi.p[3][0];
-            double *const q1 = lo ? psi.p[2][1] : psi.p[3][1];
-            const int wl = lo ? g.w[2] : g.w[3];
-            const int jj = lo ? j : j - yhi;
-            const int par = lo ? ylo - j - 1 : j - yhi + g.w[2];
-            const long long ip = ((long long)i * wl + jj) * Nx + k;
-            const double kap = tab.kap[1][par], b = tab.b[1][par], c = tab.c[1][par];
-            const double p0 = psi_step(q0, ip, c, b, dHxdy);
-            const double p1 = psi_step(q1, ip, c, b, dHzdy);
-            ez = ez - (p0 - dHxdy + dHxdy / kap) * bz;
-            ex = ex + (p1 - dHzdy + dHzdy / kap) * bx;
-        }
-        if (i < zlo || i > zhi) {
-            const bool lo = i < zlo;
-            double *const q0 = lo ? psi.p[4][0] : psi.p[5][0];
-            double *const q1 = lo ? psi.p[4][1] : psi.p[5][1];
-            const int ii = lo ? i : i - zhi;
-            const int par = lo ? zlo - i - 1 : i - zhi + g.w[4];
-            const long long ip = ((long long)ii * Ny + j) * Nx + k;
-            const double kap = tab.kap[2][par], b = tab.b[2][par], c = tab.c[2][par];
-            const double p0 = psi_step(q0, ip, c, b, dHxdz);
-            const double p1 = psi_step(q1, ip, c, b, dHydz);
-            ex = ex - (p1 - dHydz + dHydz / kap) * bx;
-            ey = ey + (p0 - dHxdz + dHxdz / kap) * by;
-        }
-
-        // electric current sources (src/fdtd.cpp:672-754)
-#pragma unroll
-        for (int q = 0; q < MAX_INLINE_SRC; ++q) {
-            long long m;
-            if (q < src.n && in_box(src.b[q], i, j, k, m)) {
-                const double2 vx = src.b[q].f[0][m], vy = src.b[q].f[1][m], vz = src.b[q].f[2][m];
-                ex = ex - sin(src.t + vx.y) * src.env * vx.x * bx;
-                ey = ey - sin(src.t + vy.y) * src.env * vy.x * by;
-                ez = ez - sin(src.t + vz.y) * src.env * vz.x * bz;
-            }
-        }
-
-        Ex[p] = ex; Ey[p] = ey; Ez[p] = ez;
-        hx_m = hx; hy_m = hy;
-    }
-}
-
-// Sources beyond the inline ones: one thread per source cell, applied after
-// the main pass in block order (same per-cell order as the reference loops).
-__global__ void k_src_h(double *__restrict__ Hx, double *__restrict__ Hy, double *__restrict__ Hz,
-                        const double *__restrict__ muRx, const double *__restrict__ muRy,
-                        const double *__restrict__ muRz, double mu_s, Geo g, SrcBlk s,
-                        double t, double env, const int *__restrict__ done)
-{
-    if (done && *done) return;
-    long long m = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    long long n = (long long)s.I * s.J * s.K;
-    if (m >= n) return;
-    int k = (int)(m % s.K), j = (int)((m / s.K) % s.J), i = (int)(m / ((long long)s.K * s.J));
-    long long p = (long long)(i + s.i0) * g.nxy + (long long)(j + s.j0) * g.Nx + (k + s.k0);
-    const double2 vx = s.f[0][m], vy = s.f[1][m], vz = s.f[2][m];
-    const double ux = muRx ? muRx[p] : mu_s, uy = muRy ? muRy[p] : mu_s, uz = muRz ? muRz[p] : mu_s;
-    Hx[p] = Hx[p] + sin(t + vx.y) * env * vx.x * g.dt / ux;
-    Hy[p] = Hy[p] + sin(t + vy.y) * env * vy.x * g.dt / uy;
-    Hz[p] = Hz[p] + sin(t + vz.y) * env * vz.x * g.dt / uz;
-}
-
-__global__ void k_src_e(double *__restrict__ Ex, double *__restrict__ Ey, double *__restrict__ Ez,
-                        const double *__restrict__ bEx, const double *__restrict__ bEy,
-                        const double *__restrict__ bEz, Geo g, SrcBlk s, double t, double env,
-                        const int *__restrict__ done)
-{
-    if (done && *done) return;
-    long long m = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    long long n = (long long)s.I * s.J * s.K;
-    if (m >= n) return;
-    int k = (int)(m % s.K), j = (int)((m / s.K) % s.J), i = (int)(m / ((long long)s.K * s.J));
-    long long p = (long long)(i + s.i0) * g.nxy + (long long)(j + s.j0) * g.Nx + (k + s.k0);
-    const double2 vx = s.f[0][m], vy = s.f[1][m], vz = s.f[2][m];
-    Ex[p] = Ex[p] - sin(t + vx.y) * env * vx.x * bEx[p];
-    Ey[p] = Ey[p] - sin(t + vy.y) * env * vy.x * bEy[p];
-    Ez[p] = Ez[p] - sin(t + vz.y) * env * vz.x * bEz[p];
 }
 
 // ------------------------------------------------------ convergence check

@@ -1,0 +1,180 @@
+// Separate H and E passes (update_H / update_E of src/fdtd.cpp:121-755), in
+// place on one buffer set, with the same argument layout as the fused step.
+//
+// HBM traffic per cell: H pass reads E(3) H(3) and writes H(3) = 72 B; E pass
+// reads H(3) E(3) b_eps(3) and writes E(3) = 96 B; plus the psi slabs.
+//
+// A 64 x TY work-group owns a tile of (x, y) columns and sweeps zc planes
+// (zc = 1 is one cell per thread).  Every load a plane needs -- fields,
+// coefficients, psi and CPML table entries -- is issued before any store, so
+// a thread waits for memory once per plane; in-plane neighbours come through
+// L1/L2.  Work-groups are numbered x fastest, then y, then z (plain order).
+#pragma once
+#include "fused.hpp"
+
+namespace emf {
+
+constexpr int PX = 64;
+
+__device__ __forceinline__ void pass_tile(const Geo &g, int ty, int &k, int &j, int &z0, int &z1)
+{
+    const unsigned nbx = gridDim.x, nby = gridDim.y;
+    const unsigned l = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    k = (int)((l % nbx) * PX + threadIdx.x);
+    j = (int)(((l / nbx) % nby) * ty + threadIdx.y);
+    z0 = (int)(l / (nbx * nby)) * g.zc;
+    z1 = min(z0 + g.zc, g.Nz);
+}
+
+// ------------------------------------------------------------------ H pass
+template <bool MU_ARRAY, int TY>
+__global__ __launch_bounds__(PX * TY) void k_pass_h(FusedArgs A, Geo g, PassSrc sh,
+                                                   const int *__restrict__ done)
+{
+    if (done && *done) return;
+    int k, j, z0, z1;
+    pass_tile(g, TY, k, j, z0, z1);
+    if (k >= g.Nx || j >= g.Ny) return;
+    const Layout L = make_layout(g);
+    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    const unsigned cs = A.cs, cs2 = 2 * A.cs;
+    const unsigned oz = (unsigned)g.nxy * 8u, oy = (unsigned)Nx * 8u;
+    const bool jn = j + 1 < Ny, kn = k + 1 < Nx;
+    bool col_src = false;   // does this column meet a source block?
+    for (int q = 0; q < sh.n; ++q) {
+        const SrcBlk &s = sh.blk[q];
+        if (k >= s.k0 && k < s.k0 + s.K && j >= s.j0 && j < s.j0 + s.J && s.i0 < z1 && s.i0 + s.I > z0)
+            col_src = true;
+    }
+    unsigned o = ((unsigned)z0 * (unsigned)Ny + (unsigned)j) * (unsigned)Nx * 8u + (unsigned)k * 8u;
+    double ex = ldo(A.Eo, o), ey = ldo(A.Eo, o + cs);
+    for (int i = z0; i < z1; ++i, o += oz) {
+        HIn d;
+        d.ex = ex; d.ey = ey;
+        d.exi = d.eyi = d.ezj = d.exj = d.ezk = d.eyk = 0;
+        if (i + 1 < Nz) { d.exi = ldo(A.Eo, o + oz); d.eyi = ldo(A.Eo, o + oz + cs); }
+        d.ez = ldo(A.Eo, o + cs2);
+        if (jn) { d.ezj = ldo(A.Eo, o + oy + cs2); d.exj = ldo(A.Eo, o + oy); }
+        if (kn) { d.ezk = ldo(A.Eo, o + 8u + cs2); d.eyk = ldo(A.Eo, o + 8u + cs); }
+        d.hx = ldo(A.Ho, o); d.hy = ldo(A.Ho, o + cs); d.hz = ldo(A.Ho, o + cs2);
+        h_load_aux<MU_ARRAY>(g, L, A, i, j, k, o, d);
+        double hx, hy, hz, np[6];
+        h_compute(g, d, hx, hy, hz, np);
+        if (col_src) h_sources<MU_ARRAY>(g, sh, A, i, j, k, o, hx, hy, hz);
+        sto(A.Hn, o, hx); sto(A.Hn, o + cs, hy); sto(A.Hn, o + cs2, hz);
+        if (d.sx.in) { const unsigned f = d.sx.lo ? 0 : 1; sto(A.phn, L.pso[f][0] + d.sx.io, np[0]); sto(A.phn, L.pso[f][1] + d.sx.io, np[1]); }
+        if (d.sy.in) { const unsigned f = d.sy.lo ? 2 : 3; sto(A.phn, L.pso[f][0] + d.sy.io, np[2]); sto(A.phn, L.pso[f][1] + d.sy.io, np[3]); }
+        if (d.sz.in) { const unsigned f = d.sz.lo ? 4 : 5; sto(A.phn, L.pso[f][0] + d.sz.io, np[4]); sto(A.phn, L.pso[f][1] + d.sz.io, np[5]); }
+        ex = d.exi; ey = d.eyi;
+    }
+}
+
+// ------------------------------------------------------------------ E pass
+template <bool CEPS, int TY>
+__global__ __launch_bounds__(PX * TY) void k_pass_e(FusedArgs A, Geo g, PassSrc se,
+                                                   const int *__restrict__ done)
+{
+    if (done && *done) return;
+    int k, j, z0, z1;
+    pass_tile(g, TY, k, j, z0, z1);
+    if (k >= g.Nx || j >= g.Ny) return;
+    const Layout L = make_layout(g);
+    const int Nx = g.Nx, Ny = g.Ny;
+    const unsigned cs = A.cs, cs2 = 2 * A.cs;
+    const unsigned oz = (unsigned)g.nxy * 8u, oy = (unsigned)Nx * 8u;
+    // min-face ghost rule: '0' -> 0, 'E' -> -H(first), 'H' -> +H(first)
+    const double sgx = g.bc[0] == 'E' ? -1.0 : (g.bc[0] == 'H' ? 1.0 : 0.0);
+    const double sgy = g.bc[1] == 'E' ? -1.0 : (g.bc[1] == 'H' ? 1.0 : 0.0);
+    const double sgz = g.bc[2] == 'E' ? -1.0 : (g.bc[2] == 'H' ? 1.0 : 0.0);
+    const bool bx0 = g.bc[0] == '0', by0 = g.bc[1] == '0', bz0 = g.bc[2] == '0';
+    bool col_src = false;
+    for (int q = 0; q < se.n; ++q) {
+        const SrcBlk &s = se.blk[q];
+        if (k >= s.k0 && k < s.k0 + s.K && j >= s.j0 && j < s.j0 + s.J && s.i0 < z1 && s.i0 + s.I > z0)
+            col_src = true;
+    }
+    unsigned o = ((unsigned)z0 * (unsigned)Ny + (unsigned)j) * (unsigned)Nx * 8u + (unsigned)k * 8u;
+    double hxm, hym;   // H at i-1
+    if (z0 > 0) { hxm = ldo(A.Ho, o - oz); hym = ldo(A.Ho, o - oz + cs); }
+    else { hxm = bz0 ? 0.0 : sgz * ldo(A.Ho, o); hym = bz0 ? 0.0 : sgz * ldo(A.Ho, o + cs); }
+    for (int i = z0; i < z1; ++i, o += oz) {
+        // ---- all loads of this plane
+        const double hx = ldo(A.Ho, o), hy = ldo(A.Ho, o + cs), hz = ldo(A.Ho, o + cs2);
+        double hz_j = 0, hx_j = 0, hz_k = 0, hy_k = 0;
+        if (j > 0) { hz_j = ldo(A.Ho, o - oy + cs2); hx_j = ldo(A.Ho, o - oy); }
+        if (k > 0) { hz_k = ldo(A.Ho, o - 8u + cs2); hy_k = ldo(A.Ho, o - 8u + cs); }
+        const double ex = ldo(A.Eo, o), ey = ldo(A.Eo, o + cs), ez = ldo(A.Eo, o + cs2);
+        const double bx = ldo(A.bE, o), by = ldo(A.bE, o + cs), bz = ldo(A.bE, o + cs2);
+        double ax = 1, ay = 1, az = 1;
+        if (CEPS) { ax = ldo(A.aE, o); ay = ldo(A.aE, o + cs); az = ldo(A.aE, o + cs2); }
+        const Slab sx = slab_x(g, i, j, k), sy = slab_y(g, i, j, k), sz = slab_z(g, i, j, k);
+        const unsigned fx = sx.lo ? 0 : 1, fy = sy.lo ? 2 : 3, fz = sz.lo ? 4 : 5;
+        double qx0 = 0, qx1 = 0, qy0 = 0, qy1 = 0, qz0 = 0, qz1 = 0;
+        double kx = 1, tbx = 0, tcx = 0, ky = 1, tby = 0, tcy = 0, kz = 1, tbz = 0, tcz = 0;
+        if (sx.in) {
+            qx0 = ldo(A.pe, L.pso[fx][0] + sx.io); qx1 = ldo(A.pe, L.pso[fx][1] + sx.io);
+            const unsigned t = L.tbo[0] + 3 * L.tbl[0] + (unsigned)sx.par * 8u;
+            kx = ldo(A.tab, t); tbx = ldo(A.tab, t + L.tbl[0]); tcx = ldo(A.tab, t + 2 * L.tbl[0]);
+        }
+        if (sy.in) {
+            qy0 = ldo(A.pe, L.pso[fy][0] + sy.io); qy1 = ldo(A.pe, L.pso[fy][1] + sy.io);
+            const unsigned t = L.tbo[1] + 3 * L.tbl[1] + (unsigned)sy.par * 8u;
+            ky = ldo(A.tab, t); tby = ldo(A.tab, t + L.tbl[1]); tcy = ldo(A.tab, t + 2 * L.tbl[1]);
+        }
+        if (sz.in) {
+            qz0 = ldo(A.pe, L.pso[fz][0] + sz.io); qz1 = ldo(A.pe, L.pso[fz][1] + sz.io);
+            const unsigned t = L.tbo[2] + 3 * L.tbl[2] + (unsigned)sz.par * 8u;
+            kz = ldo(A.tab, t); tbz = ldo(A.tab, t + L.tbl[2]); tcz = ldo(A.tab, t + 2 * L.tbl[2]);
+        }
+        // ---- boundary ghosts at the min faces
+        if (j == 0) { hz_j = by0 ? 0.0 : sgy * hz; hx_j = by0 ? 0.0 : sgy * hx; }
+        if (k == 0) { hz_k = bx0 ? 0.0 : sgx * hz; hy_k = bx0 ? 0.0 : sgx * hy; }
+        // ---- update (src/fdtd.cpp:554-666)
+        const double dHzdy = g.ry * (hz - hz_j);
+        const double dHydz = g.rz * (hy - hym);
+        double nex = CEPS ? ax * ex + (dHzdy - dHydz) * bx : ex + (dHzdy - dHydz) * bx;
+        const double dHxdz = g.rz * (hx - hxm);
+        const double dHzdx = g.rx * (hz - hz_k);
+        double ney = CEPS ? ay * ey + (dHxdz - dHzdx) * by : ey + (dHxdz - dHzdx) * by;
+        const double dHydx = g.rx * (hy - hy_k);
+        const double dHxdy = g.ry * (hx - hx_j);
+        double nez = CEPS ? az * ez + (dHydx - dHxdy) * bz : ez + (dHydx - dHxdy) * bz;
+        if (sx.in) {
+            qx0 = tcx * dHydx + tbx * qx0;
+            qx1 = tcx * dHzdx + tbx * qx1;
+            nez = nez + (qx0 - dHydx + dHydx / kx) * bz;
+            ney = ney - (qx1 - dHzdx + dHzdx / kx) * by;
+        }
+        if (sy.in) {
+            qy0 = tcy * dHxdy + tby * qy0;
+            qy1 = tcy * dHzdy + tby * qy1;
+            nez = nez - (qy0 - dHxdy + dHxdy / ky) * bz;
+            nex = nex + (qy1 - dHzdy + dHzdy / ky) * bx;
+        }
+        if (sz.in) {
+            qz0 = tcz * dHxdz + tbz * qz0;
+            qz1 = tcz * dHydz + tbz * qz1;
+            nex = nex - (qz1 - dHydz + dHydz / kz) * bx;
+            ney = ney + (qz0 - dHxdz + dHxdz / kz) * by;
+        }
+        if (col_src) {
+            for (int q = 0; q < se.n; ++q) {
+                long long m;
+                const SrcBlk &sb = se.blk[q];
+                if (in_box(sb, i, j, k, m)) {
+                    const double2 vx = sb.f[0][m], vy = sb.f[1][m], vz = sb.f[2][m];
+                    nex = nex - src_val(se, vx) * bx;
+                    ney = ney - src_val(se, vy) * by;
+                    nez = nez - src_val(se, vz) * bz;
+                }
+            }
+        }
+        sto(A.En, o, nex); sto(A.En, o + cs, ney); sto(A.En, o + cs2, nez);
+        if (sx.in) { sto(A.pe, L.pso[fx][0] + sx.io, qx0); sto(A.pe, L.pso[fx][1] + sx.io, qx1); }
+        if (sy.in) { sto(A.pe, L.pso[fy][0] + sy.io, qy0); sto(A.pe, L.pso[fy][1] + sy.io, qy1); }
+        if (sz.in) { sto(A.pe, L.pso[fz][0] + sz.io, qz0); sto(A.pe, L.pso[fz][1] + sz.io, qz1); }
+        hxm = hx; hym = hy;
+    }
+}
+
+}  // namespace emf

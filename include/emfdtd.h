@@ -234,6 +234,12 @@ EMF_API int emf_get_field(emf_solver *s, int slot, int comp, int k1, int k2, int
 EMF_API int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check_every,
                            double *out);
 
+/* Step kernel selection: kind 0 = one fused H+E launch per step, 1 = separate
+ * H and E passes (default); tyg = rows (wavefronts) per work-group of the
+ * fused kernel (1, 2, 4 or 8); zc = z planes per work-group.  Results are
+ * identical; only speed differs. */
+EMF_API int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc);
+
 /* Device copy bandwidth (GB/s, read + write bytes) of a plain vector copy of
  * `bytes` bytes, median of `reps` launches; a roofline reference point. */
 EMF_API int emf_copy_bandwidth(int device, int64_t bytes, int reps, double *gbps);

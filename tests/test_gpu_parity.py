@@ -137,3 +137,67 @@ def test_gpu_matches_reference_golden_solve():
     errs = [rel_l2(a, b) for a, b in zip(r["fields"], g["fields"])]
     print("vs reference: rel L2 Ex..Hz:", " ".join("%.2e" % e for e in errs))
     assert max(errs) <= TOL, errs
+
+
+def _run_steps(case, nsteps, kind, tyg=4, zc=8):
+    """nsteps time steps through emf_time_steps with the given step kernel."""
+    import ctypes
+    import warnings
+    from emopt_amd.fdtd import FDTD
+    from emopt_amd.misc import DomainCoordinates
+    from tests.cases import _configure
+    d = case["d"]
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim = FDTD(case["X"], case["Y"], case["Z"], d, d, d, case["wl"],
+                   complex_eps=case.get("ceps", False))
+        _configure(sim, case, DomainCoordinates)
+    sim.set_material_arrays(case["mats"][:3], case["mats"][3:])
+    sim._register(sim._sources)
+    L, h = sim._lib, sim._h
+    L.emf_set_tuning(h, kind, tyg, zc)
+    L.emf_reset_pml(h)
+    L.emf_zero_fields(h)
+    out = np.zeros(3)
+    L.emf_time_steps(h, 0, nsteps, 0, out.ctypes.data_as(ctypes.c_void_p))
+    return sim.time_fields()
+
+
+@pytest.mark.parametrize("variant", ["plain", "ceps", "mu", "bcs", "manysrc"])
+def test_fused_step_matches_two_pass(variant):
+    """The fused H+E kernel (ping-pong buffers, halo row/lane recomputation)
+    against the two-pass kernels, for every work-group shape and z-chunk."""
+    kw = dict(w=(5, 4, 3, 6, 4, 5), seed=31, nsrc=2, mu_rand=False)
+    if variant == "ceps":
+        kw["ceps"] = True
+    if variant == "mu":
+        kw["mu_rand"] = True
+    if variant == "bcs":
+        kw["bc"] = "EH0"
+    if variant == "manysrc":
+        kw["nsrc"] = 7
+    case = random_case(70, 37, 23, **kw)
+    ref = _run_steps(case, 33, kind=1)
+    for tyg in (1, 2, 4, 8):
+        for zc in (1, 5, 8, 64):
+            got = _run_steps(case, 33, kind=0, tyg=tyg, zc=zc)
+            errs = rel_l2_set(got, ref)
+            assert max(errs) < 1e-13, (variant, tyg, zc, errs)
+
+
+def test_fused_solve_matches_two_pass_solve():
+    case = strip_case(40, 28, 22, w=5)
+    import warnings
+    a = run_gpu(case, fixed_n0=700)
+    sim = a["sim"]
+    sim._lib.emf_set_tuning(sim._h, 1, 8, 8)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim._register(sim._sources)
+        sim.fixed_n0 = 700
+        sim._solve(0)
+    from emopt_amd.misc import DomainCoordinates
+    full = DomainCoordinates.from_indices(0, case["Nx"], 0, case["Ny"], 0, case["Nz"], *([case["d"]] * 3))
+    b = [sim.get_field(c, full) for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")]
+    errs = [rel_l2(x, y) for x, y in zip(a["fields"], b)]
+    assert max(errs) < 1e-12, errs
