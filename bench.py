@@ -9,20 +9,33 @@ and uploaded once; the timed region starts with everything resident in HBM.
 
 A step is one full Yee time step (H pass + E pass, CPML slabs, source
 injection) with the on-device convergence check every Tn steps, as the solve
-loop runs it.  value = Nx*Ny*Nz*K / (wall time of K steps).
+loop runs it.  value = Nx*Ny*Nz*K / (wall time of K steps); the library call
+that runs them synchronises its stream before it returns.
 
-Also reported: per-kernel HBM roofline (HIP events on the library's own
-stream), a plain device-copy bandwidth, one full converged solve, and the
-reference CPU kernel (EMopt's src/fdtd.cpp compiled from its sources,
-single thread) timed on a bounded sample of the same workload on this host.
+Also reported:
+  roofline      the dominant kernel's algorithmic bytes per launch divided by
+                its average launch time (HIP event pairs on the solver's own
+                stream, in a second run of min(K, 200) steps after the timed
+                region), against 8 TB/s; `traffic` is the HBM bytes per launch
+                measured by rocprofv3 counters (FETCH_SIZE, WRITE_SIZE, scaled
+                by the same counters on a plain copy of known size), read from
+                the committed summary profiles/hbm_traffic_step_400x200x120.md
+  cpu_baseline  EMopt's own kernel (src/fdtd.cpp compiled from its sources),
+                one thread, on a bounded sample of the same workload
+  cpu_all_cores the C restatement built with OpenMP, all host threads
+  solve         one full converged solve (solves/hour)
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --profile-summary --stats S.csv [--fetch F.csv --write W.csv]
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import ctypes
 import json
 import os
+import re
 import sys
 import time
 import warnings
@@ -33,9 +46,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-B_ALG_H = 72    # bytes per cell, H pass: read E(3) H(3), write H(3)
-B_ALG_E = 96    # bytes per cell, E pass: read H(3) E(3) b_eps(3), write E(3)
+B_ALG_H = 72    # bytes per cell, H pass: read E(3) H(3), write H(3); mu == 1
+B_ALG_E = 96    # bytes per cell, E pass: read H(3) E(3) b = dt/eps (3), write E(3)
 B_ALG_STEP = B_ALG_H + B_ALG_E
+KERNELS = {"k_pass_h": B_ALG_H, "k_pass_e": B_ALG_E}
+COPY_BYTES = 1 << 31            # k_copy reads and writes this many bytes
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_step_400x200x120.md")
 
 
 def log(*a):
@@ -56,20 +72,17 @@ def make_sim(case):
     return sim
 
 
-def time_steps(sim, n_start, nsteps):
+def time_steps(sim, n_start, nsteps, per_kernel):
     out = np.zeros(3)
-    sim._lib.emf_time_steps(sim._h, n_start, nsteps, sim.Tn, out.ctypes.data_as(
-        __import__("ctypes").c_void_p))
+    sim._lib.emf_time_steps(sim._h, n_start, nsteps, sim.Tn, 1 if per_kernel else 0,
+                            out.ctypes.data_as(ctypes.c_void_p))
     return out
 
 
-def cpu_baseline(case, seconds):
-    """EMopt's own kernel (oracle/_ref, compiled from src/fdtd.cpp) driven by
-    the oracle's restatement of the Python time loop, one thread."""
+def _oracle_sim(case, path):
     from oracle import driver as D
     from tests.cases import _configure
-    kind = "reference" if os.path.exists(D.REF_LIB) else "port"
-    lib = D.load(D.REF_LIB if kind == "reference" else D.ORACLE_LIB)
+    lib = D.load(path)
     d = case["d"]
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
@@ -80,6 +93,10 @@ def cpu_baseline(case, seconds):
     for arrs, i0, j0, k0, sI, sJ, sK in o.sources:
         o.lib.FDTD_add_source(o.h, *arrs, i0, j0, k0, sI, sJ, sK, False)
     o.lib.FDTD_reset_pml(o.h)
+    return o
+
+
+def _time_cpu(o, seconds):
     o._step(0)   # warm-up (page-in)
     n, t0 = 1, time.perf_counter()
     while True:
@@ -87,14 +104,130 @@ def cpu_baseline(case, seconds):
         n += 1
         el = time.perf_counter() - t0
         if el >= seconds and n >= 3:
-            break
-    steps = n - 1
-    cells = case["Nx"] * case["Ny"] * case["Nz"]
-    return dict(value=cells * steps / el, unit="cell-updates/s", cores=1, kind=kind,
-                sample="%d time steps of the %dx%dx%d workload (%.1f s), EMopt src/fdtd.cpp "
-                       "-O3 -march=x86-64-v3, 1 thread, Python time loop with one-rank ghost "
-                       "exchange" % (steps, case["Nx"], case["Ny"], case["Nz"], el))
+            return n - 1, el
 
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, seconds):
+    """EMopt's own kernel (oracle/_ref, compiled from src/fdtd.cpp) driven by
+    the oracle's restatement of the Python time loop, one thread."""
+    from oracle import driver as D
+    kind = "reference" if os.path.exists(D.REF_LIB) else "port"
+    o = _oracle_sim(case, D.REF_LIB if kind == "reference" else D.ORACLE_LIB)
+    steps, el = _time_cpu(o, seconds)
+    cells = case["Nx"] * case["Ny"] * case["Nz"]
+    src = "EMopt src/fdtd.cpp" if kind == "reference" else "C restatement of src/fdtd.cpp"
+    return dict(value=cells * steps / el, unit="cell-updates/s", cores=1, kind=kind,
+                sample="%d time steps of the %dx%dx%d workload (%.1f s), %s -O3 "
+                       "-march=x86-64-v3, 1 thread, Python time loop with one-rank ghost "
+                       "exchange; host CPU %s" % (steps, case["Nx"], case["Ny"], case["Nz"], el,
+                                                  src, _cpu_model()))
+
+
+def cpu_all_cores(case, seconds):
+    """The C restatement built with -fopenmp (cell loops over all threads,
+    bit-identical per cell to the serial build)."""
+    from oracle import driver as D
+    if not os.path.exists(D.ORACLE_OMP_LIB):
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    o = _oracle_sim(case, D.ORACLE_OMP_LIB)
+    steps, el = _time_cpu(o, seconds)
+    cells = case["Nx"] * case["Ny"] * case["Nz"]
+    return dict(value=cells * steps / el, unit="cell-updates/s", cores=threads, kind="port",
+                sample="%d time steps of the %dx%dx%d workload (%.1f s), oracle/cfdtd.c -O3 "
+                       "-fopenmp, %d threads; host CPU %s"
+                       % (steps, case["Nx"], case["Ny"], case["Nz"], el, threads, _cpu_model()))
+
+
+def read_traffic(path=TRAFFIC_FILE):
+    """Per-launch HBM bytes recorded by the counter runs (see --profile-summary)."""
+    out = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"^- (\w+): hbm_bytes_per_launch = ([0-9.eE+]+)", line.strip())
+                if m:
+                    out[m.group(1)] = float(m.group(2))
+    return out
+
+
+# --------------------------------------------------------- profile summaries
+
+def _short(name):
+    for k in list(KERNELS) + ["k_copy", "k_conv_check", "k_step_fused", "k_step_lds"]:
+        if k in name:
+            return k
+    return name.split("(")[0][:48]
+
+
+def _read_counters(path, counter):
+    """Average counter value per dispatch, by kernel, from a rocprofv3
+    counter_collection.csv."""
+    acc = {}
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            if row.get("Counter_Name") != counter:
+                continue
+            k = _short(row["Kernel_Name"])
+            s, n = acc.get(k, (0.0, 0))
+            acc[k] = (s + float(row["Counter_Value"]), n + 1)
+    return {k: s / n for k, (s, n) in acc.items()}
+
+
+def profile_summary(args):
+    lines = []
+    cmd = args.command or "(command not given)"
+    if args.stats:
+        lines += ["# Kernel durations, 400x200x120 step workload", "",
+                  "Command:", "", "    " + cmd, "",
+                  "| kernel | calls | avg us | min us | max us | total % |",
+                  "|---|---|---|---|---|---|"]
+        with open(args.stats) as f:
+            for row in csv.DictReader(f):
+                lines.append("| %s | %s | %.2f | %.2f | %.2f | %.1f |" % (
+                    _short(row["Name"]), row["Calls"], float(row["AverageNs"]) / 1e3,
+                    float(row["MinNs"]) / 1e3, float(row["MaxNs"]) / 1e3,
+                    float(row["Percentage"])))
+        cells = 400 * 200 * 120
+        lines += ["", "Algorithmic bytes per launch: k_pass_h %d B/cell, k_pass_e %d B/cell "
+                  "x %d cells." % (B_ALG_H, B_ALG_E, cells)]
+    if args.fetch and args.write:
+        fetch = _read_counters(args.fetch, "FETCH_SIZE")
+        write = _read_counters(args.write, "WRITE_SIZE")
+        ff = COPY_BYTES / (fetch["k_copy"] * 1024.0)
+        fw = COPY_BYTES / (write["k_copy"] * 1024.0)
+        cells = 400 * 200 * 120
+        lines += ["# HBM traffic per launch, 400x200x120 step workload", "",
+                  "Commands (one counter per run):", "", "    " + (args.fetch_command or "?"),
+                  "    " + (args.write_command or "?"), "",
+                  "Scale: k_copy moves %d B each way; FETCH_SIZE reads %.1f KiB and WRITE_SIZE "
+                  "%.1f KiB for it, so the factors are %.3f (fetch) and %.3f (write)."
+                  % (COPY_BYTES, fetch["k_copy"], write["k_copy"], ff, fw), "",
+                  "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | scaled read B | scaled write B | "
+                  "HBM B/launch | algorithmic B/launch |", "|---|---|---|---|---|---|---|"]
+        res = []
+        for k in sorted(set(fetch) & set(write)):
+            rd, wr = fetch[k] * 1024 * ff, write[k] * 1024 * fw
+            alg = KERNELS.get(k, 0) * cells if k in KERNELS else (2 * COPY_BYTES if k == "k_copy" else 0)
+            lines.append("| %s | %.0f | %.0f | %.4g | %.4g | %.4g | %s |" % (
+                k, fetch[k], write[k], rd, wr, rd + wr, "%.4g" % alg if alg else "-"))
+            res.append((k, rd + wr))
+        lines += [""] + ["- %s: hbm_bytes_per_launch = %.6g" % (k, t) for k, t in res]
+    print("\n".join(lines))
+
+
+# ---------------------------------------------------------------------- main
 
 def main():
     ap = argparse.ArgumentParser()
@@ -102,9 +235,20 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--omp-seconds", type=float, default=6.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--profile-summary", action="store_true")
+    ap.add_argument("--stats")
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--command")
+    ap.add_argument("--fetch-command")
+    ap.add_argument("--write-command")
     args = ap.parse_args()
+    if args.profile_summary:
+        profile_summary(args)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (single-GPU solver, no "
               "multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -119,25 +263,33 @@ def main():
     cells = case["Nx"] * case["Ny"] * case["Nz"]
     L = sim._lib
 
-    # warm-up, then the timed region (emf_time_steps synchronises on return)
+    # warm-up, then the timed region: exactly K steps, no per-launch events
     L.emf_reset_pml(sim._h)
     L.emf_zero_fields(sim._h)
-    time_steps(sim, 0, args.warmup)
+    time_steps(sim, 0, args.warmup, False)
     log("[bench] warm-up done")
     t0 = time.perf_counter()
-    dev = time_steps(sim, args.warmup, args.steps)
+    dev = time_steps(sim, args.warmup, args.steps, False)
     wall = time.perf_counter() - t0
     value = cells * args.steps / wall
-    ms_h = dev[1] / args.steps
-    ms_e = dev[2] / args.steps
-    log("[bench] %d steps: wall %.3fs, device %.3fs, H %.3f ms, E %.3f ms"
-        % (args.steps, wall, dev[0] / 1e3, ms_h, ms_e))
+    log("[bench] %d steps: wall %.3fs, device %.3fs" % (args.steps, wall, dev[0] / 1e3))
 
+    # per-kernel launch times, measured after the timed region
+    nk = min(args.steps, 200)
+    pk = time_steps(sim, args.warmup + args.steps, nk, True)
+    ms = {"k_pass_h": pk[1] / nk, "k_pass_e": pk[2] / nk}
+    log("[bench] per launch: k_pass_h %.3f ms, k_pass_e %.3f ms" % (ms["k_pass_h"], ms["k_pass_e"]))
     gbs = np.zeros(1)
-    L.emf_copy_bandwidth(0, 1 << 31, 10, gbs.ctypes.data_as(_lib.c_void_p))
+    L.emf_copy_bandwidth(0, COPY_BYTES, 10, gbs.ctypes.data_as(_lib.c_void_p))
 
-    ach_e = B_ALG_E * cells / (ms_e * 1e-3) / 1e9
-    ach_h = B_ALG_H * cells / (ms_h * 1e-3) / 1e9
+    traffic = read_traffic()
+    kernels = {}
+    for k, b in KERNELS.items():
+        ach = b * cells / (ms[k] * 1e-3) / 1e9
+        kernels[k] = {"avg_ms": ms[k], "alg_bytes_per_launch": b * cells, "GBps": ach,
+                      "hbm_bytes_per_launch": traffic.get(k)}
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
+    ach = kernels[dom]["GBps"]
     res = {
         "metric": "Yee cell-updates/s (FP64 CW-FDTD, 400x200x120)",
         "value": value,
@@ -154,13 +306,15 @@ def main():
         "config": {"workload": "single_large_400x200x120_dx0.03_cpml12_ellipses40_ey_plane",
                    "grid": [case["Nx"], case["Ny"], case["Nz"]], "cells": cells,
                    "cpml_cells": 12, "wavelength_um": case["wl"], "Tn": sim.Tn,
-                   "bytes_per_cell_update": B_ALG_STEP},
+                   "bytes_per_cell_update": B_ALG_STEP,
+                   "step_kernels": "k_pass_h + k_pass_e (two launches per step)"},
         "hbm_frac_168B": value * B_ALG_STEP / (PEAK_HBM_GBS * 1e9),
-        "roofline": {"bound": "hbm", "kernel": "k_update_e (E pass)", "achieved": ach_e,
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach_e / PEAK_HBM_GBS,
-                     "traffic": None},
-        "kernels": {"k_update_h": {"avg_ms": ms_h, "alg_bytes": B_ALG_H * cells, "GBps": ach_h},
-                    "k_update_e": {"avg_ms": ms_e, "alg_bytes": B_ALG_E * cells, "GBps": ach_e}},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+                     "traffic": traffic.get(dom),
+                     "alg_bytes_per_launch": kernels[dom]["alg_bytes_per_launch"],
+                     "avg_launch_ms": kernels[dom]["avg_ms"]},
+        "kernels": kernels,
         "copy_GBps": float(gbs[0]),
     }
 
@@ -177,9 +331,15 @@ def main():
         log("[bench] full solve: %d steps in %.2fs (%s)" % (ls["steps"], el, ls["status"]))
 
     if not args.no_cpu:
+        del sim
         cb = cpu_baseline(case, args.cpu_seconds)
         res["cpu_baseline"] = cb
         log("[bench] cpu baseline %.3g cell-updates/s" % cb["value"])
+        if args.omp_seconds > 0:
+            co = cpu_all_cores(case, args.omp_seconds)
+            if co:
+                res["cpu_all_cores"] = co
+                log("[bench] cpu all cores %.3g cell-updates/s" % co["value"])
     print(json.dumps(res), flush=True)
 
 

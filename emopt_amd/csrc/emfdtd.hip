@@ -145,6 +145,7 @@ struct emf_solver {
     int bc[3] = { '0', '0', '0' };
     int nconv = 10;
     int zc = 2;
+    int order = 0;                 // two-pass work-group order (pass_tile)
 
     // device state.  E/H (and the H-pass psi set) exist twice for the fused
     // step, which reads one set and writes the other; E, H and psi[0] always
@@ -207,6 +208,7 @@ struct emf_solver {
         for (int q = 0; q < 6; ++q) g.w[q] = w[q];
         for (int q = 0; q < 3; ++q) g.bc[q] = bc[q];
         g.zc = zc;
+        g.order = order;
         return g;
     }
 };
@@ -527,9 +529,42 @@ FusedArgs make_args(const emf_solver *s, int from, int to)
     return A;
 }
 
+// rows per work-group of the two passes (tyg of a fused kind may be 16)
+int pass_ty(const emf_solver *s)
+{
+    return s->tyg == 1 || s->tyg == 2 || s->tyg == 8 ? s->tyg : 4;
+}
+
 dim3 pass_grid(const emf_solver *s)
 {
-    return dim3((s->Nx + PX - 1) / PX, (s->Ny + 3) / 4, (s->Nz + s->zc - 1) / s->zc);
+    const int ty = pass_ty(s);
+    return dim3((s->Nx + PX - 1) / PX, (s->Ny + ty - 1) / ty, (s->Nz + s->zc - 1) / s->zc);
+}
+
+template <bool MU>
+void launch_h_t(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
+                const int *done)
+{
+    const dim3 gr = pass_grid(s);
+    switch (pass_ty(s)) {
+    case 1: hipLaunchKernelGGL((k_pass_h<MU, 1>), gr, dim3(PX, 1), 0, s->stream, A, g, ps, done); break;
+    case 2: hipLaunchKernelGGL((k_pass_h<MU, 2>), gr, dim3(PX, 2), 0, s->stream, A, g, ps, done); break;
+    case 8: hipLaunchKernelGGL((k_pass_h<MU, 8>), gr, dim3(PX, 8), 0, s->stream, A, g, ps, done); break;
+    default: hipLaunchKernelGGL((k_pass_h<MU, 4>), gr, dim3(PX, 4), 0, s->stream, A, g, ps, done);
+    }
+}
+
+template <bool CE>
+void launch_e_t(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
+                const int *done)
+{
+    const dim3 gr = pass_grid(s);
+    switch (pass_ty(s)) {
+    case 1: hipLaunchKernelGGL((k_pass_e<CE, 1>), gr, dim3(PX, 1), 0, s->stream, A, g, ps, done); break;
+    case 2: hipLaunchKernelGGL((k_pass_e<CE, 2>), gr, dim3(PX, 2), 0, s->stream, A, g, ps, done); break;
+    case 8: hipLaunchKernelGGL((k_pass_e<CE, 8>), gr, dim3(PX, 8), 0, s->stream, A, g, ps, done); break;
+    default: hipLaunchKernelGGL((k_pass_e<CE, 4>), gr, dim3(PX, 4), 0, s->stream, A, g, ps, done);
+    }
 }
 
 // H pass in place on the current set (update_H, src/fdtd.cpp:121-367)
@@ -541,10 +576,8 @@ int launch_h(emf_solver *s, double t, const int *done)
     PassSrc ps = pass_sources(s, 3, t);
     FusedArgs A = make_args(s, s->cur, s->cur);
     Geo g = s->geo();
-    if (s->mu_uniform)
-        hipLaunchKernelGGL((k_pass_h<false, 4>), pass_grid(s), dim3(PX, 4), 0, s->stream, A, g, ps, done);
-    else
-        hipLaunchKernelGGL((k_pass_h<true, 4>), pass_grid(s), dim3(PX, 4), 0, s->stream, A, g, ps, done);
+    if (s->mu_uniform) launch_h_t<false>(s, A, g, ps, done);
+    else launch_h_t<true>(s, A, g, ps, done);
     HIPCHK(hipGetLastError());
     return EMF_OK;
 }
@@ -558,18 +591,38 @@ int launch_e(emf_solver *s, double t, const int *done)
     PassSrc ps = pass_sources(s, 0, t);
     FusedArgs A = make_args(s, s->cur, s->cur);
     Geo g = s->geo();
-    if (s->ceps)
-        hipLaunchKernelGGL((k_pass_e<true, 4>), pass_grid(s), dim3(PX, 4), 0, s->stream, A, g, ps, done);
-    else
-        hipLaunchKernelGGL((k_pass_e<false, 4>), pass_grid(s), dim3(PX, 4), 0, s->stream, A, g, ps, done);
+    if (s->ceps) launch_e_t<true>(s, A, g, ps, done);
+    else launch_e_t<false>(s, A, g, ps, done);
     HIPCHK(hipGetLastError());
     return EMF_OK;
+}
+
+template <bool MU, bool CE>
+void launch_lds_t(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &sh,
+                  const PassSrc &se, const int *done)
+{
+    const int ty = s->tyg < 4 ? 4 : s->tyg;
+    dim3 grid((s->Nx + FXO - 1) / FXO, (s->Ny + ty - 2) / (ty - 1), (s->Nz + s->zc - 1) / s->zc);
+    switch (ty) {
+    case 16:
+        hipLaunchKernelGGL((k_step_lds<MU, CE, 16>), grid, dim3(FX, 16), 0, s->stream, A, g, sh, se, done);
+        break;
+    case 8:
+        hipLaunchKernelGGL((k_step_lds<MU, CE, 8>), grid, dim3(FX, 8), 0, s->stream, A, g, sh, se, done);
+        break;
+    default:
+        hipLaunchKernelGGL((k_step_lds<MU, CE, 4>), grid, dim3(FX, 4), 0, s->stream, A, g, sh, se, done);
+    }
 }
 
 template <bool MU, bool CE>
 void launch_fused_t(const emf_solver *s, dim3 grid, const FusedArgs &A, const Geo &g,
                     const PassSrc &sh, const PassSrc &se, const int *done)
 {
+    if (s->kind == 2) {
+        launch_lds_t<MU, CE>(s, A, g, sh, se, done);
+        return;
+    }
     switch (s->tyg) {
     case 1:
         hipLaunchKernelGGL((k_step_fused<MU, CE, 1>), grid, dim3(FX, 1), 0, s->stream, A, g, sh, se, done);
@@ -1382,29 +1435,37 @@ int emf_grad_assemble(emf_solver *s, int64_t n, const int32_t *pidx, const int32
 }
 
 int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check_every,
-                   double *out)
+                   int per_kernel, double *out)
 {
     LOCK(s);
     if (!out || nsteps < 0 || n_start < 0) EMF_FAIL(EMF_ERR_ARG, "bad arguments");
     CHECK(check_ready(s));
     if (check_every > 0) CHECK(build_samples(s));
     if (fused(s)) CHECK(ensure_alt(s));
-    std::vector<hipEvent_t> ev(2 * nsteps + 1);
+    // ev[0] start, ev[1] end; with per_kernel, ev[2+2q] / ev[3+2q] bracket the
+    // H pass (or the fused launch) and the E pass of step q.
+    std::vector<hipEvent_t> ev(per_kernel ? 2 * nsteps + 3 : 2);
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipMemsetAsync(s->conv, 0, sizeof(ConvState), s->stream));
     HIPCHK(hipEventRecord(ev[0], s->stream));
     const int start = s->cur;
     int rc = EMF_OK;
+    auto mark = [&](size_t idx) {
+        if (rc == EMF_OK && per_kernel)
+            rc = hipEventRecord(ev[idx], s->stream) == hipSuccess ? EMF_OK : EMF_ERR_HIP;
+    };
     for (int64_t q = 0; q < nsteps && rc == EMF_OK; ++q) {
         long long n = n_start + q;
+        mark(2 + 2 * q);
         if (fused(s)) {
             rc = launch_fused(s, n, from_set(s, n, n_start, start), nullptr);
-            if (rc == EMF_OK) rc = hipEventRecord(ev[2 * q + 1], s->stream) == hipSuccess ? EMF_OK : EMF_ERR_HIP;
+            mark(3 + 2 * q);
         } else {
             rc = launch_h(s, (double)n * s->dt, nullptr);
-            if (rc == EMF_OK) rc = hipEventRecord(ev[2 * q + 1], s->stream) == hipSuccess ? EMF_OK : EMF_ERR_HIP;
+            mark(3 + 2 * q);
             if (rc == EMF_OK) rc = launch_e(s, ((double)n + 0.5) * s->dt, nullptr);
         }
+        mark(4 + 2 * q);
         if (rc == EMF_OK && check_every > 0 && (q + 1) % check_every == 0) {
             const int ps = from_set(s, n + 1, n_start, start);
             hipLaunchKernelGGL(k_conv_check, dim3(1), dim3(64), 0, s->stream, s->Eb[ps][0],
@@ -1412,18 +1473,18 @@ int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check
                                (double)n * s->dt, 1e-300, 1e-300, n, s->conv, (double *)nullptr, 0,
                                (int *)nullptr);
         }
-        if (rc == EMF_OK) rc = hipEventRecord(ev[2 * q + 2], s->stream) == hipSuccess ? EMF_OK : EMF_ERR_HIP;
     }
+    if (rc == EMF_OK) rc = hipEventRecord(ev[1], s->stream) == hipSuccess ? EMF_OK : EMF_ERR_HIP;
     if (rc == EMF_OK) set_current(s, from_set(s, n_start + nsteps, n_start, start));
-    if (rc == EMF_OK && hipEventSynchronize(ev[2 * nsteps]) != hipSuccess) rc = EMF_ERR_HIP;
+    if (rc == EMF_OK && hipEventSynchronize(ev[1]) != hipSuccess) rc = EMF_ERR_HIP;
     double tot = 0, th = 0, te = 0;
     if (rc == EMF_OK) {
         float ms;
-        (void)hipEventElapsedTime(&ms, ev[0], ev[2 * nsteps]);
+        (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
         tot = ms;
-        for (int64_t q = 0; q < nsteps; ++q) {
-            (void)hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]); th += ms;
-            (void)hipEventElapsedTime(&ms, ev[2 * q + 1], ev[2 * q + 2]); te += ms;
+        for (int64_t q = 0; per_kernel && q < nsteps; ++q) {
+            (void)hipEventElapsedTime(&ms, ev[2 + 2 * q], ev[3 + 2 * q]); th += ms;
+            (void)hipEventElapsedTime(&ms, ev[3 + 2 * q], ev[4 + 2 * q]); te += ms;
         }
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
@@ -1432,15 +1493,20 @@ int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check
     return EMF_OK;
 }
 
-int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc)
+int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc, int order)
 {
     LOCK(s);
-    if (kind != 0 && kind != 1) EMF_FAIL(EMF_ERR_ARG, "kind must be 0 (fused) or 1 (two passes)");
-    if (tyg != 1 && tyg != 2 && tyg != 4 && tyg != 8) EMF_FAIL(EMF_ERR_ARG, "tyg must be 1, 2, 4 or 8");
+    if (kind < 0 || kind > 2)
+        EMF_FAIL(EMF_ERR_ARG, "kind must be 0 (fused, row recompute), 1 (two passes) or 2 (fused, LDS)");
+    const bool ok = kind == 2 ? (tyg == 4 || tyg == 8 || tyg == 16)
+                              : (tyg == 1 || tyg == 2 || tyg == 4 || tyg == 8);
+    if (!ok) EMF_FAIL(EMF_ERR_ARG, "tyg must be 1, 2, 4 or 8 (kinds 0, 1) or 4, 8, 16 (kind 2)");
     if (zc < 1) EMF_FAIL(EMF_ERR_ARG, "zc must be >= 1");
+    if (order < 0 || order > 2) EMF_FAIL(EMF_ERR_ARG, "order must be 0, 1 or 2");
     s->kind = kind;
     s->tyg = tyg;
     s->zc = zc;
+    s->order = order;
     return EMF_OK;
 }
 
@@ -1449,6 +1515,7 @@ int emf_copy_bandwidth(int device, int64_t bytes, int reps, double *gbps)
     if (!gbps || bytes < 1024 || reps < 1) EMF_FAIL(EMF_ERR_ARG, "bad arguments");
     HIPCHK(hipSetDevice(device));
     long long n = bytes / (long long)sizeof(double2);
+    if ((n + 255) / 256 > 0x7fffffffLL) EMF_FAIL(EMF_ERR_ARG, "bytes too large");
     double2 *a = nullptr, *b = nullptr;
     CHECK(dalloc(&a, n));
     if (dalloc(&b, n) != EMF_OK) { dfree(a); return EMF_ERR_NOMEM; }
@@ -1459,7 +1526,7 @@ int emf_copy_bandwidth(int device, int64_t bytes, int reps, double *gbps)
     hipEventCreate(&e0); hipEventCreate(&e1);
     hipMemsetAsync(a, 0, n * sizeof(double2), st);
     std::vector<float> t;
-    unsigned grid = (unsigned)std::min<long long>((n + 255) / 256, 256LL * 64);
+    const unsigned grid = (unsigned)((n + 255) / 256);
     for (int r = 0; r < reps + 1 && rc == EMF_OK; ++r) {
         hipEventRecord(e0, st);
         hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, st, a, b, n);

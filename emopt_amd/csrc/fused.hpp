@@ -431,3 +431,190 @@ __global__ __launch_bounds__(FX * TY) void k_step_fused(FusedArgs A, Geo g, Pass
 }
 
 }  // namespace emf
+
+namespace emf {
+
+// ---------------------------------------------------------------------------
+// Fused step, LDS variant: a 64 x TY work-group owns 63 x (TY-1) cells of a
+// tile (lane 0 = halo column k0-1, wave 0 = halo row j0-1; both compute H_new
+// only).  H_new of row j-1 comes from LDS (one barrier per plane, buffers
+// alternate by plane parity), H_new of k-1 from the neighbouring lane, H_new
+// of i-1 from registers.  CPML coefficients and psi are loaded inside the slab
+// branches, which keeps the live register set small (higher occupancy).
+template <bool MU_ARRAY, bool CEPS, int TY>
+__global__ __launch_bounds__(FX * TY) void k_step_lds(FusedArgs A, Geo g, PassSrc sh, PassSrc se,
+                                                     const int *__restrict__ done)
+{
+    if (done && *done) return;
+    __shared__ double lds[2][TY][FX][2];   // (Hx, Hz) of H_new per row
+    const Layout L = make_layout(g);
+    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    const unsigned nbx = gridDim.x, nby = gridDim.y;
+    const unsigned l = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    const int bx = (int)(l % nbx), by = (int)((l / nbx) % nby), bz = (int)(l / (nbx * nby));
+    const int lane = threadIdx.x, row = threadIdx.y;
+    const int k = bx * FXO - 1 + lane, j = by * (TY - 1) - 1 + row;
+    const int z0 = bz * g.zc, z1 = min(z0 + g.zc, Nz);
+    const bool hcell = k >= 0 && k < Nx && j >= 0 && j < Ny;
+    const bool own = hcell && lane >= 1 && row >= 1;
+    const bool jn = j + 1 < Ny, kn = k + 1 < Nx;
+    const unsigned cs = A.cs, cs2 = 2 * A.cs;
+    const unsigned oz = (unsigned)g.nxy * 8u, oy = (unsigned)Nx * 8u;
+    const unsigned o0 = hcell ? ((unsigned)j * (unsigned)Nx + (unsigned)k) * 8u : 0u;
+    bool tile_src = false;
+    {
+        const int tk0 = bx * FXO - 1, tk1 = tk0 + FX, tj0 = by * (TY - 1) - 1, tj1 = tj0 + TY;
+        const int tz0 = z0 > 0 ? z0 - 1 : z0;
+        for (int q = 0; q < sh.n; ++q) {
+            const SrcBlk &s = sh.blk[q];
+            if (s.k0 < tk1 && s.k0 + s.K > tk0 && s.j0 < tj1 && s.j0 + s.J > tj0 && s.i0 < z1 &&
+                s.i0 + s.I > tz0)
+                tile_src = true;
+        }
+    }
+    const int zs = z0 > 0 ? z0 - 1 : z0;
+    double ex = 0, ey = 0, ez = 0;
+    if (hcell) {
+        const unsigned o = (unsigned)zs * oz + o0;
+        ex = ldo(A.Eo, o); ey = ldo(A.Eo, o + cs); ez = ldo(A.Eo, o + cs2);
+    }
+    double hxm = 0, hym = 0;
+    for (int i = zs; i < z1; ++i) {
+        const unsigned o = (unsigned)i * oz + o0;
+        const bool doE = own && i >= z0;
+        // ---- loads of this plane (PML data is loaded in the slab branches)
+        double exi = 0, eyi = 0, ezi = 0, ezj = 0, exj = 0, hox = 0, hoy = 0, hoz = 0;
+        double bex = 0, bey = 0, bez = 0;
+        if (hcell) {
+            if (i + 1 < Nz) { exi = ldo(A.Eo, o + oz); eyi = ldo(A.Eo, o + oz + cs); ezi = ldo(A.Eo, o + oz + cs2); }
+            if (jn) { ezj = ldo(A.Eo, o + oy + cs2); exj = ldo(A.Eo, o + oy); }
+            hox = ldo(A.Ho, o); hoy = ldo(A.Ho, o + cs); hoz = ldo(A.Ho, o + cs2);
+        }
+        if (doE) { bex = ldo(A.bE, o); bey = ldo(A.bE, o + cs); bez = ldo(A.bE, o + cs2); }
+        double ezk = __shfl_down(ez, 1), eyk = __shfl_down(ey, 1);
+        if (lane == FX - 1) {
+            ezk = 0; eyk = 0;
+            if (hcell && kn) { ezk = ldo(A.Eo, o + 8u + cs2); eyk = ldo(A.Eo, o + 8u + cs); }
+        }
+        // ---- H_new(i, j, k)
+        double hx = 0, hy = 0, hz = 0;
+        if (hcell) {
+            const double mx = MU_ARRAY ? ldo(A.mH, o) : A.m_s;
+            const double my = MU_ARRAY ? ldo(A.mH, o + cs) : A.m_s;
+            const double mz = MU_ARRAY ? ldo(A.mH, o + cs2) : A.m_s;
+            const double dEzdy = g.ry * (ezj - ez);
+            const double dEydz = g.rz * (eyi - ey);
+            hx = hox + mx * (dEydz - dEzdy);
+            const double dExdz = g.rz * (exi - ex);
+            const double dEzdx = g.rx * (ezk - ez);
+            hy = hoy + my * (dEzdx - dExdz);
+            const double dEydx = g.rx * (eyk - ey);
+            const double dExdy = g.ry * (exj - ex);
+            hz = hoz + mz * (dExdy - dEydx);
+            const bool wpsi = doE;
+            const Slab sx = slab_x(g, i, j, k);
+            if (sx.in) {
+                const unsigned f = sx.lo ? 0 : 1, t = L.tbo[0] + (unsigned)sx.par * 8u;
+                const double kap = ldo(A.tab, t), b = ldo(A.tab, t + L.tbl[0]), c = ldo(A.tab, t + 2 * L.tbl[0]);
+                const double p0 = c * dEydx + b * ldo(A.pho, L.pso[f][0] + sx.io);
+                const double p1 = c * dEzdx + b * ldo(A.pho, L.pso[f][1] + sx.io);
+                if (wpsi) { sto(A.phn, L.pso[f][0] + sx.io, p0); sto(A.phn, L.pso[f][1] + sx.io, p1); }
+                hz = hz - mz * (p0 - dEydx + dEydx / kap);
+                hy = hy + my * (p1 - dEzdx + dEzdx / kap);
+            }
+            const Slab sy = slab_y(g, i, j, k);
+            if (sy.in) {
+                const unsigned f = sy.lo ? 2 : 3, t = L.tbo[1] + (unsigned)sy.par * 8u;
+                const double kap = ldo(A.tab, t), b = ldo(A.tab, t + L.tbl[1]), c = ldo(A.tab, t + 2 * L.tbl[1]);
+                const double p0 = c * dExdy + b * ldo(A.pho, L.pso[f][0] + sy.io);
+                const double p1 = c * dEzdy + b * ldo(A.pho, L.pso[f][1] + sy.io);
+                if (wpsi) { sto(A.phn, L.pso[f][0] + sy.io, p0); sto(A.phn, L.pso[f][1] + sy.io, p1); }
+                hz = hz + mz * (p0 - dExdy + dExdy / kap);
+                hx = hx - mx * (p1 - dEzdy + dEzdy / kap);
+            }
+            const Slab sz = slab_z(g, i, j, k);
+            if (sz.in) {
+                const unsigned f = sz.lo ? 4 : 5, t = L.tbo[2] + (unsigned)sz.par * 8u;
+                const double kap = ldo(A.tab, t), b = ldo(A.tab, t + L.tbl[2]), c = ldo(A.tab, t + 2 * L.tbl[2]);
+                const double p0 = c * dExdz + b * ldo(A.pho, L.pso[f][0] + sz.io);
+                const double p1 = c * dEydz + b * ldo(A.pho, L.pso[f][1] + sz.io);
+                if (wpsi) { sto(A.phn, L.pso[f][0] + sz.io, p0); sto(A.phn, L.pso[f][1] + sz.io, p1); }
+                hx = hx + mx * (p1 - dEydz + dEydz / kap);
+                hy = hy - my * (p0 - dExdz + dExdz / kap);
+            }
+            if (tile_src) h_sources<MU_ARRAY>(g, sh, A, i, j, k, o, hx, hy, hz);
+        }
+        const int par = i & 1;
+        lds[par][row][lane][0] = hx;
+        lds[par][row][lane][1] = hz;
+        __syncthreads();
+        double hz_k = __shfl_up(hz, 1), hy_k = __shfl_up(hy, 1);
+        if (doE) {
+            const double sgx = g.bc[0] == 'E' ? -1.0 : (g.bc[0] == 'H' ? 1.0 : 0.0);
+            const double sgy = g.bc[1] == 'E' ? -1.0 : (g.bc[1] == 'H' ? 1.0 : 0.0);
+            const double sgz = g.bc[2] == 'E' ? -1.0 : (g.bc[2] == 'H' ? 1.0 : 0.0);
+            double hx_j = lds[par][row - 1][lane][0], hz_j = lds[par][row - 1][lane][1];
+            if (j == 0) { hx_j = g.bc[1] == '0' ? 0.0 : sgy * hx; hz_j = g.bc[1] == '0' ? 0.0 : sgy * hz; }
+            if (k == 0) { hz_k = g.bc[0] == '0' ? 0.0 : sgx * hz; hy_k = g.bc[0] == '0' ? 0.0 : sgx * hy; }
+            double hx_m = hxm, hy_m = hym;
+            if (i == 0) { hx_m = g.bc[2] == '0' ? 0.0 : sgz * hx; hy_m = g.bc[2] == '0' ? 0.0 : sgz * hy; }
+            const double dHzdy = g.ry * (hz - hz_j);
+            const double dHydz = g.rz * (hy - hy_m);
+            double nex = CEPS ? ldo(A.aE, o) * ex + (dHzdy - dHydz) * bex : ex + (dHzdy - dHydz) * bex;
+            const double dHxdz = g.rz * (hx - hx_m);
+            const double dHzdx = g.rx * (hz - hz_k);
+            double ney = CEPS ? ldo(A.aE, o + cs) * ey + (dHxdz - dHzdx) * bey : ey + (dHxdz - dHzdx) * bey;
+            const double dHydx = g.rx * (hy - hy_k);
+            const double dHxdy = g.ry * (hx - hx_j);
+            double nez = CEPS ? ldo(A.aE, o + cs2) * ez + (dHydx - dHxdy) * bez : ez + (dHydx - dHxdy) * bez;
+            const Slab sx = slab_x(g, i, j, k);
+            if (sx.in) {
+                const unsigned f = sx.lo ? 0 : 1, t = L.tbo[0] + 3 * L.tbl[0] + (unsigned)sx.par * 8u;
+                const double kap = ldo(A.tab, t), b = ldo(A.tab, t + L.tbl[0]), c = ldo(A.tab, t + 2 * L.tbl[0]);
+                const double q0 = c * dHydx + b * ldo(A.pe, L.pso[f][0] + sx.io);
+                const double q1 = c * dHzdx + b * ldo(A.pe, L.pso[f][1] + sx.io);
+                sto(A.pe, L.pso[f][0] + sx.io, q0); sto(A.pe, L.pso[f][1] + sx.io, q1);
+                nez = nez + (q0 - dHydx + dHydx / kap) * bez;
+                ney = ney - (q1 - dHzdx + dHzdx / kap) * bey;
+            }
+            const Slab sy = slab_y(g, i, j, k);
+            if (sy.in) {
+                const unsigned f = sy.lo ? 2 : 3, t = L.tbo[1] + 3 * L.tbl[1] + (unsigned)sy.par * 8u;
+                const double kap = ldo(A.tab, t), b = ldo(A.tab, t + L.tbl[1]), c = ldo(A.tab, t + 2 * L.tbl[1]);
+                const double q0 = c * dHxdy + b * ldo(A.pe, L.pso[f][0] + sy.io);
+                const double q1 = c * dHzdy + b * ldo(A.pe, L.pso[f][1] + sy.io);
+                sto(A.pe, L.pso[f][0] + sy.io, q0); sto(A.pe, L.pso[f][1] + sy.io, q1);
+                nez = nez - (q0 - dHxdy + dHxdy / kap) * bez;
+                nex = nex + (q1 - dHzdy + dHzdy / kap) * bex;
+            }
+            const Slab sz = slab_z(g, i, j, k);
+            if (sz.in) {
+                const unsigned f = sz.lo ? 4 : 5, t = L.tbo[2] + 3 * L.tbl[2] + (unsigned)sz.par * 8u;
+                const double kap = ldo(A.tab, t), b = ldo(A.tab, t + L.tbl[2]), c = ldo(A.tab, t + 2 * L.tbl[2]);
+                const double q0 = c * dHxdz + b * ldo(A.pe, L.pso[f][0] + sz.io);
+                const double q1 = c * dHydz + b * ldo(A.pe, L.pso[f][1] + sz.io);
+                sto(A.pe, L.pso[f][0] + sz.io, q0); sto(A.pe, L.pso[f][1] + sz.io, q1);
+                nex = nex - (q1 - dHydz + dHydz / kap) * bex;
+                ney = ney + (q0 - dHxdz + dHxdz / kap) * bey;
+            }
+            if (tile_src) {
+                for (int q = 0; q < se.n; ++q) {
+                    long long m;
+                    const SrcBlk &sb = se.blk[q];
+                    if (in_box(sb, i, j, k, m)) {
+                        const double2 vx = sb.f[0][m], vy = sb.f[1][m], vz = sb.f[2][m];
+                        nex = nex - src_val(se, vx) * bex;
+                        ney = ney - src_val(se, vy) * bey;
+                        nez = nez - src_val(se, vz) * bez;
+                    }
+                }
+            }
+            sto(A.Hn, o, hx); sto(A.Hn, o + cs, hy); sto(A.Hn, o + cs2, hz);
+            sto(A.En, o, nex); sto(A.En, o + cs, ney); sto(A.En, o + cs2, nez);
+        }
+        hxm = hx; hym = hy;
+        ex = exi; ey = eyi; ez = ezi;
+    }
+}
+
+}  // namespace emf

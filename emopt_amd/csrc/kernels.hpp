@@ -35,6 +35,7 @@ struct Geo {
     int w[6];            // CPML widths xmin xmax ymin ymax zmin zmax
     int bc[3];           // '0', 'E', 'H' per axis x, y, z
     int zc;              // z planes per work-group
+    int order;           // work-group order of the two passes (see pass_tile)
 };
 
 // CPML coefficient tables of one pass (H uses the *_H_* set, E the *_E_*).
@@ -328,11 +329,12 @@ __global__ void k_scatter_box(double2 *__restrict__ dst, const double2 *__restri
     }
 }
 
-// plain vector copy (bandwidth reference)
+// plain vector copy, one 16-byte element per thread (bandwidth reference;
+// on MI355X this form streams faster than a grid-stride loop)
 __global__ void k_copy(const double2 *__restrict__ a, double2 *__restrict__ b, long long n)
 {
-    long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    for (; g < n; g += (long long)gridDim.x * blockDim.x) b[g] = a[g];
+    const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (g < n) b[g] = a[g];
 }
 
 // ------------------------------------------------------ gradient assembly

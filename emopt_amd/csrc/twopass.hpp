@@ -8,7 +8,7 @@
 // (zc = 1 is one cell per thread).  Every load a plane needs -- fields,
 // coefficients, psi and CPML table entries -- is issued before any store, so
 // a thread waits for memory once per plane; in-plane neighbours come through
-// L1/L2.  Work-groups are numbered x fastest, then y, then z (plain order).
+// L1/L2.
 #pragma once
 #include "fused.hpp"
 
@@ -16,13 +16,29 @@ namespace emf {
 
 constexpr int PX = 64;
 
+// Tile of work-group blockIdx.  order 0: x fastest, then y, then z.  order 1:
+// z fastest, then x, then y.  order 2: as 1, after regrouping the linear id
+// by XCD (the dispatcher deals consecutive work-groups to the 8 XCDs in turn),
+// so that z-neighbouring tiles, which share a halo plane, run on one XCD and
+// share its L2.  The mapping is a bijection for any work-group count.
 __device__ __forceinline__ void pass_tile(const Geo &g, int ty, int &k, int &j, int &z0, int &z1)
 {
-    const unsigned nbx = gridDim.x, nby = gridDim.y;
-    const unsigned l = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
-    k = (int)((l % nbx) * PX + threadIdx.x);
-    j = (int)(((l / nbx) % nby) * ty + threadIdx.y);
-    z0 = (int)(l / (nbx * nby)) * g.zc;
+    const unsigned nbx = gridDim.x, nby = gridDim.y, nbz = gridDim.z;
+    unsigned l = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    unsigned bx, by, bz;
+    if (g.order == 0) {
+        bx = l % nbx; by = (l / nbx) % nby; bz = l / (nbx * nby);
+    } else {
+        if (g.order == 2) {
+            const unsigned nb = nbx * nby * nbz, q = nb / 8, r = nb % 8;
+            const unsigned xcd = l % 8, slot = l / 8;
+            l = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot;
+        }
+        bz = l % nbz; bx = (l / nbz) % nbx; by = l / (nbz * nbx);
+    }
+    k = (int)(bx * PX + threadIdx.x);
+    j = (int)(by * ty + threadIdx.y);
+    z0 = (int)bz * g.zc;
     z1 = min(z0 + g.zc, g.Nz);
 }
 

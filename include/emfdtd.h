@@ -229,16 +229,20 @@ EMF_API int emf_get_field(emf_solver *s, int slot, int comp, int k1, int k2, int
  * state (sources, CPML and fields as they are), with a convergence check
  * every `check_every` steps (0 = none), exactly as emf_solve's loop does, and
  * times them with HIP events on the handle's stream.  out[0] = total device
- * ms of the region, out[1] = summed ms of the H-pass launches, out[2] =
- * summed ms of the E-pass launches. */
+ * ms of the region.  With per_kernel != 0 an event pair also brackets every
+ * launch: out[1] = summed ms of the H-pass (or fused-step) launches, out[2] =
+ * summed ms of the E-pass launches; otherwise both are 0. */
 EMF_API int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check_every,
-                           double *out);
+                           int per_kernel, double *out);
 
-/* Step kernel selection: kind 0 = one fused H+E launch per step, 1 = separate
- * H and E passes (default); tyg = rows (wavefronts) per work-group of the
- * fused kernel (1, 2, 4 or 8); zc = z planes per work-group.  Results are
+/* Step kernel selection: kind 1 = separate H and E passes (default), 0 = one
+ * fused H+E launch per step recomputing the H row it needs, 2 = one fused
+ * launch sharing H through LDS; tyg = rows (wavefronts) per work-group (1, 2,
+ * 4 or 8; 4, 8 or 16 for kind 2); zc = z planes per work-group; order =
+ * work-group order of the two passes (0 x-fastest, 1 z-fastest, 2 z-fastest
+ * grouped by XCD).  Defaults kind 1, tyg 4, zc 2, order 0.  Results are
  * identical; only speed differs. */
-EMF_API int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc);
+EMF_API int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc, int order);
 
 /* Device copy bandwidth (GB/s, read + write bytes) of a plain vector copy of
  * `bytes` bytes, median of `reps` launches; a roofline reference point. */

@@ -265,6 +265,10 @@ static void h_pass(cfdtd *s, int n, double t)
             Ex[p] = 0.0; Ey[p] = 0.0;
         }
 
+    /* Cells are independent within a pass (each writes only its own H and
+     * psi entries), so the z loop may run on several threads (built with
+     * -fopenmp); the arithmetic per cell is unchanged. */
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < I; i++) for (int j = 0; j < J; j++) for (int k = 0; k < K; k++) {
         long p = pidx(s, i, j, k), g = gidx(s, i, j, k);
         double mx = s->dt / s->mu[0][g].re, my = s->dt / s->mu[1][g].re, mz = s->dt / s->mu[2][g].re;
@@ -361,6 +365,7 @@ static void e_pass(cfdtd *s, int n, double t)
             else if (s->bc[2] == 'H') { Hx[p] = Hx[p + si]; Hy[p] = Hy[p + si]; }
         }
 
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < I; i++) for (int j = 0; j < J; j++) for (int k = 0; k < K; k++) {
         long p = pidx(s, i, j, k), g = gidx(s, i, j, k);
         double ax, ay, az, bx, by, bz;

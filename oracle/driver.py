@@ -37,6 +37,7 @@ from numpy.ctypeslib import ndpointer
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "libcfdtd.so")
+ORACLE_OMP_LIB = os.path.join(HERE, "libcfdtd_omp.so")
 REF_LIB = os.path.join(HERE, "_ref", "libfdtd_ref.so")
 
 _c_complex_p = ndpointer(np.complex128, ndim=1, flags="C")

@@ -139,7 +139,7 @@ def test_gpu_matches_reference_golden_solve():
     assert max(errs) <= TOL, errs
 
 
-def _run_steps(case, nsteps, kind, tyg=4, zc=8):
+def _run_steps(case, nsteps, kind, tyg=4, zc=8, order=0):
     """nsteps time steps through emf_time_steps with the given step kernel."""
     import ctypes
     import warnings
@@ -155,18 +155,20 @@ def _run_steps(case, nsteps, kind, tyg=4, zc=8):
     sim.set_material_arrays(case["mats"][:3], case["mats"][3:])
     sim._register(sim._sources)
     L, h = sim._lib, sim._h
-    L.emf_set_tuning(h, kind, tyg, zc)
+    L.emf_set_tuning(h, kind, tyg, zc, order)
     L.emf_reset_pml(h)
     L.emf_zero_fields(h)
     out = np.zeros(3)
-    L.emf_time_steps(h, 0, nsteps, 0, out.ctypes.data_as(ctypes.c_void_p))
+    L.emf_time_steps(h, 0, nsteps, 0, 1, out.ctypes.data_as(ctypes.c_void_p))
     return sim.time_fields()
 
 
 @pytest.mark.parametrize("variant", ["plain", "ceps", "mu", "bcs", "manysrc"])
-def test_fused_step_matches_two_pass(variant):
-    """The fused H+E kernel (ping-pong buffers, halo row/lane recomputation)
-    against the two-pass kernels, for every work-group shape and z-chunk."""
+def test_step_kernels_agree(variant):
+    """Every step kernel and launch shape against the default two-pass one:
+    the fused H+E kernels (ping-pong buffers; kind 0 recomputes the halo row,
+    kind 2 shares it through LDS) for every work-group shape and z-chunk, and
+    the two passes for other row counts, z-chunks and work-group orders."""
     kw = dict(w=(5, 4, 3, 6, 4, 5), seed=31, nsrc=2, mu_rand=False)
     if variant == "ceps":
         kw["ceps"] = True
@@ -178,19 +180,29 @@ def test_fused_step_matches_two_pass(variant):
         kw["nsrc"] = 7
     case = random_case(70, 37, 23, **kw)
     ref = _run_steps(case, 33, kind=1)
-    for tyg in (1, 2, 4, 8):
-        for zc in (1, 5, 8, 64):
-            got = _run_steps(case, 33, kind=0, tyg=tyg, zc=zc)
-            errs = rel_l2_set(got, ref)
-            assert max(errs) < 1e-13, (variant, tyg, zc, errs)
+    for kind, tygs in ((0, (1, 2, 4, 8)), (2, (4, 8, 16))):
+        for tyg in tygs:
+            for zc in (1, 5, 8, 64):
+                got = _run_steps(case, 33, kind=kind, tyg=tyg, zc=zc)
+                errs = rel_l2_set(got, ref)
+                assert max(errs) < 1e-13, (variant, kind, tyg, zc, errs)
+    for tyg in (1, 2, 8):
+        for zc in (1, 5):
+            for order in (0, 1, 2):
+                got = _run_steps(case, 33, kind=1, tyg=tyg, zc=zc, order=order)
+                errs = rel_l2_set(got, ref)
+                assert max(errs) < 1e-13, (variant, tyg, zc, order, errs)
 
 
-def test_fused_solve_matches_two_pass_solve():
+@pytest.mark.parametrize("tuning", [(0, 2, 8, 0), (2, 8, 4, 0), (1, 8, 8, 2)])
+def test_tuned_solve_matches_default_solve(tuning):
+    """A whole solve (ping-pong set parity across the capture points and the
+    gated steps after convergence) with another step kernel or launch shape."""
     case = strip_case(40, 28, 22, w=5)
     import warnings
     a = run_gpu(case, fixed_n0=700)
     sim = a["sim"]
-    sim._lib.emf_set_tuning(sim._h, 1, 8, 8)
+    sim._lib.emf_set_tuning(sim._h, *tuning)
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         sim._register(sim._sources)
@@ -200,4 +212,4 @@ def test_fused_solve_matches_two_pass_solve():
     full = DomainCoordinates.from_indices(0, case["Nx"], 0, case["Ny"], 0, case["Nz"], *([case["d"]] * 3))
     b = [sim.get_field(c, full) for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")]
     errs = [rel_l2(x, y) for x, y in zip(a["fields"], b)]
-    assert max(errs) < 1e-12, errs
+    assert max(errs) < 1e-12, (tuning, errs)

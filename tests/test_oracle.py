@@ -133,6 +133,17 @@ def test_oracle_matches_compiled_reference_steps(olib, bc, ceps):
     assert max(errs) < 1e-13, errs
 
 
+@pytest.mark.skipif(not os.path.exists(D.ORACLE_OMP_LIB), reason="OpenMP oracle not built")
+def test_threaded_oracle_is_bit_identical(olib):
+    """The -fopenmp build (the all-cores CPU baseline) splits only the cell
+    loops over threads, so every field must match the serial build exactly."""
+    case = random_case(16, 12, 10, seed=5, ceps=True, bc="HE0", nsrc=2)
+    a = run_small_steps(olib, case, 25)
+    b = run_small_steps(D.load(D.ORACLE_OMP_LIB), case, 25)
+    for x, y in zip(a.fields, b.fields):
+        assert np.array_equal(x, y)
+
+
 @pytest.mark.skipif(not HAVE_REF, reason="reference sources not compiled here")
 def test_oracle_matches_compiled_reference_solve():
     case = strip_case(24, 20, 16, w=4)
