@@ -85,10 +85,18 @@ _SIGS = {
     "emf_copy_bandwidth": ([c_int, c_int64, c_int, _P], c_int),
     "emf_calc_ydAx": ([_P, _P, _P, _P] + [c_int] * 6 + [_P], c_int),
     "emf_grad_assemble": ([_P, c_int64, _P, _P, _P, _P, c_int, c_double, _P], c_int),
+    "emf_batch_new": ([c_int, POINTER(_P)], c_int),
+    "emf_batch_delete": ([_P], c_int),
+    "emf_batch_set_tuning": ([_P, c_int, c_int, c_int], c_int),
+    "emf_batch_add": ([_P, _P, c_int, c_double, c_double, c_int64, c_int64, POINTER(c_int)], c_int),
+    "emf_batch_run": ([_P, c_int, c_int, POINTER(c_int)], c_int),
+    "emf_batch_poll": ([_P, c_int, POINTER(SolveInfo)], c_int),
+    "emf_batch_remove": ([_P, c_int], c_int),
+    "emf_batch_count": ([_P, POINTER(c_int), POINTER(c_int)], c_int),
 }
 
 # calls whose int result is a count, not a status
-_COUNT_RESULT = {"emf_device_count", "emf_get_history"}
+_COUNT_RESULT = {"emf_device_count", "emf_get_history", "emf_batch_poll"}
 
 _lock = threading.Lock()
 _lib = None

@@ -1,0 +1,222 @@
+// Batched solves: many independent CW solves of different sizes advance
+// together.  One launch per half step covers every solve in the batch: a
+// persistent grid walks a device work table of tiles (all solves' columns,
+// back to back).  Each solve's whole __solve loop (emopt/fdtd.py:994-1194) runs
+// on the device as a small state machine advanced once per step by
+// k_batch_tick, so a chunk of steps can be captured once into a hipGraph and
+// replayed with no host work per step; the host only reads the solves' states
+// between replays, retires finished solves and admits new ones.
+//
+// Per step:  k_batch_h   H half step of every tile (or the snapshot / phasor
+//                        pass of solves at those points of their loop)
+//            k_batch_e   E half step of every tile
+//            k_batch_tick  per solve: step count, convergence check every Tn
+//                        steps, choice of what the next step does
+//
+// Arithmetic per cell is that of k_pass_h / k_pass_e / k_phasor3 with the same
+// host-computed source factors, so a solve's result agrees with the
+// single-solve path to roundoff and does not depend at all on what else is in
+// the batch.
+#pragma once
+#include "twopass.hpp"
+
+namespace emf {
+
+constexpr int BATCH_MAX = 128;   // solves in one batch
+constexpr int BTY = 4;           // rows per work-group of the batch passes
+// host-computed factors per step n (t = n dt, te = (n + 1/2) dt):
+// sin t, cos t, env(t), sin te, cos te, env(te), sin(t - dt/2), cos(t - dt/2)
+constexpr int NFAC = 8;
+
+struct BMember {                  // written by the host when the solve is admitted
+    FusedArgs A;                  // in place: Eo == En, Ho == Hn, pho == phn
+    Geo g;
+    const SrcBlk *blk;
+    int nsrc;
+    int ntx, nty, ntz;            // tile grid of this solve
+    const double *fac;            // [nfac][NFAC]
+    long long nfac;
+    double *res;                  // result slot: double2[6 n] viewed as doubles
+    long long n;                  // cells
+    long long n_stop;             // first step index past Nmax
+    long long fixed_n0;           // >= 0: run exactly this many steps before the snapshots
+    long long Tn;
+    double amp_rtol, phi_rtol, dt;
+    const long long *samples;
+    int nsamples;
+    int hist_cap;
+    ConvState *conv;
+    double *hist;
+};
+
+enum { BM_STEP = 0, BM_CAP0 = 1, BM_CAP1 = 2, BM_PHASOR = 3, BM_IDLE = 4 };
+enum { BP_CONV = 0, BP_RUN1 = 1, BP_RUN2 = 2, BP_DONE = 3 };
+
+struct BState {                   // advanced on the device by k_batch_tick
+    long long step;               // steps completed
+    long long n0;                 // step count at the t0 snapshot
+    int mode;                     // what the next H pass does (BM_*)
+    int phase;                    // where the solve is in its loop (BP_*)
+    int status;                   // 0 converged, 1 Nmax, 2 fixed step count
+    int nchecks;                  // convergence checks so far
+    double a_change, phi_change;  // last check's changes
+    PhTimes te, th;               // phasor sample times (set with BM_PHASOR)
+};
+
+struct BTable {                   // the tiles of the solves in the batch
+    int nact;                     // entries
+    int total;                    // tiles over all entries
+    int mem[BATCH_MAX];           // member slot of entry e
+    int off[BATCH_MAX + 1];       // first tile of entry e
+};
+
+// entry e with off[e] <= t < off[e + 1] (t < total)
+__device__ __forceinline__ int batch_entry(const BTable *__restrict__ T, int t)
+{
+    int lo = 0, hi = T->nact - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (T->off[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// tile -> column of member B; false when this thread's column is off the grid
+__device__ __forceinline__ bool batch_column(const BMember &B, int lt, int &k, int &j, int &z0,
+                                             int &z1)
+{
+    const int bx = lt % B.ntx, by = (lt / B.ntx) % B.nty, bz = lt / (B.ntx * B.nty);
+    k = bx * PX + threadIdx.x;
+    j = by * BTY + threadIdx.y;
+    z0 = bz * B.g.zc;
+    z1 = min(z0 + B.g.zc, B.g.Nz);
+    return k < B.g.Nx && j < B.g.Ny;
+}
+
+// 3-sample phasors of one column (k_phasor3's arithmetic), in place in the
+// result slot, whose halves hold the t0 and t1 snapshots
+__device__ __forceinline__ void phasor_column(const BMember &B, const BState &S, int k, int j,
+                                              int z0, int z1)
+{
+    double2 *r = reinterpret_cast<double2 *>(B.res);
+    const long long n = B.n;
+    for (int i = z0; i < z1; ++i) {
+        const long long gc = ((long long)i * B.g.Ny + j) * B.g.Nx + k;
+        const unsigned o = (unsigned)(gc * 8);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const double f = c < 3 ? ldo(B.A.Eo, o + (unsigned)c * B.A.cs)
+                                   : ldo(B.A.Ho, o + (unsigned)(c - 3) * B.A.cs);
+            const double2 v = r[c * n + gc];
+            r[c * n + gc] = phasor3(v.x, v.y, f, c < 3 ? S.te : S.th);
+        }
+    }
+}
+
+__global__ __launch_bounds__(PX * BTY) void k_batch_h(const BTable *__restrict__ T,
+                                                      const BMember *__restrict__ M,
+                                                      const BState *__restrict__ S)
+{
+    const int total = T->total;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int e = batch_entry(T, t);
+        const int m = T->mem[e];
+        const int mode = S[m].mode;
+        if (mode == BM_IDLE) continue;
+        const BMember &B = M[m];
+        int k, j, z0, z1;
+        if (!batch_column(B, t - T->off[e], k, j, z0, z1)) continue;
+        if (mode == BM_PHASOR) {
+            phasor_column(B, S[m], k, j, z0, z1);
+            continue;
+        }
+        const double *f = B.fac + S[m].step * NFAC;
+        PassSrc ps;
+        ps.n = B.nsrc; ps.comp0 = 3; ps.blk = B.blk;
+        ps.st = f[0]; ps.ct = f[1]; ps.env = f[2];
+        double *cap = mode == BM_CAP0 ? B.res : (mode == BM_CAP1 ? B.res + 1 : nullptr);
+        h_column<false>(B.A, B.g, ps, k, j, z0, z1, cap, B.n);
+    }
+}
+
+__global__ __launch_bounds__(PX * BTY) void k_batch_e(const BTable *__restrict__ T,
+                                                      const BMember *__restrict__ M,
+                                                      const BState *__restrict__ S)
+{
+    const int total = T->total;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int e = batch_entry(T, t);
+        const int m = T->mem[e];
+        const int mode = S[m].mode;
+        if (mode == BM_IDLE || mode == BM_PHASOR) continue;
+        const BMember &B = M[m];
+        int k, j, z0, z1;
+        if (!batch_column(B, t - T->off[e], k, j, z0, z1)) continue;
+        const double *f = B.fac + S[m].step * NFAC;
+        PassSrc ps;
+        ps.n = B.nsrc; ps.comp0 = 0; ps.blk = B.blk;
+        ps.st = f[3]; ps.ct = f[4]; ps.env = f[5];
+        e_column<false>(B.A, B.g, ps, k, j, z0, z1);
+    }
+}
+
+// One step of every solve's loop (emopt/fdtd.py:1049-1194), one work-group
+// per table entry, lane 0 working (the convergence norms are summed in a
+// fixed order, as in k_conv_check).
+__global__ void k_batch_tick(const BTable *__restrict__ T, const BMember *__restrict__ M,
+                             BState *__restrict__ S)
+{
+    const int e = blockIdx.x;
+    if (e >= T->nact || threadIdx.x != 0) return;
+    const int m = T->mem[e];
+    const BMember &B = M[m];
+    BState st = S[m];
+    if (st.mode == BM_IDLE) return;
+    if (st.mode == BM_PHASOR) {      // the phasor pass ran: the solve is finished
+        st.mode = BM_IDLE;
+        st.phase = BP_DONE;
+        S[m] = st;
+        return;
+    }
+    const long long nd = ++st.step;  // steps completed
+    int mode = BM_STEP;
+    if (st.phase == BP_CONV) {
+        bool fin = false;
+        if (B.fixed_n0 >= 0) {
+            if (nd >= B.fixed_n0) { fin = true; st.status = 2; }
+        } else {
+            if (nd % B.Tn == 0) {
+                const double *Ex = B.A.Eo;
+                const double *Ey = reinterpret_cast<const double *>(
+                    reinterpret_cast<const char *>(B.A.Eo) + B.A.cs);
+                const double *Ez = reinterpret_cast<const double *>(
+                    reinterpret_cast<const char *>(B.A.Eo) + 2 * (size_t)B.A.cs);
+                conv_check(Ex, Ey, Ez, B.samples, B.nsamples, (double)(nd - 1) * B.dt,
+                           B.amp_rtol, B.phi_rtol, nd - 1, B.conv, B.hist, B.hist_cap);
+                st.nchecks = B.conv->nchecks;
+                st.a_change = B.conv->a_change;
+                st.phi_change = B.conv->phi_change;
+                if (B.conv->done) { fin = true; st.status = 0; }
+            }
+            if (!fin && nd >= B.n_stop) { fin = true; st.status = 1; }
+        }
+        if (fin) { st.n0 = nd; st.phase = BP_RUN1; mode = BM_CAP0; }
+    } else if (st.phase == BP_RUN1) {
+        if (nd == st.n0 + B.Tn) { st.phase = BP_RUN2; mode = BM_CAP1; }
+    } else if (st.phase == BP_RUN2) {
+        if (nd == st.n0 + 2 * B.Tn) {
+            const double *f0 = B.fac + st.n0 * NFAC, *f1 = B.fac + (st.n0 + B.Tn) * NFAC,
+                         *f2 = B.fac + (st.n0 + 2 * B.Tn) * NFAC;
+            st.te.s21 = f2[0] - f1[0]; st.te.s10 = f1[0] - f0[0];
+            st.te.c10 = f1[1] - f0[1]; st.te.c21 = f2[1] - f1[1];
+            st.th.s21 = f2[6] - f1[6]; st.th.s10 = f1[6] - f0[6];
+            st.th.c10 = f1[7] - f0[7]; st.th.c21 = f2[7] - f1[7];
+            mode = BM_PHASOR;
+        }
+    }
+    st.mode = mode;
+    S[m] = st;
+}
+
+}  // namespace emf

@@ -1,0 +1,425 @@
+// Host side of batched solves (emf_batch_* entry points).  Included at the end
+// of emfdtd.hip: it uses the solver internals defined there.
+//
+// A batch owns a stream, three small device arrays (work table, member
+// descriptors, member states) and one hipGraph of `spg` steps, captured on
+// first use and replayed until every member has finished.  The graph's
+// kernels read everything that varies (which solves, which step, what the
+// next step does) from those device arrays, so admitting or retiring a solve
+// only rewrites them between replays -- the graph is never re-captured and
+// records no memset or copy nodes.
+#pragma once
+
+namespace {
+
+struct BatchSlot {
+    emf_solver *s = nullptr;
+    int slot = 0;
+    bool finished = false;
+    double *fac = nullptr;              // device per-step factors
+    emf_solve_info info{};
+    std::chrono::steady_clock::time_point t_add;
+};
+
+}  // namespace
+
+struct emf_batch {
+    std::mutex mtx;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    int spg = 64;                       // steps per graph replay
+    int zc = 2;                         // z planes per work-group
+    int bpc = 8;                        // persistent work-groups per CU
+    int ncu = 0;
+    BTable *T = nullptr;
+    BMember *M = nullptr;
+    BState *S = nullptr;
+    BTable hT{};
+    BMember hM[BATCH_MAX]{};
+    BState *hS = nullptr;               // pinned ring of 3 state snapshots
+    int ring = 0;
+    BatchSlot sl[BATCH_MAX];
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int batch_upload_table(emf_batch *b)
+{
+    BTable &t = b->hT;
+    t.nact = 0;
+    t.total = 0;
+    for (int m = 0; m < BATCH_MAX; ++m) {
+        if (!b->sl[m].s || b->sl[m].finished) continue;
+        const BMember &B = b->hM[m];
+        const long long tiles = (long long)B.ntx * B.nty * B.ntz;
+        if ((long long)t.total + tiles > 0x7fffffffLL) EMF_FAIL(EMF_ERR_ARG, "batch too large");
+        t.mem[t.nact] = m;
+        t.off[t.nact] = t.total;
+        t.total += (int)tiles;
+        ++t.nact;
+    }
+    t.off[t.nact] = t.total;
+    HIPCHK(hipMemcpyAsync(b->T, &t, sizeof(BTable), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));   // t is rewritten by the next call
+    return EMF_OK;
+}
+
+int batch_graph(emf_batch *b)
+{
+    if (b->exec) return EMF_OK;
+    const unsigned blocks = (unsigned)(b->ncu * b->bpc);
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (int q = 0; q < b->spg; ++q) {
+        hipLaunchKernelGGL(k_batch_h, dim3(blocks), dim3(PX, BTY), 0, b->stream, b->T, b->M, b->S);
+        hipLaunchKernelGGL(k_batch_e, dim3(blocks), dim3(PX, BTY), 0, b->stream, b->T, b->M, b->S);
+        hipLaunchKernelGGL(k_batch_tick, dim3(BATCH_MAX), dim3(64), 0, b->stream, b->T, b->M,
+                           b->S);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+    if (ec != hipSuccess || !g) EMF_FAIL(EMF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return EMF_OK;
+}
+
+void batch_drop_graph(emf_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// a member whose state shows it finished: fill its solve info and hand the
+// result back to its solver handle
+void batch_finish(emf_batch *b, int m, const BState &st)
+{
+    BatchSlot &q = b->sl[m];
+    const BMember &B = b->hM[m];
+    q.finished = true;
+    q.info.n0 = st.n0;
+    q.info.Tn = B.Tn;
+    q.info.steps = st.n0 + 2 * B.Tn;
+    q.info.status = st.status;
+    q.info.nchecks = st.nchecks;
+    q.info.a_change = st.a_change;
+    q.info.phi_change = st.phi_change;
+    q.info.seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - q.t_add).count();
+    std::lock_guard<std::mutex> lks(q.s->mtx);
+    q.s->nhist = std::min(st.nchecks, q.s->hist_cap);
+}
+
+}  // namespace
+
+extern "C" {
+
+int emf_batch_new(int device, emf_batch **out)
+{
+    if (!out) EMF_FAIL(EMF_ERR_ARG, "null output");
+    *out = nullptr;
+    HIPCHK(hipSetDevice(device));
+    emf_batch *b = new emf_batch();
+    b->dev = device;
+    int rc = EMF_OK;
+    if (hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = EMF_ERR_HIP;
+    if (rc == EMF_OK) rc = dalloc(&b->T, 1);
+    if (rc == EMF_OK) rc = dalloc(&b->M, BATCH_MAX);
+    if (rc == EMF_OK) rc = dalloc(&b->S, BATCH_MAX);
+    if (rc == EMF_OK && hipHostMalloc((void **)&b->hS, 3 * BATCH_MAX * sizeof(BState)) != hipSuccess)
+        rc = EMF_ERR_NOMEM;
+    if (rc == EMF_OK) {
+        memset(b->hS, 0, 3 * BATCH_MAX * sizeof(BState));
+        rc = batch_upload_table(b);
+    }
+    if (rc != EMF_OK) {
+        dfree(b->T); dfree(b->M); dfree(b->S);
+        if (b->hS) (void)hipHostFree(b->hS);
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+        EMF_FAIL(rc, "batch creation failed");
+    }
+    *out = b;
+    return EMF_OK;
+}
+
+int emf_batch_delete(emf_batch *b)
+{
+    if (!b) return EMF_OK;
+    {
+        std::lock_guard<std::mutex> lk(b->mtx);
+        (void)hipSetDevice(b->dev);
+        (void)hipStreamSynchronize(b->stream);
+        batch_drop_graph(b);
+        for (auto &q : b->sl) {
+            if (q.s) {
+                std::lock_guard<std::mutex> lks(q.s->mtx);
+                q.s->in_batch = nullptr;
+            }
+            dfree(q.fac);
+        }
+        dfree(b->T); dfree(b->M); dfree(b->S);
+        if (b->hS) (void)hipHostFree(b->hS);
+        (void)hipStreamDestroy(b->stream);
+    }
+    delete b;
+    return EMF_OK;
+}
+
+#define BLOCK(b)                                                    \
+    if (!(b)) EMF_FAIL(EMF_ERR_ARG, "null batch handle");          \
+    std::lock_guard<std::mutex> blk_((b)->mtx);                     \
+    if (hipSetDevice((b)->dev) != hipSuccess)                       \
+        EMF_FAIL(EMF_ERR_HIP, "hipSetDevice(%d) failed", (b)->dev);
+
+int emf_batch_set_tuning(emf_batch *b, int steps_per_graph, int zc, int blocks_per_cu)
+{
+    BLOCK(b);
+    if (steps_per_graph < 1 || steps_per_graph > 4096) EMF_FAIL(EMF_ERR_ARG, "steps_per_graph must be 1..4096");
+    if (zc < 1) EMF_FAIL(EMF_ERR_ARG, "zc must be >= 1");
+    if (blocks_per_cu < 1 || blocks_per_cu > 64) EMF_FAIL(EMF_ERR_ARG, "blocks_per_cu must be 1..64");
+    for (auto &q : b->sl)
+        if (q.s && !q.finished) EMF_FAIL(EMF_ERR_STATE, "set the tuning while no solve is running");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (steps_per_graph != b->spg || blocks_per_cu != b->bpc) batch_drop_graph(b);
+    b->spg = steps_per_graph;
+    b->zc = zc;
+    b->bpc = blocks_per_cu;
+    return EMF_OK;
+}
+
+int emf_batch_add(emf_batch *b, emf_solver *s, int slot, double rtol, double nmax, int64_t Tn,
+                  int64_t fixed_n0, int *member)
+{
+    BLOCK(b);
+    if (!s || !member) EMF_FAIL(EMF_ERR_ARG, "null solver or output");
+    std::lock_guard<std::mutex> lks(s->mtx);
+    if (s->in_batch) EMF_FAIL(EMF_ERR_ARG, "solver is already a member of a batch");
+    if (s->dev != b->dev) EMF_FAIL(EMF_ERR_ARG, "solver is on device %d, batch on %d", s->dev, b->dev);
+    if (!(rtol > 0)) EMF_FAIL(EMF_ERR_ARG, "rtol must be > 0");
+    if (Tn < 1) EMF_FAIL(EMF_ERR_ARG, "Tn must be >= 1 in a batch");
+    int id = -1;
+    for (int m = 0; m < BATCH_MAX && id < 0; ++m)
+        if (!b->sl[m].s) id = m;
+    if (id < 0) EMF_FAIL(EMF_ERR_STATE, "batch is full (%d solves)", BATCH_MAX);
+    CHECK(check_ready(s));
+    if (s->ceps || !s->mu_uniform)
+        EMF_FAIL(EMF_ERR_ARG, "batched solves need real eps and uniform mu");
+    const long long n_stop = nmax < 0 ? 0 : (long long)floor(std::min(nmax, 1e15)) + 1;
+    const long long nlast = fixed_n0 >= 0 ? fixed_n0 : n_stop;
+    const long long nfac = nlast + 2 * Tn + 1;
+    if (nfac > (1LL << 22)) EMF_FAIL(EMF_ERR_ARG, "step limit %lld too large for a batch", nlast);
+
+    // the solver's state as emf_solve sets it up (emopt/fdtd.py:1016-1018)
+    CHECK(ensure_slot(s, slot));
+    CHECK(build_samples(s));
+    PassTab th, te;
+    CHECK(refresh_tables(s, &th, &te));
+    CHECK(refresh_sources(s));
+    long long cap = std::min<long long>(nlast / Tn + 4, 1LL << 20);
+    if (cap > s->hist_cap) {
+        dfree(s->hist);
+        CHECK(dalloc(&s->hist, 3 * (size_t)cap));
+        s->hist_cap = (int)cap;
+    }
+    set_current(s, 0);
+    CHECK(reset_psi(s));
+    for (int c = 0; c < 3; ++c) {
+        HIPCHK(hipMemsetAsync(s->E[c], 0, s->n * sizeof(double), s->stream));
+        HIPCHK(hipMemsetAsync(s->H[c], 0, s->n * sizeof(double), s->stream));
+    }
+    HIPCHK(hipMemsetAsync(s->conv, 0, sizeof(ConvState), s->stream));
+    s->nhist = 0;
+
+    // per-step factors, computed exactly as the single-solve launches do
+    std::vector<double> f((size_t)nfac * NFAC);
+    for (long long q = 0; q < nfac; ++q) {
+        const double t = (double)q * s->dt, tE = ((double)q + 0.5) * s->dt;
+        double *r = &f[(size_t)q * NFAC];
+        r[0] = sin(t); r[1] = cos(t); r[2] = env_at(s, t);
+        r[3] = sin(tE); r[4] = cos(tE); r[5] = env_at(s, tE);
+        const double h = t - 0.5 * s->dt;
+        r[6] = sin(h); r[7] = cos(h);
+    }
+    BatchSlot &q = b->sl[id];
+    CHECK(dalloc(&q.fac, f.size()));
+    int rc = hipMemcpyAsync(q.fac, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice,
+                            s->stream) == hipSuccess ? EMF_OK : EMF_ERR_HIP;
+    if (rc == EMF_OK && hipStreamSynchronize(s->stream) != hipSuccess) rc = EMF_ERR_HIP;
+    if (rc != EMF_OK) { dfree(q.fac); EMF_FAIL(rc, "upload of the step factors failed"); }
+
+    BMember &B = b->hM[id];
+    B.A = make_args(s, 0, 0);
+    B.g = s->geo();
+    B.g.zc = b->zc;
+    B.g.order = 0;
+    B.blk = s->src_desc;
+    B.nsrc = (int)s->srcs.size();
+    B.ntx = (s->Nx + PX - 1) / PX;
+    B.nty = (s->Ny + BTY - 1) / BTY;
+    B.ntz = (s->Nz + b->zc - 1) / b->zc;
+    B.fac = q.fac;
+    B.nfac = nfac;
+    B.res = reinterpret_cast<double *>(s->res[slot][0]);
+    B.n = s->n;
+    B.n_stop = n_stop;
+    B.fixed_n0 = fixed_n0;
+    B.Tn = Tn;
+    B.amp_rtol = rtol;
+    B.phi_rtol = sqrt(rtol);
+    B.dt = s->dt;
+    B.samples = s->samples;
+    B.nsamples = s->nsamples;
+    B.hist_cap = s->hist_cap;
+    B.conv = s->conv;
+    B.hist = s->hist;
+
+    BState st{};
+    st.mode = BM_STEP;
+    st.phase = BP_CONV;
+    if (nlast == 0) {                    // no steps before the t0 snapshot
+        st.mode = BM_CAP0;
+        st.phase = BP_RUN1;
+        st.status = fixed_n0 >= 0 ? 2 : 1;
+    }
+    // the batch's kernels may still be reading M/S/T: wait for them
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpyAsync(&b->M[id], &B, sizeof(BMember), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(&b->S[id], &st, sizeof(BState), hipMemcpyHostToDevice, b->stream));
+    q.s = s;
+    s->in_batch = b;
+    q.slot = slot;
+    q.finished = false;
+    q.info = emf_solve_info{};
+    q.t_add = std::chrono::steady_clock::now();
+    rc = batch_upload_table(b);           // synchronises the stream (B, st are locals)
+    if (rc != EMF_OK) {
+        q.s = nullptr;
+        s->in_batch = nullptr;
+        dfree(q.fac);
+        return rc;
+    }
+    *member = id;
+    return EMF_OK;
+}
+
+int emf_batch_run(emf_batch *b, int max_graphs, int until, int *finished)
+{
+    BLOCK(b);
+    if (max_graphs < 0 || until < 0 || until > 2) EMF_FAIL(EMF_ERR_ARG, "bad arguments");
+    int nfin = 0;
+    auto running = [&]() {
+        int r = 0;
+        for (auto &q : b->sl) r += q.s && !q.finished;
+        return r;
+    };
+    if (running() > 0) {
+        CHECK(batch_graph(b));
+        struct Pend { int r; hipEvent_t ev; };
+        std::vector<Pend> pend;
+        int launched = 0;
+        bool dirty = false, stop = false;
+        int rc = EMF_OK;
+        auto retire = [&](const Pend &p) {
+            if (hipEventSynchronize(p.ev) != hipSuccess) rc = EMF_ERR_HIP;
+            (void)hipEventDestroy(p.ev);
+            const BState *snap = b->hS + (size_t)p.r * BATCH_MAX;
+            for (int m = 0; m < BATCH_MAX; ++m) {
+                BatchSlot &q = b->sl[m];
+                if (q.s && !q.finished && snap[m].phase == BP_DONE) {
+                    batch_finish(b, m, snap[m]);
+                    ++nfin;
+                    dirty = true;
+                }
+            }
+        };
+        while (rc == EMF_OK) {
+            if (until == 1 && nfin > 0) stop = true;
+            if (until == 2 && running() == 0) stop = true;
+            if (dirty && !stop) {
+                // compact the work table: wait for the replays in flight first
+                for (auto &p : pend) retire(p);
+                pend.clear();
+                if (rc == EMF_OK) rc = batch_upload_table(b);
+                dirty = false;
+                if (running() == 0) stop = true;
+                continue;
+            }
+            while (!stop && rc == EMF_OK && (int)pend.size() < 2 &&
+                   (until == 2 || launched < max_graphs)) {
+                hipEvent_t ev;
+                if (hipGraphLaunch(b->exec, b->stream) != hipSuccess ||
+                    hipMemcpyAsync(b->hS + (size_t)b->ring * BATCH_MAX, b->S,
+                                   BATCH_MAX * sizeof(BState), hipMemcpyDeviceToHost,
+                                   b->stream) != hipSuccess ||
+                    hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+                    hipEventRecord(ev, b->stream) != hipSuccess) {
+                    rc = EMF_ERR_HIP;
+                    break;
+                }
+                pend.push_back({ b->ring, ev });
+                b->ring = (b->ring + 1) % 3;
+                ++launched;
+            }
+            if (pend.empty()) break;
+            const Pend p = pend.front();
+            pend.erase(pend.begin());
+            retire(p);
+        }
+        for (auto &p : pend) retire(p);
+        if (rc == EMF_OK && dirty) rc = batch_upload_table(b);
+        if (rc != EMF_OK) EMF_FAIL(rc, "batch run failed");
+    }
+    if (finished) *finished = nfin;
+    return EMF_OK;
+}
+
+int emf_batch_poll(emf_batch *b, int member, emf_solve_info *info)
+{
+    BLOCK(b);
+    if (member < 0 || member >= BATCH_MAX || !b->sl[member].s)
+        EMF_FAIL(EMF_ERR_ARG, "no member %d in this batch", member);
+    const BatchSlot &q = b->sl[member];
+    if (!q.finished) return 0;
+    if (info) *info = q.info;
+    return 1;
+}
+
+int emf_batch_remove(emf_batch *b, int member)
+{
+    BLOCK(b);
+    if (member < 0 || member >= BATCH_MAX || !b->sl[member].s)
+        EMF_FAIL(EMF_ERR_ARG, "no member %d in this batch", member);
+    BatchSlot &q = b->sl[member];
+    const bool was_running = !q.finished;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    {
+        std::lock_guard<std::mutex> lks(q.s->mtx);
+        q.s->in_batch = nullptr;
+    }
+    dfree(q.fac);
+    q = BatchSlot{};
+    if (was_running) CHECK(batch_upload_table(b));
+    return EMF_OK;
+}
+
+int emf_batch_count(emf_batch *b, int *running, int *finished)
+{
+    BLOCK(b);
+    int r = 0, f = 0;
+    for (auto &q : b->sl) {
+        if (!q.s) continue;
+        if (q.finished) ++f;
+        else ++r;
+    }
+    if (running) *running = r;
+    if (finished) *finished = f;
+    return EMF_OK;
+}
+
+}  // extern "C"

@@ -19,7 +19,7 @@
 #include <vector>
 
 #include "../../include/emfdtd.h"
-#include "twopass.hpp"
+#include "batch.hpp"
 
 using namespace emf;
 
@@ -177,7 +177,7 @@ struct emf_solver {
     long long psi_n[6] = {};
     bool pml_built = false;
 
-    double2 *res[2][6] = {};       // result slots: fwd, adj
+    double2 *res[2][6] = {};       // result slots: fwd, adj (one 6n block per slot)
 
     std::vector<SourceDev> srcs;
     SrcBlk *src_desc = nullptr;    // device copy of the source descriptors
@@ -197,6 +197,8 @@ struct emf_solver {
 
     double2 *scratch = nullptr;
     size_t scratch_n = 0;
+
+    void *in_batch = nullptr;      // the emf_batch this handle is a member of
 
     Geo geo() const
     {
@@ -278,8 +280,10 @@ void free_fields(emf_solver *s)
         s->bE[c] = s->aE[c] = s->mH[c] = s->muR[c] = nullptr;
         dfree(s->eps[c]); dfree(s->mu[c]);
     }
-    for (int sl = 0; sl < 2; ++sl)
-        for (int c = 0; c < 6; ++c) dfree(s->res[sl][c]);
+    for (int sl = 0; sl < 2; ++sl) {
+        dfree(s->res[sl][0]);   // one block per slot; [1..5] point into it
+        for (int c = 1; c < 6; ++c) s->res[sl][c] = nullptr;
+    }
     s->mat_set = false;
     s->coef_dirty = true;
     s->cur = 0;
@@ -289,10 +293,10 @@ int ensure_slot(emf_solver *s, int slot)
 {
     if (slot < 0 || slot > 1) EMF_FAIL(EMF_ERR_ARG, "result slot must be 0 or 1, got %d", slot);
     if (s->res[slot][0]) return EMF_OK;
-    for (int c = 0; c < 6; ++c) {
-        CHECK(dalloc(&s->res[slot][c], s->n));
-        HIPCHK(hipMemsetAsync(s->res[slot][c], 0, s->n * sizeof(double2), s->stream));
-    }
+    // six components in one block: component c at res[slot][0] + c * n
+    CHECK(dalloc(&s->res[slot][0], 6 * s->n));
+    for (int c = 1; c < 6; ++c) s->res[slot][c] = s->res[slot][0] + (size_t)c * s->n;
+    HIPCHK(hipMemsetAsync(s->res[slot][0], 0, 6 * s->n * sizeof(double2), s->stream));
     return EMF_OK;
 }
 
@@ -486,6 +490,12 @@ int check_ready(emf_solver *s)
         EMF_FAIL(EMF_ERR_ARG, "opposite pml slabs overlap");
     if (s->R <= 0 || s->dx <= 0 || s->dy <= 0 || s->dz <= 0)
         EMF_FAIL(EMF_ERR_STATE, "wavelength / physical dims not set");
+    // the step kernels address every array by a 32-bit byte offset
+    if (3ULL * (unsigned long long)s->n * 8ULL >= (1ULL << 32) ||
+        (unsigned long long)s->psi_tot * 8ULL >= (1ULL << 32))
+        EMF_FAIL(EMF_ERR_ARG,
+                 "grid of %lld cells exceeds the kernels' 32-bit offset range (at most %llu cells)",
+                 s->n, ((1ULL << 32) - 1) / 24ULL);
     CHECK(refresh_coeffs(s));
     return EMF_OK;
 }
@@ -759,6 +769,8 @@ int build_samples(emf_solver *s)
 #define LOCK(s)                                                     \
     if (!(s)) EMF_FAIL(EMF_ERR_ARG, "null solver handle");         \
     std::lock_guard<std::mutex> lk_((s)->mtx);                      \
+    if ((s)->in_batch)                                              \
+        EMF_FAIL(EMF_ERR_STATE, "handle is a member of a batch (emf_batch_remove it first)"); \
     if (hipSetDevice((s)->dev) != hipSuccess)                       \
         EMF_FAIL(EMF_ERR_HIP, "hipSetDevice(%d) failed", (s)->dev);
 
@@ -812,6 +824,8 @@ int emf_delete(emf_solver *s)
     if (!s) return EMF_OK;
     {
         std::lock_guard<std::mutex> lk(s->mtx);
+        if (s->in_batch)
+            EMF_FAIL(EMF_ERR_STATE, "handle is a member of a batch (emf_batch_remove it first)");
         (void)hipSetDevice(s->dev);
         (void)hipStreamSynchronize(s->stream);
         free_fields(s);
@@ -1547,3 +1561,5 @@ int emf_copy_bandwidth(int device, int64_t bytes, int reps, double *gbps)
 }
 
 }  // extern "C"
+
+#include "batch_host.hpp"

@@ -133,14 +133,12 @@ __device__ inline double amp3(double t0, double t1, double t2, double f0, double
     return f10 / (cos(ph) * (sin(t1) - sin(t0)) + sin(ph) * (cos(t1) - cos(t0)));
 }
 
-__global__ void k_conv_check(const double *__restrict__ Ex, const double *__restrict__ Ey,
-                             const double *__restrict__ Ez, const long long *__restrict__ sample,
-                             int nc, double tnow, double amp_rtol, double phi_rtol,
-                             long long nstep, ConvState *st, double *hist, int hist_cap,
-                             int *done_out)
+__device__ inline int conv_check(const double *__restrict__ Ex, const double *__restrict__ Ey,
+                                 const double *__restrict__ Ez,
+                                 const long long *__restrict__ sample, int nc, double tnow,
+                                 double amp_rtol, double phi_rtol, long long nstep, ConvState *st,
+                                 double *hist, int hist_cap)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (st->done) return;
     st->t0 = st->t1; st->t1 = st->t2; st->t2 = tnow;
     const double t0 = st->t0, t1 = st->t1, t2 = st->t2;
     const double *E[3] = { Ex, Ey, Ez };
@@ -172,6 +170,19 @@ __global__ void k_conv_check(const double *__restrict__ Ex, const double *__rest
     }
     const int conv = (ac <= amp_rtol) && (pc <= phi_rtol);
     st->done = conv;
+    return conv;
+}
+
+__global__ void k_conv_check(const double *__restrict__ Ex, const double *__restrict__ Ey,
+                             const double *__restrict__ Ez, const long long *__restrict__ sample,
+                             int nc, double tnow, double amp_rtol, double phi_rtol,
+                             long long nstep, ConvState *st, double *hist, int hist_cap,
+                             int *done_out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (st->done) return;
+    const int conv = conv_check(Ex, Ey, Ez, sample, nc, tnow, amp_rtol, phi_rtol, nstep, st, hist,
+                                hist_cap);
     if (done_out) *done_out = conv;
 }
 

@@ -43,14 +43,15 @@ __device__ __forceinline__ void pass_tile(const Geo &g, int ty, int &k, int &j, 
 }
 
 // ------------------------------------------------------------------ H pass
-template <bool MU_ARRAY, int TY>
-__global__ __launch_bounds__(PX * TY) void k_pass_h(FusedArgs A, Geo g, PassSrc sh,
-                                                   const int *__restrict__ done)
+// Cells (i, j, k), z0 <= i < z1, of one (k, j) column.  With cap != nullptr
+// the pre-update E and H (the state after the steps taken so far) are also
+// stored as a snapshot: component c of cell g goes to cap[2 * (c * n + g)]
+// (cap points at the .x or .y half of a six-component result block).
+template <bool MU_ARRAY>
+__device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const PassSrc &sh,
+                                         int k, int j, int z0, int z1, double *cap,
+                                         long long capn)
 {
-    if (done && *done) return;
-    int k, j, z0, z1;
-    pass_tile(g, TY, k, j, z0, z1);
-    if (k >= g.Nx || j >= g.Ny) return;
     const Layout L = make_layout(g);
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
     const unsigned cs = A.cs, cs2 = 2 * A.cs;
@@ -74,6 +75,12 @@ __global__ __launch_bounds__(PX * TY) void k_pass_h(FusedArgs A, Geo g, PassSrc 
         if (kn) { d.ezk = ldo(A.Eo, o + 8u + cs2); d.eyk = ldo(A.Eo, o + 8u + cs); }
         d.hx = ldo(A.Ho, o); d.hy = ldo(A.Ho, o + cs); d.hz = ldo(A.Ho, o + cs2);
         h_load_aux<MU_ARRAY>(g, L, A, i, j, k, o, d);
+        if (cap) {
+            const long long gc = (long long)(o >> 3);
+            cap[2 * gc] = d.ex; cap[2 * (capn + gc)] = d.ey; cap[2 * (2 * capn + gc)] = d.ez;
+            cap[2 * (3 * capn + gc)] = d.hx; cap[2 * (4 * capn + gc)] = d.hy;
+            cap[2 * (5 * capn + gc)] = d.hz;
+        }
         double hx, hy, hz, np[6];
         h_compute(g, d, hx, hy, hz, np);
         if (col_src) h_sources<MU_ARRAY>(g, sh, A, i, j, k, o, hx, hy, hz);
@@ -85,15 +92,22 @@ __global__ __launch_bounds__(PX * TY) void k_pass_h(FusedArgs A, Geo g, PassSrc 
     }
 }
 
-// ------------------------------------------------------------------ E pass
-template <bool CEPS, int TY>
-__global__ __launch_bounds__(PX * TY) void k_pass_e(FusedArgs A, Geo g, PassSrc se,
+template <bool MU_ARRAY, int TY>
+__global__ __launch_bounds__(PX * TY) void k_pass_h(FusedArgs A, Geo g, PassSrc sh,
                                                    const int *__restrict__ done)
 {
     if (done && *done) return;
     int k, j, z0, z1;
     pass_tile(g, TY, k, j, z0, z1);
     if (k >= g.Nx || j >= g.Ny) return;
+    h_column<MU_ARRAY>(A, g, sh, k, j, z0, z1, nullptr, 0);
+}
+
+// ------------------------------------------------------------------ E pass
+template <bool CEPS>
+__device__ __forceinline__ void e_column(const FusedArgs &A, const Geo &g, const PassSrc &se,
+                                         int k, int j, int z0, int z1)
+{
     const Layout L = make_layout(g);
     const int Nx = g.Nx, Ny = g.Ny;
     const unsigned cs = A.cs, cs2 = 2 * A.cs;
@@ -191,6 +205,17 @@ __global__ __launch_bounds__(PX * TY) void k_pass_e(FusedArgs A, Geo g, PassSrc 
         if (sz.in) { sto(A.pe, L.pso[fz][0] + sz.io, qz0); sto(A.pe, L.pso[fz][1] + sz.io, qz1); }
         hxm = hx; hym = hy;
     }
+}
+
+template <bool CEPS, int TY>
+__global__ __launch_bounds__(PX * TY) void k_pass_e(FusedArgs A, Geo g, PassSrc se,
+                                                   const int *__restrict__ done)
+{
+    if (done && *done) return;
+    int k, j, z0, z1;
+    pass_tile(g, TY, k, j, z0, z1);
+    if (k >= g.Nx || j >= g.Ny) return;
+    e_column<CEPS>(A, g, se, k, j, z0, z1);
 }
 
 }  // namespace emf

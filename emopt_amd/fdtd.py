@@ -376,6 +376,10 @@ class FDTD(MaxwellSolver):
         n0 = -1 if self.fixed_n0 is None else int(self.fixed_n0)
         self._lib.emf_solve(self._h, slot, float(self._rtol), float(self.Nmax), self.Tn, n0,
                             _lib.ctypes.byref(info))
+        self._solved(slot, info)
+
+    def _solved(self, slot, info):
+        """Bookkeeping after a solve into `slot` (by emf_solve or a batch)."""
         nrec = self._lib.emf_get_history(self._h, None, 0)
         hist = np.zeros((max(nrec, 0), 3))
         if nrec > 0:
@@ -398,6 +402,9 @@ class FDTD(MaxwellSolver):
         """Forward solve (emopt/fdtd.py:1196-1250)."""
         self._register(self._sources)
         self._solve(0)
+        self._forward_done()
+
+    def _forward_done(self):
         self.update_saved_fields()
         self._source_power = self.get_source_power()
         self.forward_count += 1

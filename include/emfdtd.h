@@ -248,6 +248,53 @@ EMF_API int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc, int order);
  * `bytes` bytes, median of `reps` launches; a roofline reference point. */
 EMF_API int emf_copy_bandwidth(int device, int64_t bytes, int reps, double *gbps);
 
+/* ---- batched solves (no counterpart in the reference) ------------------ */
+
+/* A batch advances many configured solver handles on one device together:
+ * one launch per half step covers every member (a persistent grid over a
+ * device table of all members' tiles), and each member's whole solve loop --
+ * the emf_solve loop of emopt/fdtd.py:994-1194, with its own grid size,
+ * wavelength, sources, Tn and convergence test -- runs on the device, so a
+ * chunk of steps is one captured hipGraph replayed with no host work per
+ * step.  Members are admitted and retired between replays (continuous
+ * batching).  A member's result agrees with emf_solve on the same handle to
+ * roundoff, with the same n0 and step counts (per cell the arithmetic and the
+ * source factors are the same), and does not depend on the other members.
+ * Members need real eps and uniform mu.  While a handle is a running member
+ * it must not be used through any other call. */
+typedef struct emf_batch emf_batch;
+
+EMF_API int emf_batch_new(int device, emf_batch **out);
+EMF_API int emf_batch_delete(emf_batch *b);
+
+/* steps_per_graph = steps in one graph replay (default 64); zc = z planes per
+ * work-group for members admitted from now on (default 2); blocks_per_cu =
+ * persistent work-groups per CU (default 8).  Only while nothing runs. */
+EMF_API int emf_batch_set_tuning(emf_batch *b, int steps_per_graph, int zc, int blocks_per_cu);
+
+/* Admits solver s: resets its fields and CPML as emf_solve does and queues
+ * the solve (result slot, rtol, nmax, Tn >= 1, fixed_n0 as for emf_solve).
+ * *member receives the member id.  The solver handle stays owned by the
+ * caller. */
+EMF_API int emf_batch_add(emf_batch *b, emf_solver *s, int slot, double rtol, double nmax,
+                          int64_t Tn, int64_t fixed_n0, int *member);
+
+/* Replays the graph: until = 0 runs max_graphs replays; 1 returns as soon as
+ * a member finishes (or after max_graphs replays); 2 runs until every member
+ * has finished.  *finished (may be NULL) = members that finished during the
+ * call. */
+EMF_API int emf_batch_run(emf_batch *b, int max_graphs, int until, int *finished);
+
+/* 1 and *info filled when the member has finished, 0 while it runs. */
+EMF_API int emf_batch_poll(emf_batch *b, int member, emf_solve_info *info);
+
+/* Detaches a member (finished or not); its handle is free for other calls,
+ * and after a finished solve its result slot holds the phasors. */
+EMF_API int emf_batch_remove(emf_batch *b, int member);
+
+/* Numbers of running and finished (not yet removed) members. */
+EMF_API int emf_batch_count(emf_batch *b, int *running, int *finished);
+
 /* ---- gradient assembly ------------------------------------------------- */
 
 /* y^T dA x of emopt/fdtd.py:1694-1717 on the device:

@@ -190,3 +190,83 @@ def run_oracle(case, fixed_n0=None, lib="oracle", max_steps=None):
         o.solve_forward(fixed_n0=fixed_n0, max_steps=max_steps)
     return dict(n0=o.n0, Tn=o.Tn, fields=o.phasors(), history=np.array(o.history),
                 seconds=o.solve_seconds, status=o.status, sim=o)
+
+
+def batch_case(q, seed=3, dims=((64, 256), (64, 160), (48, 96)), w_range=(8, 12), d=0.04,
+               rtol=1e-4, n_fwd=16):
+    """Solve q of the batched workload: grid dims uniform in `dims`, lambda
+    uniform in 1.50-1.60 um, CPML width uniform in w_range cells, a random
+    slab (eps in [1.444^2, 3.48^2]) holding 4-12 random ellipses; solves
+    q < n_fwd carry a forward Jz dipole at a random interior point, the others
+    an adjoint source: a complex Gaussian times a random phase on an x
+    monitor plane."""
+    from emopt_amd.grid import StructuredMaterial3D, Rectangle, Ellipse, ConstantMaterial3D
+    rng = np.random.default_rng([seed, q])
+    Nx, Ny, Nz = (int(rng.integers(lo, hi + 1)) for lo, hi in dims)
+    wl = float(rng.uniform(1.50, 1.60))
+    w = int(rng.integers(w_range[0], w_range[1] + 1))
+    X, Y, Z = span(Nx, d), span(Ny, d), span(Nz, d)
+    eps = StructuredMaterial3D(X, Y, Z, d, d, d)
+    eps.add_primitive(Rectangle(X / 2, Y / 2, 2 * X, 2 * Y, material_value=EPS_OX, layer=10),
+                      -Z, 2 * Z)
+    t = float(rng.uniform(0.15, 0.35))
+    zc = Z / 2
+    slab_eps = float(rng.uniform(EPS_OX, EPS_SI))
+    eps.add_primitive(Rectangle(X / 2, Y / 2, 2 * X, 2 * Y, material_value=slab_eps, layer=5),
+                      zc - t / 2, zc + t / 2)
+    for _ in range(int(rng.integers(4, 13))):
+        cx, cy = rng.uniform(0.2 * X, 0.8 * X), rng.uniform(0.2 * Y, 0.8 * Y)
+        a, b = rng.uniform(0.1, 0.5, size=2)
+        eps.add_primitive(Ellipse(cx, cy, a, b, rng.uniform(0, np.pi),
+                                  material_value=float(rng.uniform(EPS_OX, EPS_SI)), layer=1),
+                          zc - t / 2, zc + t / 2)
+    mu = ConstantMaterial3D(1.0)
+    mats = [sample(eps if c < 3 else mu, Nx, Ny, Nz, c) for c in range(6)]
+    adjoint = q >= n_fwd
+    src = None
+    if not adjoint:
+        k = int(rng.integers(w + 2, Nx - w - 2)); j = int(rng.integers(w + 2, Ny - w - 2))
+        i = int(round(zc / d))
+        arrs = [np.zeros((1, 1, 1), dtype=np.complex128) for _ in range(6)]
+        arrs[2][0, 0, 0] = 1.0
+        src = (arrs, (k, k + 1, j, j + 1, i, i + 1))
+    else:
+        k = int(rng.integers(w + 2, Nx - w - 2))
+        J0, J1, I0, I1 = w + 1, Ny - w - 1, w + 1, Nz - w - 1
+        y = (np.arange(J0, J1) * d)[None, :]
+        z = (np.arange(I0, I1) * d)[:, None]
+        g = np.exp(-((y - Y / 2) ** 2 + (z - zc) ** 2) / (2 * 0.6 ** 2))
+        ph = np.exp(1j * rng.uniform(0, 2 * np.pi))
+        arrs = [np.zeros((I1 - I0, J1 - J0, 1), dtype=np.complex128) for _ in range(6)]
+        arrs[2][:, :, 0] = g * ph
+        arrs[4][:, :, 0] = 0.5 * g * ph * 1j
+        src = (arrs, (k, k + 1, J0, J1, I0, I1))
+    return dict(Nx=Nx, Ny=Ny, Nz=Nz, d=d, wl=wl, X=X, Y=Y, Z=Z, w=[w] * 6, rtol=rtol,
+                ramp=None, bc="000", mats=mats, sources=[] if adjoint else [src],
+                adj_sources=[src] if adjoint else [], adjoint=adjoint)
+
+
+def build_sim(case, device=0):
+    """FDTD object for `case` with materials uploaded and (adjoint) sources set."""
+    import warnings
+    from emopt_amd.fdtd import FDTD
+    from emopt_amd.misc import DomainCoordinates
+    d = case["d"]
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim = FDTD(case["X"], case["Y"], case["Z"], d, d, d, case["wl"], rtol=case["rtol"],
+                   complex_eps=case.get("ceps", False), device=device)
+        _configure(sim, case, DomainCoordinates)
+    for arrs, (k1, k2, j1, j2, i1, i2) in case.get("adj_sources", []):
+        sim.add_adjoint_sources(arrs, DomainCoordinates.from_indices(k1, k2, j1, j2, i1, i2, d, d, d))
+    sim.set_material_arrays(case["mats"][:3], case["mats"][3:])
+    sim.verbose = 0
+    return sim
+
+
+def all_fields(sim, case, adjoint=False):
+    from emopt_amd.misc import DomainCoordinates
+    d = case["d"]
+    full = DomainCoordinates.from_indices(0, case["Nx"], 0, case["Ny"], 0, case["Nz"], d, d, d)
+    get = sim.get_adjoint_field if adjoint else sim.get_field
+    return [get(c, full) for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")]
