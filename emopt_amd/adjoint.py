@@ -1,0 +1,414 @@
+"""Adjoint gradients of CW-FDTD figures of merit, and a differentiable solve.
+
+Restated from EMopt (semantics and conventions kept, so a user's adjoint
+method ports unchanged):
+
+* :class:`ModeMatch`             emopt/fomutils.py:925-1196
+* :func:`interpolated_dFdx_3D`   emopt/fomutils.py:1282-1429
+* :func:`power_norm_dFdx_3D`     emopt/fomutils.py:1654-1920
+* :class:`AdjointMethod`         emopt/adjoint_method.py:97-547 (the FDTD
+  subset: fom, dFdx, adjoint solve, the per-parameter gradient loop)
+* :class:`AdjointMethodPNF3D`    emopt/adjoint_method.py:1329-1486
+
+Conventions (those of the reference): dFdx are the derivatives of F with
+respect to the complex fields (z, not conj(z)); they are injected as the
+adjoint J (from dF/dE) and M (from dF/dH) currents; the gradient is
+``-2 Re(sum E_adj * 1j * dA/dp * E)``, A's diagonal being eps (mu is never a
+design variable here).
+
+On top of that, :class:`FDTDAdjointFunction` is a ``torch.autograd.Function``:
+forward = material update + forward solve + FOM; backward = adjoint solve +
+the hand-written gradient-assembly kernel (``emf_grad_assemble``) over the
+boundary cells whose eps moves with each parameter, fed by
+``AdjointMethod.calc_deps`` (sparse dEps/dp, exact for the piecewise-linear
+fill fractions of moving rectangle edges).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .misc import DomainCoordinates
+
+__all__ = ["ModeMatch", "interpolated_dFdx_3D", "power_norm_dFdx_3D", "AdjointMethod",
+           "AdjointMethodPNF3D", "FDTDAdjointFunction", "fdtd_fom", "rect_edge_deps"]
+
+
+class ModeMatch:
+    """Projection of fields onto reference fields through a plane
+    (emopt/fomutils.py:925-1196): ``mode_match_fwd`` is the power of the
+    fields carried by the reference mode."""
+
+    def __init__(self, normal, ds1, ds2=1.0, Exm=None, Eym=None, Ezm=None, Hxm=None, Hym=None,
+                 Hzm=None):
+        given = [Exm, Eym, Ezm, Hxm, Hym, Hzm]
+        shp = next((np.shape(f) for f in given if f is not None), None)
+        if shp is None:
+            raise ValueError("ModeMatch needs at least one reference field")
+        z = lambda f: np.zeros(shp, dtype=np.complex128) if f is None else np.asarray(f)
+        self.fshape = shp
+        self.Exm, self.Eym, self.Ezm = z(Exm), z(Eym), z(Ezm)
+        self.Hxm, self.Hym, self.Hzm = z(Hxm), z(Hym), z(Hzm)
+        self.normal = np.array(normal)
+        self.ds1, self.ds2 = ds1, ds2
+        self.sx, self.sy, self.sz = (float(v) for v in self.normal)
+        ds = ds1 * ds2
+        Pxm = self.Eym * np.conj(self.Hzm) - self.Ezm * np.conj(self.Hym)
+        Pym = -self.Exm * np.conj(self.Hzm) + self.Ezm * np.conj(self.Hxm)
+        Pzm = self.Exm * np.conj(self.Hym) - self.Eym * np.conj(self.Hxm)
+        self.Pm = ds * np.sum(self.sx * Pxm + self.sy * Pym + self.sz * Pzm)
+        self.am = self.bm = 0.0
+        self.mode_match_fwd = self.mode_match_back = 0.0
+
+    def compute(self, Ex=None, Ey=None, Ez=None, Hx=None, Hy=None, Hz=None):
+        z = lambda f: np.zeros(self.fshape, dtype=np.complex128) if f is None else np.asarray(f)
+        Ex, Ey, Ez, Hx, Hy, Hz = z(Ex), z(Ey), z(Ez), z(Hx), z(Hy), z(Hz)
+        ds = self.ds1 * self.ds2
+        sx, sy, sz = self.sx, self.sy, self.sz
+        c = np.conj
+        self.am1 = 0.5 * ds * np.sum((Ey * c(self.Hzm) - Ez * c(self.Hym)) * sx
+                                     - (Ex * c(self.Hzm) - Ez * c(self.Hxm)) * sy
+                                     + (Ex * c(self.Hym) - Ey * c(self.Hxm)) * sz) / self.Pm
+        self.am2 = 0.5 * ds * np.sum((c(self.Eym) * Hz - c(self.Ezm) * Hy) * sx
+                                     - (c(self.Exm) * Hz - c(self.Ezm) * Hx) * sy
+                                     + (c(self.Exm) * Hy - c(self.Eym) * Hx) * sz) / c(self.Pm)
+        self.am = self.am1 + self.am2
+        self.bm = self.am1 - self.am2
+        self.mode_match_fwd = 0.5 * self.am * np.conj(self.am) * np.real(self.Pm)
+        self.mode_match_back = 0.5 * self.bm * np.conj(self.bm) * np.real(self.Pm)
+
+    def get_mode_match_forward(self, P_in):
+        return np.real(self.mode_match_fwd) / P_in
+
+    def get_mode_match_back(self, P_in):
+        return np.real(self.mode_match_back) / P_in
+
+    # dF/dfield of the unnormalised forward mode match (emopt/fomutils.py:1118-1196)
+    def _k(self, conj_pm):
+        ds = self.ds1 * self.ds2
+        Pm = np.conj(self.Pm) if conj_pm else self.Pm
+        return 0.25 * ds * np.real(self.Pm) * np.conj(self.am) / Pm
+
+    def get_dFdEx(self):
+        return self._k(False) * (-np.conj(self.Hzm) * self.sy + np.conj(self.Hym) * self.sz)
+
+    def get_dFdEy(self):
+        return self._k(False) * (np.conj(self.Hzm) * self.sx - np.conj(self.Hxm) * self.sz)
+
+    def get_dFdEz(self):
+        return self._k(False) * (-np.conj(self.Hym) * self.sx + np.conj(self.Hxm) * self.sy)
+
+    def get_dFdHx(self):
+        return self._k(True) * (np.conj(self.Ezm) * self.sy - np.conj(self.Eym) * self.sz)
+
+    def get_dFdHy(self):
+        return self._k(True) * (-np.conj(self.Ezm) * self.sx + np.conj(self.Exm) * self.sz)
+
+    def get_dFdHz(self):
+        return self._k(True) * (np.conj(self.Eym) * self.sx - np.conj(self.Exm) * self.sy)
+
+
+def interpolated_dFdx_3D(sim, domain, dFdExi, dFdEyi, dFdEzi, dFdHxi, dFdHyi, dFdHzi):
+    """Map derivatives with respect to fields interpolated onto the Ez grid
+    (get_field_interp) back onto the Yee fields: the transpose of the
+    interpolation stencils (emopt/fomutils.py:1282-1429).  Returns the grown
+    domain (one cell more at -x, -y and +z) and six arrays on it."""
+    Nz, Ny, Nx = domain.shape
+    shp = (Nz + 2, Ny + 2, Nx + 2)
+    I, O, M = slice(1, -1), slice(0, -2), slice(2, None)   # i/j/k, minus one, plus one
+    bc = sim.bc
+
+    def zeros():
+        return np.zeros(shp, dtype=np.complex128)
+
+    dEx = zeros()
+    for s in ((I, I, I), (I, I, O), (M, I, I), (M, I, O)):
+        dEx[s] += dFdExi / 4.0
+    if domain.k1 == 0 and bc[0] == "H":
+        dEx[I, I, 1] += dFdExi[:, :, 0] / 4.0
+        dEx[M, I, 1] += dFdExi[:, :, 0] / 4.0
+    elif domain.k1 == 0 and bc[0] == "E":
+        # as in the reference, this branch indexes the y axis
+        dEx[I, 1, I] -= dFdExi[:, 0, :] / 4.0
+        dEx[M, 1, I] -= dFdExi[:, 0, :] / 4.0
+
+    dEy = zeros()
+    for s in ((I, I, I), (I, O, I), (M, I, I), (M, O, I)):
+        dEy[s] += dFdEyi / 4.0
+    if domain.j1 == 0 and bc[1] in "HE":
+        sgn = 1.0 if bc[1] == "H" else -1.0
+        dEy[I, 1, I] += sgn * dFdEyi[:, 0, :] / 4.0
+        dEy[M, 1, I] += sgn * dFdEyi[:, 0, :] / 4.0
+
+    dEz = zeros()
+    dEz[I, I, I] = dFdEzi
+
+    dHx = zeros()
+    for s in ((I, I, I), (I, O, I)):
+        dHx[s] += dFdHxi / 2.0
+    if domain.j1 == 0 and bc[1] in "HE":
+        sgn = 1.0 if bc[1] == "H" else -1.0
+        dHx[I, 1, I] += sgn * dFdHxi[:, 0, :] / 2.0
+
+    dHy = zeros()
+    for s in ((I, I, I), (I, I, O)):
+        dHy[s] += dFdHyi / 2.0
+
+    dHz = zeros()
+    for s in ((I, I, I), (I, O, I), (I, I, O), (I, O, O), (M, I, I), (M, O, I), (M, I, O),
+              (M, O, O)):
+        dHz[s] += dFdHzi / 8.0
+    if domain.j1 == 0 and bc[1] in "HE":
+        sgn = 1.0 if bc[1] == "H" else -1.0
+        for s in ((I, 1, I), (I, 1, O), (M, 1, I), (M, 1, O)):
+            dHz[s] += sgn * dFdHzi[:, 0, :] / 8.0
+
+    crop = (slice(1, None), slice(0, -1), slice(0, -1))
+    out = [a[crop] for a in (dEx, dEy, dEz, dHx, dHy, dHz)]
+    dom = domain.copy()
+    dom.set_region(dom.k1 - 1, dom.k2, dom.j1 - 1, dom.j2, dom.i1, dom.i2 + 1, dom.dx, dom.dy,
+                   dom.dz)
+    return (dom, *out)
+
+
+def _box_planes(sim):
+    """The six planes of the source-power box (emopt/fomutils.py:1710-1745)."""
+    w = sim.w_pml
+    dx, dy, dz = sim.dx, sim.dy, sim.dz
+    X, Y, Z = sim.X, sim.Y, sim.Z
+    xmin = w[0] + dx if w[0] > 0 else 0.0
+    xmax = X - w[1] - dx if w[1] > 0 else X
+    ymin = w[2] + dy if w[2] > 0 else 0.0
+    ymax = Y - w[3] - dy if w[3] > 0 else Y
+    zmin = w[4] + dz if w[4] > 0 else 0.0
+    zmax = Z - w[5] - dz if w[5] > 0 else Z
+    D = lambda *a: DomainCoordinates(*a, dx, dy, dz)
+    return (D(xmin, xmin, ymin, ymax, zmin, zmax), D(xmax, xmax, ymin, ymax, zmin, zmax),
+            D(xmin, xmax, ymin, ymin, zmin, zmax), D(xmin, xmax, ymax, ymax, zmin, zmax),
+            D(xmin, xmax, ymin, ymax, zmin, zmin), D(xmin, xmax, ymin, ymax, zmax, zmax))
+
+
+def power_norm_dFdx_3D(sim, f, domain, dfdEx, dfdEy, dfdEz, dfdHx, dfdHy, dfdHz):
+    """Adjoint sources of F = f / Psrc (emopt/fomutils.py:1654-1920): the
+    interpolation-compensated df/dx / Psrc on `domain`, plus -f dPsrc/dx /
+    Psrc^2 on the six planes of the source-power box (min planes skipped under
+    an 'E'/'H' boundary).  Returns [sources, domains] for
+    ``set_adjoint_sources``."""
+    Psrc = sim.source_power
+    srcs, doms = [], []
+    d = interpolated_dFdx_3D(sim, domain, dfdEx / Psrc, dfdEy / Psrc, dfdEz / Psrc,
+                             dfdHx / Psrc, dfdHy / Psrc, dfdHz / Psrc)
+    srcs.append(d[1:]); doms.append(d[0])
+    dx, dy, dz = sim.dx, sim.dy, sim.dz
+    c = np.conj
+    g = sim.get_field_interp
+    x1, x2, y1, y2, z1, z2 = _box_planes(sim)
+    # (plane, sign, skip-if-symmetric axis, components (E1, E2, H1, H2), area)
+    # dS/dE and dS/dH of +/- 0.5 Re(E1 H2* - E2 H1*) for the plane's normal
+    faces = ((x1, -1, 0, ("Ey", "Ez", "Hy", "Hz"), dy * dz),
+             (x2, +1, None, ("Ey", "Ez", "Hy", "Hz"), dy * dz),
+             (y1, -1, 1, ("Ez", "Ex", "Hz", "Hx"), dx * dz),
+             (y2, +1, None, ("Ez", "Ex", "Hz", "Hx"), dx * dz),
+             (z1, -1, 2, ("Ex", "Ey", "Hx", "Hy"), dx * dy),
+             (z2, +1, None, ("Ex", "Ey", "Hx", "Hy"), dx * dy))
+    order = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+    for dom, sgn, sym_axis, (e1, e2, h1, h2), area in faces:
+        if sym_axis is not None and sim.bc[sym_axis] in "EH":
+            continue
+        F = {k: g(k, dom) for k in (e1, e2, h1, h2)}
+        dS = {k: np.zeros(dom.shape, dtype=np.complex128) for k in order}
+        # S_n = 0.5 Re(E1 H2* - E2 H1*): dS/dE1 = H2*/4, dS/dE2 = -H1*/4,
+        # dS/dH1 = -E2*/4, dS/dH2 = E1*/4 (times the plane's sign and area)
+        dS[e1] = sgn * 0.25 * area * c(F[h2])
+        dS[e2] = -sgn * 0.25 * area * c(F[h1])
+        dS[h1] = -sgn * 0.25 * area * c(F[e2])
+        dS[h2] = sgn * 0.25 * area * c(F[e1])
+        dP = [-1 * f * dS[k] / Psrc ** 2 for k in order]
+        d = interpolated_dFdx_3D(sim, dom, *dP)
+        srcs.append(d[1:]); doms.append(d[0])
+    return [srcs, doms]
+
+
+def rect_edge_deps(sim, eps, prim, attr, delta, bbox):
+    """Sparse dEps/dp of moving one coordinate of a primitive: the cells of
+    `bbox` (index box k1, k2, j1, j2, i1, i2) whose sampled eps changes,
+    from a central difference of the material sampling with a step `delta`
+    that stays inside one cell.  Rectangle fill fractions are linear in an
+    edge position within a cell, so this is the exact derivative.  Returns
+    (component, flat cell index, complex dEps) arrays."""
+    k1, k2, j1, j2, i1, i2 = bbox
+    comps, cells, vals = [], [], []
+    v0 = getattr(prim, attr)
+    samples = []  # eps at p + delta and p - delta, per component
+    for sgn in (+1, -1):
+        setattr(prim, attr, v0 + sgn * delta)
+        samples.append([eps.get_values(k1, k2, j1, j2, i1, i2, sx=sx, sy=sy, sz=sz)
+                        for (sx, sy, sz) in sim.YEE_SHIFTS[:3]])
+    setattr(prim, attr, v0)
+    for c in range(3):
+        d = (samples[0][c] - samples[1][c]) / (2 * delta)
+        nz = np.nonzero(np.abs(d) > 0)
+        if len(nz[0]) == 0:
+            continue
+        i = nz[0] + i1; j = nz[1] + j1; k = nz[2] + k1
+        comps.append(np.full(len(i), c, dtype=np.int32))
+        cells.append(((i.astype(np.int64) * sim.Ny + j) * sim.Nx + k))
+        vals.append(d[nz])
+    if not comps:
+        return (np.zeros(0, np.int32), np.zeros(0, np.int64), np.zeros(0, np.complex128))
+    return np.concatenate(comps), np.concatenate(cells), np.concatenate(vals)
+
+
+class AdjointMethod:
+    """The FDTD subset of emopt.adjoint_method.AdjointMethod
+    (emopt/adjoint_method.py:97-547).  Subclasses define update_system,
+    calc_fom, calc_dFdx, calc_grad_p and get_update_boxes as in the
+    reference; optionally calc_deps (sparse dEps/dp) for the device gradient
+    assembly used by :class:`FDTDAdjointFunction`."""
+
+    def __init__(self, sim, step=1e-8):
+        self.sim = sim
+        self._step = step
+
+    # --- user hooks (emopt/adjoint_method.py:130-260)
+    def update_system(self, params):
+        raise NotImplementedError
+
+    def calc_fom(self, sim, params):
+        raise NotImplementedError
+
+    def calc_dFdx(self, sim, params):
+        raise NotImplementedError
+
+    def calc_grad_p(self, sim, params):
+        return np.zeros(len(params))
+
+    def get_update_boxes(self, sim, params):
+        N = len(params)
+        return [(0, sim.X, 0, sim.Y, 0, sim.Z) for _ in range(N)]
+
+    def calc_deps(self, sim, params):
+        """(pidx, comp, cell, deps) of every parameter; needed only by the
+        device gradient assembly."""
+        raise NotImplementedError
+
+    # --- the solves
+    def fom(self, params):
+        self.update_system(params)
+        self.sim.update()
+        self.sim.solve_forward()
+        return self.calc_fom(self.sim, params)
+
+    def solve_adjoint_for(self, params):
+        src = self.calc_dFdx(self.sim, params)
+        self.sim.set_adjoint_sources(src)
+        self.sim.solve_adjoint()
+
+    def calc_gradient(self, sim, params):
+        """The reference's gradient loop (emopt/adjoint_method.py:448-547):
+        per parameter, step it, re-sample eps in its update box(es), and
+        take -2 Re(y^T (A - A0) x) / step."""
+        Ai = sim.get_A_diag()
+        step = self._step
+        boxes = self.get_update_boxes(sim, params)
+        params = np.array(params, dtype=np.double)
+        grad = np.zeros(len(params))
+        for i in range(len(params)):
+            p0 = params[i]
+            ub = boxes[i]
+            ubs = ub if isinstance(ub[0], (list, tuple, np.ndarray)) else [ub]
+            params[i] = p0 + step
+            self.update_system(params)
+            for box in ubs:
+                sim.update(box)
+            grad[i] = -2 * np.real(sim.calc_ydAx(Ai) / step)
+            params[i] = p0
+            self.update_system(params)
+            for box in ubs:
+                sim.update(box)
+        return grad
+
+    def gradient(self, params):
+        """Forward solve, adjoint solve, reference gradient loop."""
+        self.fom(params)
+        self.solve_adjoint_for(params)
+        return self.calc_gradient(self.sim, params) + self.calc_grad_p(self.sim, params)
+
+    def gradient_device(self, params, fom_done=False):
+        """Forward (unless fom_done) and adjoint solves, then the device
+        gradient assembly from calc_deps."""
+        if not fom_done:
+            self.fom(params)
+        self.solve_adjoint_for(params)
+        pidx, comp, cell, deps = self.calc_deps(self.sim, params)
+        g = self.sim.grad_assemble(pidx, comp, cell, deps, len(params))
+        return g + self.calc_grad_p(self.sim, params)
+
+
+class AdjointMethodPNF3D(AdjointMethod):
+    """Power-normalised FOM F = f / Psrc (emopt/adjoint_method.py:1329-1486):
+    subclasses define calc_f, calc_dfdx (derivatives of f with respect to
+    the interpolated fields on the FOM domain) and get_fom_domains."""
+
+    def calc_f(self, sim, params):
+        raise NotImplementedError
+
+    def calc_dfdx(self, sim, params):
+        raise NotImplementedError
+
+    def get_fom_domains(self):
+        raise NotImplementedError
+
+    def calc_penalty(self, sim, params):
+        return 0.0
+
+    def calc_fom(self, sim, params):
+        return self.calc_f(sim, params) / sim.source_power + self.calc_penalty(sim, params)
+
+    def calc_dFdx(self, sim, params):
+        f = self.calc_f(sim, params)
+        dfdxs = self.calc_dfdx(sim, params)
+        srcs = [[], []]
+        for dfdx, dom in zip(dfdxs, self.get_fom_domains()):
+            a = power_norm_dFdx_3D(sim, f, dom, *dfdx)
+            srcs = [srcs[0] + a[0], srcs[1] + a[1]]
+        return srcs
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class FDTDAdjointFunction:
+    """``torch.autograd.Function`` whose forward is the FDTD forward solve
+    and FOM, and whose backward is the adjoint solve plus the device gradient
+    assembly.  Use through :func:`fdtd_fom`."""
+    _cls = None
+
+    @classmethod
+    def get(cls):
+        if cls._cls is None:
+            torch = _torch()
+
+            class _Fn(torch.autograd.Function):
+                @staticmethod
+                def forward(ctx, params, am):
+                    p = params.detach().cpu().numpy().astype(np.double).copy()
+                    F = am.fom(p)
+                    ctx.am = am
+                    ctx.p = p
+                    return torch.tensor(float(np.real(F)), dtype=torch.float64)
+
+                @staticmethod
+                def backward(ctx, gF):
+                    g = ctx.am.gradient_device(ctx.p, fom_done=True)
+                    return gF * torch.from_numpy(np.asarray(g, dtype=np.double)), None
+
+            cls._cls = _Fn
+        return cls._cls
+
+
+def fdtd_fom(params, am):
+    """Differentiable FOM: ``F = fdtd_fom(p, am); F.backward()`` gives
+    p.grad from one forward and one adjoint solve."""
+    return FDTDAdjointFunction.get().apply(params, am)

@@ -462,14 +462,33 @@ class FDTD(MaxwellSolver):
         self._lib.emf_get_mat_arrays(self._h, *[ptr(a) for a in arrs])
         return tuple(arrs)
 
-    def calc_ydAx(self, Adiag0):
+    def calc_ydAx(self, Adiag0, box=None):
         """y^T (A - A0) x on the device (emopt/fdtd.py:1694-1717); the mu
-        term vanishes because mu is never updated."""
+        term vanishes because mu is never updated.  Adiag0 holds full-grid
+        arrays (as get_A_diag returns them); with box = (k1, k2, j1, j2, i1,
+        i2) only that index box is summed (the rest of A - A0 is zero)."""
         out = np.zeros(2)
-        e0 = [c128(a).ravel() for a in Adiag0[:3]]
-        self._lib.emf_calc_ydAx(self._h, *[ptr(a) for a in e0], 0, self._Nx, 0, self._Ny, 0,
-                                self._Nz, ptr(out))
+        k1, k2, j1, j2, i1, i2 = box if box is not None else (0, self._Nx, 0, self._Ny, 0, self._Nz)
+        shp = (self._Nz, self._Ny, self._Nx)
+        e0 = [c128(np.asarray(a).reshape(shp)[i1:i2, j1:j2, k1:k2]).ravel() for a in Adiag0[:3]]
+        self._lib.emf_calc_ydAx(self._h, *[ptr(a) for a in e0], k1, k2, j1, j2, i1, i2, ptr(out))
         return complex(out[0], out[1])
+
+    def grad_assemble(self, pidx, comp, cell, deps, nparam, step=1.0):
+        """grad[p] = -2 Re(sum E_adj * 1j * deps * E) / step over sparse
+        (parameter, component, cell, dEps) entries, on the device
+        (k_grad): the reference's per-parameter loop
+        (emopt/adjoint_method.py:504-533) in one pass."""
+        pidx = np.asarray(pidx, dtype=np.int32)
+        order = np.argsort(pidx, kind="stable")
+        pidx = np.ascontiguousarray(pidx[order])
+        comp = np.ascontiguousarray(np.asarray(comp, dtype=np.int32)[order])
+        cell = np.ascontiguousarray(np.asarray(cell, dtype=np.int64)[order])
+        deps = c128(np.asarray(deps, dtype=np.complex128)[order])
+        grad = np.zeros(int(nparam))
+        self._lib.emf_grad_assemble(self._h, len(pidx), ptr(pidx), ptr(comp), ptr(cell),
+                                    ptr(deps), int(nparam), float(step), ptr(grad))
+        return grad
 
     # ---------------------------------------------------------- step-level
     def time_fields(self):

@@ -270,3 +270,175 @@ def all_fields(sim, case, adjoint=False):
     full = DomainCoordinates.from_indices(0, case["Nx"], 0, case["Ny"], 0, case["Nz"], d, d, d)
     get = sim.get_adjoint_field if adjoint else sim.get_field
     return [get(c, full) for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")]
+
+
+# ------------------------------------------------------- grating coupler
+
+def _edge_rect_cls():
+    from emopt_amd.grid import Rectangle
+
+    class EdgeRect(Rectangle):
+        """Axis-aligned rectangle given by its x edges (xl, xr) and y span."""
+
+        def __init__(self, xl, xr, y0, yspan, material_value=1.0, layer=1):
+            super().__init__(0.5 * (xl + xr), y0, xr - xl, yspan, material_value, layer)
+            self.xl, self.xr = xl, xr
+
+        def bbox(self):
+            return (self.xl, self.xr, self.y0 - self.height / 2, self.y0 + self.height / 2)
+
+    return EdgeRect
+
+
+def grating_problem(Nx=360, Ny=60, Nz=120, d=0.03, ntr=32, w=10, wl=1.55, seed=4,
+                    w0=10.4, theta_deg=8.0, rtol=1e-4, t_si=0.22, t_box=2.0, etch=0.11,
+                    pitch=(0.3, 0.7)):
+    """Grating-coupler-like slab: Si (t_si) on oxide (t_box above the bottom
+    CPML), air above; ntr etched trenches (depth `etch`) whose 2*ntr edge
+    positions are the design parameters, pitches drawn uniform in `pitch`
+    (seed) and scaled together so the grating fits the window between 0.3 um
+    past the x CPML on both sides, duty cycle 0.5.  A Jy line current across
+    the slab at x index w + 3 launches the slab's TE wave; the FOM is the
+    power-normalised overlap, on a z plane 0.4 um above the slab, with a
+    Gaussian beam of waist w0 tilted by theta in the x-z plane, E along y.
+    Returns (sim, am, params)."""
+    import warnings
+    from emopt_amd.adjoint import AdjointMethodPNF3D, ModeMatch, rect_edge_deps
+    from emopt_amd.fdtd import FDTD
+    from emopt_amd.grid import StructuredMaterial3D, Rectangle, ConstantMaterial3D
+    from emopt_amd.misc import DomainCoordinates, gaussian_mode
+    EdgeRect = _edge_rect_cls()
+    X, Y, Z = span(Nx, d), span(Ny, d), span(Nz, d)
+    wd = w * d
+    z_si0 = wd + t_box
+    z_si1 = z_si0 + t_si
+    z_mon = z_si1 + 0.4
+    if z_mon > Z - wd - 2 * d:
+        raise ValueError("grid too short for the monitor plane")
+    eps = StructuredMaterial3D(X, Y, Z, d, d, d, supersample=4)
+    eps.add_primitive(Rectangle(X / 2, Y / 2, 4 * X, 4 * Y, material_value=EPS_OX, layer=3), -Z, z_si0)
+    eps.add_primitive(Rectangle(X / 2, Y / 2, 4 * X, 4 * Y, material_value=EPS_SI, layer=2), z_si0, z_si1)
+    rng = np.random.default_rng(seed)
+    pit = rng.uniform(pitch[0], pitch[1], size=ntr)
+    x_start, x_end = wd + 0.3 + 4 * d, X - wd - 0.3
+    pit *= (x_end - x_start) / pit.sum()
+    edges = []
+    x = x_start
+    trenches = []
+    for q in range(ntr):
+        xl, xr = x + 0.25 * pit[q], x + 0.75 * pit[q]
+        edges += [xl, xr]
+        r = EdgeRect(xl, xr, Y / 2, 4 * Y, material_value=1.0, layer=1)
+        eps.add_primitive(r, z_si1 - etch, z_si1)
+        trenches.append(r)
+        x += pit[q]
+    params = np.array(edges)
+    mu = ConstantMaterial3D(1.0)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim = FDTD(X, Y, Z, d, d, d, wl, rtol=rtol)
+        sim.w_pml = [wd + 1e-9 * d] * 6
+    sim.set_materials(eps, mu)
+    # source: Jy across the slab (Gaussian in z), all y
+    ks = w + 3
+    i0, i1 = int(z_si0 / d) - 3, int(z_si1 / d) + 4
+    z = np.arange(i0, i1) * d
+    prof = np.exp(-((z - 0.5 * (z_si0 + z_si1)) / 0.12) ** 2)
+    arrs = [np.zeros((i1 - i0, Ny, 1), dtype=np.complex128) for _ in range(6)]
+    arrs[1][:, :, 0] = prof[:, None]
+    sim.set_sources(arrs, DomainCoordinates.from_indices(ks, ks + 1, 0, Ny, i0, i1, d, d, d))
+    # FOM plane and target beam
+    mon = DomainCoordinates(wd + d, X - wd - d, wd + d, Y - wd - d, z_mon, z_mon, d, d, d)
+    sim.field_domains = [mon]
+    xs = mon.x - 0.5 * (x_start + x_end)
+    ys = mon.y - Y / 2
+    XX, YY = np.meshgrid(xs, ys)
+    Eg, Hxg, Hyg = gaussian_mode(XX, YY, w0, theta_deg * np.pi / 180, wl, 1.0)
+    mm = ModeMatch([0, 0, 1], d, d, Eym=Eg[None], Hxm=-Hxg[None], Hzm=-Hyg[None])
+    e_i0, e_i1 = int((z_si1 - etch) / d) - 1, int(z_si1 / d) + 2
+
+    class GratingAM(AdjointMethodPNF3D):
+        def update_system(self, p):
+            for q, r in enumerate(trenches):
+                r.xl, r.xr = float(p[2 * q]), float(p[2 * q + 1])
+                r.x0, r.width = 0.5 * (r.xl + r.xr), r.xr - r.xl
+
+        def get_update_boxes(self, sim, p):
+            return [(p[i] - 2 * d, p[i] + 2 * d, 0, Y, z_si1 - etch - 2 * d, z_si1 + 2 * d)
+                    for i in range(len(p))]
+
+        def get_fom_domains(self):
+            return [mon]
+
+        def calc_f(self, sim, p):
+            Ex, Ey, Ez, Hx, Hy, Hz = sim.saved_fields[0]
+            mm.compute(Ex=Ex, Ey=Ey, Hx=Hx, Hy=Hy)
+            return mm.get_mode_match_forward(1.0)
+
+        def calc_dfdx(self, sim, p):
+            Ex, Ey, Ez, Hx, Hy, Hz = sim.saved_fields[0]
+            mm.compute(Ex=Ex, Ey=Ey, Hx=Hx, Hy=Hy)
+            zero = np.zeros(mon.shape, dtype=np.complex128)
+            return [(mm.get_dFdEx(), mm.get_dFdEy(), zero, mm.get_dFdHx(), mm.get_dFdHy(), zero)]
+
+        def calc_deps(self, sim, p):
+            out = [[], [], [], []]
+            for i in range(len(p)):
+                r = trenches[i // 2]
+                k = int(p[i] / d)
+                comp, cell, dv = rect_edge_deps(sim, eps, r, "xl" if i % 2 == 0 else "xr",
+                                                1e-6 * d, (max(k - 2, 0), min(k + 3, sim.Nx), 0,
+                                                           sim.Ny, e_i0, e_i1))
+                out[0].append(np.full(len(comp), i, np.int32))
+                out[1].append(comp); out[2].append(cell); out[3].append(dv)
+            return tuple(np.concatenate(a) for a in out)
+
+    am = GratingAM(sim, step=1e-8)
+    am.update_system(params)
+    return sim, am, params
+
+
+def serve_case(q, seed=5, lo=(64, 64, 64), hi=(320, 240, 120), n_ell=12, w=10, rtol=1e-4,
+               materials=True):
+    """Request q of the serving workload: grid dims log-uniform between `lo`
+    and `hi` (one shared log factor), d uniform in 0.03-0.05 um, lambda
+    uniform in 1.26-1.62 um, a slab of random ellipses (eps 3.48^2 in
+    1.444^2) across the middle, CPML `w` cells, planar Ey source across the
+    input face as in the single large workload.  materials=False skips the
+    (host-heavy) material sampling."""
+    from emopt_amd.grid import ConstantMaterial3D
+    rng = np.random.default_rng([seed, q])
+    s = rng.uniform()
+    Nx, Ny, Nz = (int(round(a * (b / a) ** s)) for a, b in zip(lo, hi))
+    d = float(rng.uniform(0.03, 0.05))
+    wl = float(rng.uniform(1.26, 1.62))
+    eseed = int(rng.integers(1 << 30))
+    X, Y, Z = span(Nx, d), span(Ny, d), span(Nz, d)
+    ks = w + 2
+    J0, J1, I0, I1 = w, Ny - w, w, Nz - w
+    src = [np.zeros((I1 - I0, J1 - J0, 1), dtype=np.complex128) for _ in range(6)]
+    src[1][:] = 1.0
+    mats = None
+    if materials:
+        design = (max(Nx - 2 * w - 8, 4), max(Ny - 2 * w - 8, 4), min(10, max(Nz // 6, 2)))
+        eps = ellipse_design_eps(Nx, Ny, Nz, d, design=design, n_ell=n_ell, seed=eseed,
+                                 axes=(0.1, 0.6))
+        mu = ConstantMaterial3D(1.0)
+        mats = [sample(eps if c < 3 else mu, Nx, Ny, Nz, c) for c in range(6)]
+    return dict(Nx=Nx, Ny=Ny, Nz=Nz, d=d, wl=wl, X=X, Y=Y, Z=Z, w=[w] * 6, rtol=rtol,
+                ramp=None, bc="000", mats=mats, sources=[(src, (ks, ks + 1, J0, J1, I0, I1))],
+                adjoint=False)
+
+
+def serve_request(q, **kw):
+    """A serving Request for serve_case(q): dims (for the admission estimate)
+    now, the problem itself built on the server's worker thread."""
+    from emopt_amd.serve import Request, state_bytes
+    c = serve_case(q, materials=False, **kw)
+
+    def build():
+        return build_sim(serve_case(q, **kw))
+    req = Request(q, build, state_bytes(c["Nx"], c["Ny"], c["Nz"], c["w"]), adjoint=False,
+                  cells=c["Nx"] * c["Ny"] * c["Nz"],
+                  meta=dict(dims=(c["Nx"], c["Ny"], c["Nz"]), wl=c["wl"], d=c["d"]))
+    return req
