@@ -24,8 +24,16 @@ Also reported:
                 one thread, on a bounded sample of the same workload
   cpu_all_cores the C restatement built with OpenMP, all host threads
   solve         one full converged solve (solves/hour)
+  batched       32 independent solves of uneven size and wavelength (half
+                forward dipoles, half adjoint plane sources): one after the
+                other through the single-solve path, then all together in one
+                batch; solves/hour, latency p50/p99, result agreement
+  serving       Poisson arrivals of solve requests of mixed grid size served
+                by the continuous-batching server under an HBM budget, and
+                the same requests through the single-solve path as a FIFO
+                queue; solves/hour and latency p50/p99 of both
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch N] [--serve-seconds S]
     python bench.py --profile-summary --stats S.csv [--fetch F.csv --write W.csv]
 """
 from __future__ import annotations
@@ -162,6 +170,118 @@ def read_traffic(path=TRAFFIC_FILE):
     return out
 
 
+def _pcts(lat):
+    lat = np.asarray(lat, dtype=np.double)
+    return dict(p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)))
+
+
+def _plain_solve(sim, adjoint):
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        if adjoint:
+            sim._register(sim._adj_sources)
+            sim._solve(1)
+        else:
+            sim._register(sim._sources)
+            sim._solve(0)
+
+
+def bench_batched(n, seed=3):
+    """The batched workload: n solves, dims uniform in [64,256]x[64,160]x
+    [48,96] at d = 0.04 um, lambda 1.50-1.60 um, 8-12 cell CPML, half with a
+    forward Jz dipole, half with an adjoint plane source.  All submitted at
+    t = 0: the single-solve path runs them one after the other, the batch
+    runs them together."""
+    from emopt_amd.batch import Batch
+    from tests.cases import all_fields, batch_case, build_sim, rel_l2
+    t = time.perf_counter()
+    cases = [batch_case(q, seed=seed, n_fwd=n // 2) for q in range(n)]
+    cells = [c["Nx"] * c["Ny"] * c["Nz"] for c in cases]
+    log("[bench] batched workload built in %.1fs (%d solves, %.3g cells)"
+        % (time.perf_counter() - t, n, sum(cells)))
+    sims = [build_sim(c) for c in cases]
+    _plain_solve(build_sim(cases[0]), cases[0]["adjoint"])        # warm-up
+    plain_t, plain_info = [], []
+    t0 = time.perf_counter()
+    for sim, c in zip(sims, cases):
+        _plain_solve(sim, c["adjoint"])
+        plain_t.append(time.perf_counter() - t0)
+        plain_info.append(dict(sim.last_solve))
+    tp = time.perf_counter() - t0
+    sample = list(range(0, n, max(1, n // 8)))
+    ref = {q: all_fields(sims[q], cases[q], cases[q]["adjoint"]) for q in sample}
+    log("[bench] batched workload, single-solve path: %.2fs" % tp)
+    b = Batch()
+    t0 = time.perf_counter()
+    for sim, c in zip(sims, cases):
+        b.add(sim, adjoint=c["adjoint"], postprocess=False)
+    b.run()
+    tb = time.perf_counter() - t0
+    log("[bench] batched workload, one batch: %.2fs" % tb)
+    n0_same = all(sims[q].last_solve["n0"] == plain_info[q]["n0"] for q in range(n))
+    err = max(max(rel_l2(x, y) for x, y in zip(all_fields(sims[q], cases[q], cases[q]["adjoint"]),
+                                               ref[q])) for q in sample)
+    steps = sum(i["steps"] * c for i, c in zip(plain_info, cells))
+    lat_b = [v["seconds"] for v in b.finished.values()]
+    b.close()
+    return {"workload": "batch_%d_dims64-256x64-160x48-96_dx0.04_wl1.50-1.60_cpml8-12" % n,
+            "solves": n, "cells_total": int(sum(cells)), "cell_updates_total": int(steps),
+            "plain": dict(seconds=tp, solves_per_hr=n * 3600 / tp, cell_updates_per_s=steps / tp,
+                          **_pcts(plain_t)),
+            "batched": dict(seconds=tb, solves_per_hr=n * 3600 / tb, cell_updates_per_s=steps / tb,
+                            **_pcts(lat_b)),
+            "speedup_solves_per_hr": tp / tb, "n0_identical": bool(n0_same),
+            "max_rel_l2_vs_plain_sample": err, "sample": sample}
+
+
+def bench_serving(rate, seconds, seed=5, budget=200e9):
+    """Poisson arrivals at `rate` per second for `seconds` of requests of the
+    serving workload (dims log-uniform 64^3..320x240x120, d 0.03-0.05 um,
+    lambda 1.26-1.62 um, random ellipses), served by the continuous-batching
+    server; then the same requests through the single-solve path as one FIFO
+    queue with the same arrival times (service times measured one by one)."""
+    from emopt_amd.serve import Server
+    from tests.cases import build_sim, serve_case, serve_request
+    rng = np.random.default_rng(seed)
+    arr, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > seconds:
+            break
+        arr.append(t)
+    reqs = [serve_request(q, seed=seed) for q in range(len(arr))]
+    srv = Server(hbm_budget=budget, prep_workers=8)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        st = srv.run_trace(list(zip(arr, reqs)))
+    srv.close()
+    log("[bench] serving: %d requests, p50 %.2fs p99 %.2fs" % (len(arr), st.get("p50_s", 0),
+                                                               st.get("p99_s", 0)))
+    # single-solve path: FIFO with the same arrivals (host build not timed)
+    service = []
+    steps = 0
+    for q in range(len(arr)):
+        c = serve_case(q, seed=seed)
+        sim = build_sim(c)
+        t0 = time.perf_counter()
+        _plain_solve(sim, False)
+        service.append(time.perf_counter() - t0)
+        steps += sim.last_solve["steps"] * c["Nx"] * c["Ny"] * c["Nz"]
+        del sim
+    fin, lat = 0.0, []
+    for a, s in zip(arr, service):
+        fin = max(fin, a) + s
+        lat.append(fin - a)
+    span = fin
+    plain = dict(solves_per_hr=len(arr) * 3600 / span, cell_updates_per_s=steps / span,
+                 busy_seconds=float(sum(service)), **_pcts(lat))
+    return {"workload": "serve_poisson_%gps_%gs_dims64^3-320x240x120" % (rate, seconds),
+            "requests": len(arr), "budget_bytes": budget, "served": st, "plain_fifo": plain,
+            "p99_ratio_plain_over_served": plain["p99_s"] / st["p99_s"] if st.get("p99_s") else None,
+            "solves_per_hr_ratio": (st["solves_per_hr"] / plain["solves_per_hr"]
+                                    if st.get("solves_per_hr") else None)}
+
+
 # --------------------------------------------------------- profile summaries
 
 def _short(name):
@@ -238,6 +358,10 @@ def main():
     ap.add_argument("--omp-seconds", type=float, default=6.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--batch", type=int, default=32, help="solves in the batched section (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="arrival window of the serving section (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--profile-summary", action="store_true")
     ap.add_argument("--stats")
     ap.add_argument("--fetch")
@@ -329,6 +453,11 @@ def main():
                         "seconds": el, "solves_per_hr": 3600.0 / el,
                         "cell_updates_per_s": cells * ls["steps"] / el}
         log("[bench] full solve: %d steps in %.2fs (%s)" % (ls["steps"], el, ls["status"]))
+
+    if args.batch > 0:
+        res["batched"] = bench_batched(args.batch)
+    if args.serve_seconds > 0:
+        res["serving"] = bench_serving(args.serve_rate, args.serve_seconds)
 
     if not args.no_cpu:
         del sim

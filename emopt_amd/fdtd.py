@@ -288,6 +288,7 @@ class FDTD(MaxwellSolver):
             if a.size != n:
                 raise ValueError("material arrays must have Nz*Ny*Nx = %d entries" % n)
         self._lib.emf_set_mat_arrays(self._h, *[ptr(a) for a in arrs])
+        self._mu_uniform = all(bool(np.all(np.real(a) == np.real(a[0]))) for a in arrs[3:])
         self._mats_ready = True
 
     def update(self, bbox=None):
