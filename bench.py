@@ -234,12 +234,14 @@ def bench_batched(n, seed=3):
             "max_rel_l2_vs_plain_sample": err, "sample": sample}
 
 
-def bench_serving(rate, seconds, seed=5, budget=200e9):
+def bench_serving(rate, seconds, seed=5, budget=200e9, max_running=2):
     """Poisson arrivals at `rate` per second for `seconds` of requests of the
     serving workload (dims log-uniform 64^3..320x240x120, d 0.03-0.05 um,
     lambda 1.26-1.62 um, random ellipses), served by the continuous-batching
     server; then the same requests through the single-solve path as one FIFO
-    queue with the same arrival times (service times measured one by one)."""
+    queue with the same arrival times, each request's service time being its
+    problem build (host material sampling and upload) plus its solve,
+    measured one by one -- the work a plain request loop does per request."""
     from emopt_amd.serve import Server
     from tests.cases import build_sim, serve_case, serve_request
     rng = np.random.default_rng(seed)
@@ -250,7 +252,7 @@ def bench_serving(rate, seconds, seed=5, budget=200e9):
             break
         arr.append(t)
     reqs = [serve_request(q, seed=seed) for q in range(len(arr))]
-    srv = Server(hbm_budget=budget, prep_workers=8)
+    srv = Server(hbm_budget=budget, prep_workers=8, max_running=max_running)
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         st = srv.run_trace(list(zip(arr, reqs)))
@@ -260,12 +262,15 @@ def bench_serving(rate, seconds, seed=5, budget=200e9):
     # single-solve path: FIFO with the same arrivals (host build not timed)
     service = []
     steps = 0
+    build_s = 0.0
     for q in range(len(arr)):
+        t0 = time.perf_counter()
         c = serve_case(q, seed=seed)
         sim = build_sim(c)
-        t0 = time.perf_counter()
+        t1 = time.perf_counter()
         _plain_solve(sim, False)
         service.append(time.perf_counter() - t0)
+        build_s += t1 - t0
         steps += sim.last_solve["steps"] * c["Nx"] * c["Ny"] * c["Nz"]
         del sim
     fin, lat = 0.0, []
@@ -274,12 +279,69 @@ def bench_serving(rate, seconds, seed=5, budget=200e9):
         lat.append(fin - a)
     span = fin
     plain = dict(solves_per_hr=len(arr) * 3600 / span, cell_updates_per_s=steps / span,
-                 busy_seconds=float(sum(service)), **_pcts(lat))
+                 busy_seconds=float(sum(service)), build_seconds=build_s, **_pcts(lat))
     return {"workload": "serve_poisson_%gps_%gs_dims64^3-320x240x120" % (rate, seconds),
-            "requests": len(arr), "budget_bytes": budget, "served": st, "plain_fifo": plain,
+            "requests": len(arr), "budget_bytes": budget, "max_running": max_running,
+            "served": st, "plain_fifo": plain,
             "p99_ratio_plain_over_served": plain["p99_s"] / st["p99_s"] if st.get("p99_s") else None,
             "solves_per_hr_ratio": (st["solves_per_hr"] / plain["solves_per_hr"]
                                     if st.get("solves_per_hr") else None)}
+
+
+def bench_diff(n_fd=4, rtol=1e-5):
+    """The differentiation workload at full size (360x60x120 grating, 64
+    edge parameters): gradient through the autograd function (forward +
+    adjoint solve + device assembly), the reference gradient formula on the
+    same fields (device assembly without the CPML / dispersion factors, and
+    the reference's per-parameter loop), and central differences (h = 1e-3
+    um) on n_fd edges.  After a first converged solve every solve runs a
+    fixed step count, three times the converged one, so the differences are
+    not limited by the convergence test's phase tolerance sqrt(rtol)."""
+    import torch
+    from emopt_amd.adjoint import fdtd_fom
+    from tests.cases import grating_problem
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim, am, p = grating_problem(rtol=rtol)
+        am.fom(p)
+        n0 = sim.last_solve["n0"]
+        sim.fixed_n0 = 3 * n0 - (3 * n0) % sim.Tn
+        pt = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+        t = time.perf_counter()
+        F = fdtd_fom(pt, am)
+        t_fwd = time.perf_counter() - t
+        t = time.perf_counter()
+        F.backward()
+        t_bwd = time.perf_counter() - t
+        g = pt.grad.numpy().copy()
+        pidx, comp, cell, deps = am.calc_deps(sim, p)
+        g_formula = sim.grad_assemble(pidx, comp, cell, deps, len(p))
+        t = time.perf_counter()
+        g_ref = am.calc_gradient(sim, p)
+        t_ref = time.perf_counter() - t
+        log("[bench] diff: F %.4e, forward %.2fs, backward %.2fs, reference loop %.2fs"
+            % (float(F.detach()), t_fwd, t_bwd, t_ref))
+        idx = list(np.linspace(0, len(p) - 1, n_fd).astype(int))
+        fd = []
+        for i in idx:
+            v = []
+            for sgn in (1, -1):
+                q = p.copy()
+                q[i] += sgn * 1e-3
+                v.append(am.fom(q))
+            fd.append((v[0] - v[1]) / 2e-3)
+        am.fom(p)
+    fd = np.array(fd)
+    F = float(F.detach())
+    return {"workload": "grating_360x60x120_dx0.03_64edges_gauss_w0_10.4", "F": float(F),
+            "params": len(p), "converged_n0": n0, "fixed_steps": int(sim.fixed_n0),
+            "forward_s": t_fwd, "backward_s": t_bwd, "reference_loop_s": t_ref,
+            "formula_device_vs_reference_loop": float(np.linalg.norm(g_formula - g_ref)
+                                                      / np.linalg.norm(g_ref)),
+            "exact_vs_reference_formula": float(np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)),
+            "fd_formula": g_ref[idx].tolist(),
+            "fd_edges": [int(i) for i in idx], "fd": fd.tolist(), "adjoint": g[idx].tolist(),
+            "fd_rel_err": (np.abs(g[idx] - fd) / np.abs(fd)).tolist()}
 
 
 # --------------------------------------------------------- profile summaries
@@ -362,6 +424,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="arrival window of the serving section (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0)
+    ap.add_argument("--serve-running", type=int, default=2, help="solves stepping together")
+    ap.add_argument("--diff", action="store_true", help="also run the differentiation workload")
     ap.add_argument("--profile-summary", action="store_true")
     ap.add_argument("--stats")
     ap.add_argument("--fetch")
@@ -457,7 +521,10 @@ def main():
     if args.batch > 0:
         res["batched"] = bench_batched(args.batch)
     if args.serve_seconds > 0:
-        res["serving"] = bench_serving(args.serve_rate, args.serve_seconds)
+        res["serving"] = bench_serving(args.serve_rate, args.serve_seconds,
+                                       max_running=args.serve_running)
+    if args.diff:
+        res["differentiation"] = bench_diff()
 
     if not args.no_cpu:
         del sim

@@ -7,7 +7,9 @@ method ports unchanged):
 * :func:`interpolated_dFdx_3D`   emopt/fomutils.py:1282-1429
 * :func:`power_norm_dFdx_3D`     emopt/fomutils.py:1654-1920
 * :class:`AdjointMethod`         emopt/adjoint_method.py:97-547 (the FDTD
-  subset: fom, dFdx, adjoint solve, the per-parameter gradient loop)
+  subset: fom, dFdx, adjoint solve, the per-parameter gradient loop; the
+  device assembly adds the CPML-stretch and discrete-frequency factors that
+  make the gradient exact for the time-stepped system, see gradient_device)
 * :class:`AdjointMethodPNF3D`    emopt/adjoint_method.py:1329-1486
 
 Conventions (those of the reference): dFdx are the derivatives of F with
@@ -265,6 +267,8 @@ class AdjointMethod:
     reference; optionally calc_deps (sparse dEps/dp) for the device gradient
     assembly used by :class:`FDTDAdjointFunction`."""
 
+    exact_adjoint = True
+
     def __init__(self, sim, step=1e-8):
         self.sim = sim
         self._step = step
@@ -333,13 +337,25 @@ class AdjointMethod:
         self.solve_adjoint_for(params)
         return self.calc_gradient(self.sim, params) + self.calc_grad_p(self.sim, params)
 
-    def gradient_device(self, params, fom_done=False):
+    def gradient_device(self, params, fom_done=False, exact=None):
         """Forward (unless fom_done) and adjoint solves, then the device
-        gradient assembly from calc_deps."""
+        gradient assembly from calc_deps.
+
+        exact (default: self.exact_adjoint) applies the two factors by which
+        the time-stepped system's adjoint differs from the reference formula
+        -2 Re(E_adj i dEps E): the CPML stretch product s_x s_y s_z at each
+        perturbed cell (the CPML operator is complex symmetric only after
+        that scaling, so perturbations inside the slabs need it) and the
+        discrete frequency 2 sin(dt/2)/dt of the leapfrog (the CW steady
+        state of the Yee scheme solves Maxwell's equations at that
+        frequency).  With exact=False the result is the reference formula."""
         if not fom_done:
             self.fom(params)
         self.solve_adjoint_for(params)
         pidx, comp, cell, deps = self.calc_deps(self.sim, params)
+        if self.exact_adjoint if exact is None else exact:
+            dt = self.sim.dt
+            deps = deps * self.sim.pml_stretch(comp, cell) * (2.0 * np.sin(0.5 * dt) / dt)
         g = self.sim.grad_assemble(pidx, comp, cell, deps, len(params))
         return g + self.calc_grad_p(self.sim, params)
 

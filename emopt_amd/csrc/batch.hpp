@@ -7,9 +7,10 @@
 // replayed with no host work per step; the host only reads the solves' states
 // between replays, retires finished solves and admits new ones.
 //
-// Per step:  k_batch_h   H half step of every tile (or the snapshot / phasor
-//                        pass of solves at those points of their loop)
+// Per step:  k_batch_h   H half step of every tile (storing the t0 / t1
+//                        snapshot for solves at those points of their loop)
 //            k_batch_e   E half step of every tile
+//            k_batch_phasor  phasor pass of the solves that reached it
 //            k_batch_tick  per solve: step count, convergence check every Tn
 //                        steps, choice of what the next step does
 //
@@ -114,28 +115,58 @@ __device__ __forceinline__ void phasor_column(const BMember &B, const BState &S,
     }
 }
 
+// Each work-group takes one contiguous chunk of the tile range, so the member
+// of a tile changes a handful of times per work-group: its descriptor and
+// state are looked up when it changes, not per tile, and the tiles of an idle
+// member are skipped in one jump.
+struct BCursor {
+    int e, next, m, mode;
+    long long step;
+};
+
+__device__ __forceinline__ void batch_seek(const BTable *__restrict__ T,
+                                           const BState *__restrict__ S, int t, BCursor &c)
+{
+    c.e = batch_entry(T, t);
+    c.next = T->off[c.e + 1];
+    c.m = T->mem[c.e];
+    c.mode = S[c.m].mode;
+    c.step = S[c.m].step;
+}
+
+__device__ __forceinline__ void batch_advance(const BTable *__restrict__ T,
+                                              const BState *__restrict__ S, int t, BCursor &c)
+{
+    while (t >= c.next) {
+        ++c.e;
+        c.next = T->off[c.e + 1];
+        c.m = T->mem[c.e];
+        c.mode = S[c.m].mode;
+        c.step = S[c.m].step;
+    }
+}
+
 __global__ __launch_bounds__(PX * BTY) void k_batch_h(const BTable *__restrict__ T,
                                                       const BMember *__restrict__ M,
                                                       const BState *__restrict__ S)
 {
     const int total = T->total;
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int e = batch_entry(T, t);
-        const int m = T->mem[e];
-        const int mode = S[m].mode;
-        if (mode == BM_IDLE) continue;
-        const BMember &B = M[m];
+    const int chunk = (total + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t0 = (int)blockIdx.x * chunk, t1 = min(t0 + chunk, total);
+    if (t0 >= t1) return;
+    BCursor c;
+    batch_seek(T, S, t0, c);
+    for (int t = t0; t < t1; ++t) {
+        batch_advance(T, S, t, c);
+        if (c.mode == BM_IDLE || c.mode == BM_PHASOR) { t = c.next - 1; continue; }
+        const BMember &B = M[c.m];
         int k, j, z0, z1;
-        if (!batch_column(B, t - T->off[e], k, j, z0, z1)) continue;
-        if (mode == BM_PHASOR) {
-            phasor_column(B, S[m], k, j, z0, z1);
-            continue;
-        }
-        const double *f = B.fac + S[m].step * NFAC;
+        if (!batch_column(B, t - T->off[c.e], k, j, z0, z1)) continue;
+        const double *f = B.fac + c.step * NFAC;
         PassSrc ps;
         ps.n = B.nsrc; ps.comp0 = 3; ps.blk = B.blk;
         ps.st = f[0]; ps.ct = f[1]; ps.env = f[2];
-        double *cap = mode == BM_CAP0 ? B.res : (mode == BM_CAP1 ? B.res + 1 : nullptr);
+        double *cap = c.mode == BM_CAP0 ? B.res : (c.mode == BM_CAP1 ? B.res + 1 : nullptr);
         h_column<false>(B.A, B.g, ps, k, j, z0, z1, cap, B.n);
     }
 }
@@ -145,19 +176,45 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_e(const BTable *__restrict__
                                                       const BState *__restrict__ S)
 {
     const int total = T->total;
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int e = batch_entry(T, t);
-        const int m = T->mem[e];
-        const int mode = S[m].mode;
-        if (mode == BM_IDLE || mode == BM_PHASOR) continue;
-        const BMember &B = M[m];
+    const int chunk = (total + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t0 = (int)blockIdx.x * chunk, t1 = min(t0 + chunk, total);
+    if (t0 >= t1) return;
+    BCursor c;
+    batch_seek(T, S, t0, c);
+    for (int t = t0; t < t1; ++t) {
+        batch_advance(T, S, t, c);
+        if (c.mode == BM_IDLE || c.mode == BM_PHASOR) { t = c.next - 1; continue; }
+        const BMember &B = M[c.m];
         int k, j, z0, z1;
-        if (!batch_column(B, t - T->off[e], k, j, z0, z1)) continue;
-        const double *f = B.fac + S[m].step * NFAC;
+        if (!batch_column(B, t - T->off[c.e], k, j, z0, z1)) continue;
+        const double *f = B.fac + c.step * NFAC;
         PassSrc ps;
         ps.n = B.nsrc; ps.comp0 = 0; ps.blk = B.blk;
         ps.st = f[3]; ps.ct = f[4]; ps.env = f[5];
         e_column<false>(B.A, B.g, ps, k, j, z0, z1);
+    }
+}
+
+// The phasor pass of the solves that reached it (after the E pass; the H and
+// E kernels skip them).  A kernel of its own so that its transcendental math
+// does not set the register budget, and so the occupancy, of the H pass.
+__global__ __launch_bounds__(PX * BTY) void k_batch_phasor(const BTable *__restrict__ T,
+                                                           const BMember *__restrict__ M,
+                                                           const BState *__restrict__ S)
+{
+    const int total = T->total;
+    const int chunk = (total + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t0 = (int)blockIdx.x * chunk, t1 = min(t0 + chunk, total);
+    if (t0 >= t1) return;
+    BCursor c;
+    batch_seek(T, S, t0, c);
+    for (int t = t0; t < t1; ++t) {
+        batch_advance(T, S, t, c);
+        if (c.mode != BM_PHASOR) { t = c.next - 1; continue; }
+        const BMember &B = M[c.m];
+        int k, j, z0, z1;
+        if (!batch_column(B, t - T->off[c.e], k, j, z0, z1)) continue;
+        phasor_column(B, S[c.m], k, j, z0, z1);
     }
 }
 

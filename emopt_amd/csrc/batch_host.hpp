@@ -74,6 +74,8 @@ int batch_graph(emf_batch *b)
     for (int q = 0; q < b->spg; ++q) {
         hipLaunchKernelGGL(k_batch_h, dim3(blocks), dim3(PX, BTY), 0, b->stream, b->T, b->M, b->S);
         hipLaunchKernelGGL(k_batch_e, dim3(blocks), dim3(PX, BTY), 0, b->stream, b->T, b->M, b->S);
+        hipLaunchKernelGGL(k_batch_phasor, dim3(blocks), dim3(PX, BTY), 0, b->stream, b->T, b->M,
+                           b->S);
         hipLaunchKernelGGL(k_batch_tick, dim3(BATCH_MAX), dim3(64), 0, b->stream, b->T, b->M,
                            b->S);
     }

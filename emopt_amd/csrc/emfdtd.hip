@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -168,6 +169,7 @@ struct emf_solver {
     bool coef_dirty = true;
 
     double *tab = nullptr;         // CPML tables: per axis kH bH cH kE bE cE
+    std::vector<double> tab_host;  // host copy of the same tables
     bool tab_dirty = true;
     double *psi[2][6][2] = {};     // [pass H/E][face][component]; psi[0] = psiH[cur]
     double *psiH[2][6][2] = {};    // the two H-pass sets
@@ -386,6 +388,7 @@ int refresh_tables(emf_solver *s, PassTab *th, PassTab *te)
         HIPCHK(hipMemcpyAsync(s->tab, h.data(), total * sizeof(double), hipMemcpyHostToDevice,
                               s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));   // h is a temporary
+        s->tab_host = h;
         s->tab_dirty = false;
     }
     for (int a = 0; a < 3; ++a) {
@@ -1446,6 +1449,44 @@ int emf_grad_assemble(emf_solver *s, int64_t n, const int32_t *pidx, const int32
     } while (0);
     dfree(dseg); dfree(dcell); dfree(dcomp); dfree(ddeps); dfree(dgrad);
     return rc;
+}
+
+int emf_pml_stretch(emf_solver *s, int64_t n, const int32_t *comp, const int64_t *cell,
+                    double *out)
+{
+    LOCK(s);
+    if (n < 0 || (n > 0 && (!comp || !cell || !out))) EMF_FAIL(EMF_ERR_ARG, "null entries");
+    CHECK(check_ready(s));
+    PassTab th, te;
+    CHECK(refresh_tables(s, &th, &te));
+    size_t off[3], len[3], total = 0;
+    for (int a = 0; a < 3; ++a) {
+        len[a] = std::max(1, s->w[2 * a] + s->w[2 * a + 1]);
+        off[a] = total;
+        total += 6 * len[a];
+    }
+    const Geo g = s->geo();
+    const std::complex<double> ph = std::exp(std::complex<double>(0.0, s->dt));
+    // 1/s = 1/kappa + c / (1 - b e^{i dt}): the CPML recursion psi = c D + b psi_prev
+    // at steady state (phasors in the e^{-i w t} sense, one step = e^{i dt})
+    auto stretch = [&](int a, bool htab, int par) {
+        const double *t = &s->tab_host[off[a] + (htab ? 0 : 3 * len[a])];
+        const double kap = t[par], b = t[len[a] + par], c = t[2 * len[a] + par];
+        return 1.0 / (1.0 / kap + c / (1.0 - b * ph));
+    };
+    for (int64_t e = 0; e < n; ++e) {
+        if (comp[e] < 0 || comp[e] > 2) EMF_FAIL(EMF_ERR_ARG, "component must be 0..2");
+        if (cell[e] < 0 || cell[e] >= s->n) EMF_FAIL(EMF_ERR_ARG, "cell index out of range");
+        const long long q = cell[e];
+        const int k = (int)(q % s->Nx), j = (int)((q / s->Nx) % s->Ny), i = (int)(q / g.nxy);
+        const Slab sl[3] = { slab_x(g, i, j, k), slab_y(g, i, j, k), slab_z(g, i, j, k) };
+        std::complex<double> P = 1.0;
+        for (int a = 0; a < 3; ++a)
+            if (sl[a].in) P *= stretch(a, a == comp[e], sl[a].par);
+        out[2 * e] = P.real();
+        out[2 * e + 1] = P.imag();
+    }
+    return EMF_OK;
 }
 
 int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check_every,

@@ -97,7 +97,7 @@ struct Slab {
     int par;       // table index
 };
 
-__device__ __forceinline__ Slab slab_x(const Geo &g, int i, int j, int k)
+__host__ __device__ __forceinline__ Slab slab_x(const Geo &g, int i, int j, int k)
 {
     Slab s;
     const int xlo = g.w[0], xhi = g.Nx - g.w[1];
@@ -110,7 +110,7 @@ __device__ __forceinline__ Slab slab_x(const Geo &g, int i, int j, int k)
     return s;
 }
 
-__device__ __forceinline__ Slab slab_y(const Geo &g, int i, int j, int k)
+__host__ __device__ __forceinline__ Slab slab_y(const Geo &g, int i, int j, int k)
 {
     Slab s;
     const int ylo = g.w[2], yhi = g.Ny - g.w[3];
@@ -123,7 +123,7 @@ __device__ __forceinline__ Slab slab_y(const Geo &g, int i, int j, int k)
     return s;
 }
 
-__device__ __forceinline__ Slab slab_z(const Geo &g, int i, int j, int k)
+__host__ __device__ __forceinline__ Slab slab_z(const Geo &g, int i, int j, int k)
 {
     Slab s;
     const int zlo = g.w[4], zhi = g.Nz - g.w[5];

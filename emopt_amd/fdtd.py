@@ -475,6 +475,16 @@ class FDTD(MaxwellSolver):
         self._lib.emf_calc_ydAx(self._h, *[ptr(a) for a in e0], k1, k2, j1, j2, i1, i2, ptr(out))
         return complex(out[0], out[1])
 
+    def pml_stretch(self, comp, cell):
+        """CPML stretch product s_x s_y s_z at the positions of E component
+        `comp` of the given cells (1 outside the CPML slabs); see
+        emf_pml_stretch."""
+        comp = np.ascontiguousarray(comp, dtype=np.int32)
+        cell = np.ascontiguousarray(cell, dtype=np.int64)
+        out = np.zeros(2 * len(comp))
+        self._lib.emf_pml_stretch(self._h, len(comp), ptr(comp), ptr(cell), ptr(out))
+        return out[0::2] + 1j * out[1::2]
+
     def grad_assemble(self, pidx, comp, cell, deps, nparam, step=1.0):
         """grad[p] = -2 Re(sum E_adj * 1j * deps * E) / step over sparse
         (parameter, component, cell, dEps) entries, on the device

@@ -9,15 +9,19 @@ batch at once, release their budget and hand their result to a callback.  The
 server records per-request latency (arrival to result) and throughput.
 
 Admission is first-come first-served: the queue head waits until it fits
-(no overtaking, so no request starves).  Results do not depend on admission
-order or on the other requests in the batch: each member's arithmetic is its
-own.
+(no overtaking, so no request starves).  At most `max_running` solves step
+together; up to `prep_ahead` more are built ahead on the workers, so a
+finished solve is replaced at the next replay boundary.  A small
+`max_running` keeps latency close to first-come first-served order (one
+device's bandwidth is shared by fewer solves); a large one shares it among
+all admitted solves.  Results do not depend on admission order or on the
+other requests in the batch: each member's arithmetic is its own.
 """
 from __future__ import annotations
 
 import time
 from collections import deque
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
 
 import numpy as np
 
@@ -63,7 +67,8 @@ class Request:
 
 class Server:
     def __init__(self, device=0, hbm_budget=200e9, prep_workers=4, steps_per_graph=32, zc=2,
-                 blocks_per_cu=8, graphs_per_poll=4, on_done=None, batch=None):
+                 blocks_per_cu=8, graphs_per_poll=4, on_done=None, batch=None, max_running=4,
+                 prep_ahead=None):
         self.batch = batch if batch is not None else Batch(device, steps_per_graph, zc,
                                                            blocks_per_cu)
         self.budget = float(hbm_budget)
@@ -75,6 +80,8 @@ class Server:
         self.pool = ThreadPoolExecutor(max_workers=prep_workers)
         self.graphs_per_poll = graphs_per_poll
         self.on_done = on_done
+        self.max_running = min(int(max_running), BATCH_MAX)
+        self.prep_ahead = prep_workers if prep_ahead is None else int(prep_ahead)
 
     # ------------------------------------------------------------ requests
     def submit(self, req, now=None):
@@ -90,7 +97,7 @@ class Server:
         while self.queue:
             head = self.queue[0]
             busy = len(self.running) + len(self.preparing)
-            if busy >= BATCH_MAX or self.used + head.nbytes > self.budget:
+            if busy >= self.max_running + self.prep_ahead or self.used + head.nbytes > self.budget:
                 break
             self.queue.popleft()
             self.used += head.nbytes
@@ -99,8 +106,8 @@ class Server:
 
     def _start_prepared(self):
         keep = []
-        for req, fut in self.preparing:
-            if not fut.done():
+        for req, fut in self.preparing:      # in admission order
+            if not fut.done() or len(self.running) >= self.max_running:
                 keep.append((req, fut))
                 continue
             try:
@@ -135,8 +142,8 @@ class Server:
         self._admit()
         self._start_prepared()
         if not self.running:
-            if self.preparing:
-                time.sleep(0.001)
+            if self.preparing:                     # wait for a build to finish
+                wait([f for _, f in self.preparing], return_when=FIRST_COMPLETED)
             return 0
         ids = self.batch.run(until="any", max_graphs=self.graphs_per_poll)
         self._collect(ids)

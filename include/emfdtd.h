@@ -317,6 +317,18 @@ EMF_API int emf_grad_assemble(emf_solver *s, int64_t n, const int32_t *pidx,
                       const int32_t *comp, const int64_t *cell, const double *deps,
                       int nparam, double step, double *grad);
 
+/* CPML stretch product s_x*s_y*s_z at the position of field component comp
+ * (0..2 = Ex, Ey, Ez) of each cell, as complex (re, im) pairs in out[2n].
+ * No counterpart in the reference.  In the frequency domain the CPML
+ * recursion of the step kernels acts as a coordinate stretch
+ * 1/s = 1/kappa + c/(1 - b e^{i dt}); the CPML operator is complex symmetric
+ * only after scaling by these factors, so the adjoint gradient term of an eps
+ * perturbation inside a CPML slab is -2 Re(s_x s_y s_z E_adj i d_eps E).
+ * Outside the slabs the product is 1.  (The reference's gradient omits this
+ * factor; adjoint sources are assumed to lie outside the slabs.) */
+EMF_API int emf_pml_stretch(emf_solver *s, int64_t n, const int32_t *comp, const int64_t *cell,
+                            double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,3 +70,17 @@ def test_admission_budget_and_order():
     assert st["completed"] == 40 and st["failed"] == 1
     assert max(used_trace) <= budget and srv.used == 0
     assert st["p99_s"] >= st["p50_s"] >= 0
+
+
+def test_max_running_bounds_concurrency():
+    rng = np.random.default_rng(1)
+    life = {q: int(rng.integers(1, 4)) for q in range(20)}
+    fb = FakeBatch(life)
+    srv = Server(hbm_budget=1e9, prep_workers=3, batch=fb, max_running=2)
+    for q in range(20):
+        srv.submit(Request(q, (lambda q=q: q), 10, cells=1))
+    srv.drain()
+    srv.close()
+    assert fb.max_members <= 2
+    assert fb.order == list(range(20))
+    assert srv.stats()["completed"] == 20
