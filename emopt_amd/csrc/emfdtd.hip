@@ -186,7 +186,7 @@ struct emf_solver {
     size_t src_desc_cap = 0;
     bool src_dirty = true;
 
-    int kind = 1;                  // 0 = fused step kernel, 1 = separate H / E passes
+    int kind = 1;                  // 0, 2 = fused step kernels; 1, 3 = separate H / E passes
     int tyg = 4;                   // fused: waves (rows) per work-group
 
     ConvState *conv = nullptr;
@@ -554,30 +554,47 @@ dim3 pass_grid(const emf_solver *s)
     return dim3((s->Nx + PX - 1) / PX, (s->Ny + ty - 1) / ty, (s->Nz + s->zc - 1) / s->zc);
 }
 
+template <bool MU, int OCC>
+void launch_h_occ(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
+                  const int *done)
+{
+    const dim3 gr = pass_grid(s);
+    switch (pass_ty(s)) {
+    case 1: hipLaunchKernelGGL((k_pass_h<MU, 1, OCC>), gr, dim3(PX, 1), 0, s->stream, A, g, ps, done); break;
+    case 2: hipLaunchKernelGGL((k_pass_h<MU, 2, OCC>), gr, dim3(PX, 2), 0, s->stream, A, g, ps, done); break;
+    case 8: hipLaunchKernelGGL((k_pass_h<MU, 8, OCC>), gr, dim3(PX, 8), 0, s->stream, A, g, ps, done); break;
+    default: hipLaunchKernelGGL((k_pass_h<MU, 4, OCC>), gr, dim3(PX, 4), 0, s->stream, A, g, ps, done);
+    }
+}
+
+template <bool CE, int OCC>
+void launch_e_occ(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
+                  const int *done)
+{
+    const dim3 gr = pass_grid(s);
+    switch (pass_ty(s)) {
+    case 1: hipLaunchKernelGGL((k_pass_e<CE, 1, OCC>), gr, dim3(PX, 1), 0, s->stream, A, g, ps, done); break;
+    case 2: hipLaunchKernelGGL((k_pass_e<CE, 2, OCC>), gr, dim3(PX, 2), 0, s->stream, A, g, ps, done); break;
+    case 8: hipLaunchKernelGGL((k_pass_e<CE, 8, OCC>), gr, dim3(PX, 8), 0, s->stream, A, g, ps, done); break;
+    default: hipLaunchKernelGGL((k_pass_e<CE, 4, OCC>), gr, dim3(PX, 4), 0, s->stream, A, g, ps, done);
+    }
+}
+
+// kind 3 = the two passes compiled for 5 waves per SIMD
 template <bool MU>
 void launch_h_t(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
                 const int *done)
 {
-    const dim3 gr = pass_grid(s);
-    switch (pass_ty(s)) {
-    case 1: hipLaunchKernelGGL((k_pass_h<MU, 1>), gr, dim3(PX, 1), 0, s->stream, A, g, ps, done); break;
-    case 2: hipLaunchKernelGGL((k_pass_h<MU, 2>), gr, dim3(PX, 2), 0, s->stream, A, g, ps, done); break;
-    case 8: hipLaunchKernelGGL((k_pass_h<MU, 8>), gr, dim3(PX, 8), 0, s->stream, A, g, ps, done); break;
-    default: hipLaunchKernelGGL((k_pass_h<MU, 4>), gr, dim3(PX, 4), 0, s->stream, A, g, ps, done);
-    }
+    if (s->kind == 3) launch_h_occ<MU, 5>(s, A, g, ps, done);
+    else launch_h_occ<MU, 4>(s, A, g, ps, done);
 }
 
 template <bool CE>
 void launch_e_t(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
                 const int *done)
 {
-    const dim3 gr = pass_grid(s);
-    switch (pass_ty(s)) {
-    case 1: hipLaunchKernelGGL((k_pass_e<CE, 1>), gr, dim3(PX, 1), 0, s->stream, A, g, ps, done); break;
-    case 2: hipLaunchKernelGGL((k_pass_e<CE, 2>), gr, dim3(PX, 2), 0, s->stream, A, g, ps, done); break;
-    case 8: hipLaunchKernelGGL((k_pass_e<CE, 8>), gr, dim3(PX, 8), 0, s->stream, A, g, ps, done); break;
-    default: hipLaunchKernelGGL((k_pass_e<CE, 4>), gr, dim3(PX, 4), 0, s->stream, A, g, ps, done);
-    }
+    if (s->kind == 3) launch_e_occ<CE, 5>(s, A, g, ps, done);
+    else launch_e_occ<CE, 4>(s, A, g, ps, done);
 }
 
 // H pass in place on the current set (update_H, src/fdtd.cpp:121-367)
@@ -691,7 +708,7 @@ int step(emf_solver *s, long long nidx, const int *done, int from)
 // and psi set below 4 GiB (the kernel addresses with 32-bit byte offsets).
 inline bool fused(const emf_solver *s)
 {
-    return s->kind != 1 && 3ULL * (unsigned long long)s->n * 8ULL < (1ULL << 32) &&
+    return (s->kind == 0 || s->kind == 2) && 3ULL * (unsigned long long)s->n * 8ULL < (1ULL << 32) &&
            (unsigned long long)s->psi_tot * 8ULL < (1ULL << 32);
 }
 
@@ -1551,8 +1568,9 @@ int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check
 int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc, int order)
 {
     LOCK(s);
-    if (kind < 0 || kind > 2)
-        EMF_FAIL(EMF_ERR_ARG, "kind must be 0 (fused, row recompute), 1 (two passes) or 2 (fused, LDS)");
+    if (kind < 0 || kind > 3)
+        EMF_FAIL(EMF_ERR_ARG, "kind must be 0 (fused, row recompute), 1 (two passes), 2 (fused, LDS) "
+                              "or 3 (two passes, 5 waves per SIMD)");
     const bool ok = kind == 2 ? (tyg == 4 || tyg == 8 || tyg == 16)
                               : (tyg == 1 || tyg == 2 || tyg == 4 || tyg == 8);
     if (!ok) EMF_FAIL(EMF_ERR_ARG, "tyg must be 1, 2, 4 or 8 (kinds 0, 1) or 4, 8, 16 (kind 2)");

@@ -92,9 +92,12 @@ __device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const
     }
 }
 
-template <bool MU_ARRAY, int TY>
-__global__ __launch_bounds__(PX * TY) void k_pass_h(FusedArgs A, Geo g, PassSrc sh,
-                                                   const int *__restrict__ done)
+// OCC = minimum waves per SIMD the register allocation must allow (4 fits
+// the passes without spilling; 5 trades a few spilled registers for 25 %
+// more waves in flight).
+template <bool MU_ARRAY, int TY, int OCC = 4>
+__global__ __launch_bounds__(PX * TY, OCC) void k_pass_h(FusedArgs A, Geo g, PassSrc sh,
+                                                        const int *__restrict__ done)
 {
     if (done && *done) return;
     int k, j, z0, z1;
@@ -207,9 +210,9 @@ __device__ __forceinline__ void e_column(const FusedArgs &A, const Geo &g, const
     }
 }
 
-template <bool CEPS, int TY>
-__global__ __launch_bounds__(PX * TY) void k_pass_e(FusedArgs A, Geo g, PassSrc se,
-                                                   const int *__restrict__ done)
+template <bool CEPS, int TY, int OCC = 4>
+__global__ __launch_bounds__(PX * TY, OCC) void k_pass_e(FusedArgs A, Geo g, PassSrc se,
+                                                        const int *__restrict__ done)
 {
     if (done && *done) return;
     int k, j, z0, z1;

@@ -235,8 +235,9 @@ EMF_API int emf_get_field(emf_solver *s, int slot, int comp, int k1, int k2, int
 EMF_API int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check_every,
                            int per_kernel, double *out);
 
-/* Step kernel selection: kind 1 = separate H and E passes (default), 0 = one
- * fused H+E launch per step recomputing the H row it needs, 2 = one fused
+/* Step kernel selection: kind 1 = separate H and E passes (default), 3 = the
+ * same passes compiled for 5 waves per SIMD (a few registers spilled), 0 =
+ * one fused H+E launch per step recomputing the H row it needs, 2 = one fused
  * launch sharing H through LDS; tyg = rows (wavefronts) per work-group (1, 2,
  * 4 or 8; 4, 8 or 16 for kind 2); zc = z planes per work-group; order =
  * work-group order of the two passes (0 x-fastest, 1 z-fastest, 2 z-fastest

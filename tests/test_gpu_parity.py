@@ -186,12 +186,15 @@ def test_step_kernels_agree(variant):
                 got = _run_steps(case, 33, kind=kind, tyg=tyg, zc=zc)
                 errs = rel_l2_set(got, ref)
                 assert max(errs) < 1e-13, (variant, kind, tyg, zc, errs)
-    for tyg in (1, 2, 8):
-        for zc in (1, 5):
-            for order in (0, 1, 2):
-                got = _run_steps(case, 33, kind=1, tyg=tyg, zc=zc, order=order)
-                errs = rel_l2_set(got, ref)
-                assert max(errs) < 1e-13, (variant, tyg, zc, order, errs)
+    for kind in (1, 3):
+        for tyg in (1, 2, 4, 8):
+            for zc in (1, 5):
+                for order in (0, 1, 2):
+                    if kind == 1 and tyg == 4 and zc == 8 and order == 0:
+                        continue
+                    got = _run_steps(case, 33, kind=kind, tyg=tyg, zc=zc, order=order)
+                    errs = rel_l2_set(got, ref)
+                    assert max(errs) < 1e-13, (variant, kind, tyg, zc, order, errs)
 
 
 @pytest.mark.parametrize("tuning", [(0, 2, 8, 0), (2, 8, 4, 0), (1, 8, 8, 2)])
