@@ -54,6 +54,7 @@ _SIGS = {
     "emf_set_mat_arrays": ([_P] + [_P] * 6, c_int),
     "emf_update_eps_box": ([_P, c_int, _P] + [c_int] * 6, c_int),
     "emf_get_mat_arrays": ([_P] + [_P] * 6, c_int),
+    "emf_set_compression": ([_P, c_int], c_int),
     "emf_set_bc": ([_P, c_char_p], c_int),
     "emf_set_pml_widths": ([_P] + [c_int] * 6, c_int),
     "emf_set_pml_properties": ([_P] + [c_double] * 4, c_int),

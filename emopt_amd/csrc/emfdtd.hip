@@ -13,10 +13,12 @@
 #include <cmath>
 #include <complex>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/emfdtd.h"
@@ -187,7 +189,7 @@ struct emf_solver {
     bool src_dirty = true;
 
     int kind = 1;                  // 0, 2 = fused step kernels; 1, 3 = separate H / E passes
-    int tyg = 4;                   // fused: waves (rows) per work-group
+    int tyg = 2;                   // waves (rows) per work-group
 
     ConvState *conv = nullptr;
     long long *samples = nullptr;
@@ -201,6 +203,17 @@ struct emf_solver {
     size_t scratch_n = 0;
 
     void *in_batch = nullptr;      // the emf_batch this handle is a member of
+
+    // Coefficient compression (real eps): a per-cell index into a table of
+    // the distinct (eps_x, eps_y, eps_z).re triples; the E pass then reads
+    // 2 or 4 bytes per cell instead of three 8-byte coefficients.
+    int compress = 1;              // allowed (emf_set_compression)
+    bool cidx_valid = false;       // index built from the current materials
+    int cidx_bytes = 0;            // width of the index in use (0: off)
+    void *cidx = nullptr;          // device index, n entries
+    double *ctabE = nullptr;       // device table of eps triples
+    double *ctabB = nullptr;       // device table of b = dt/eps triples
+    long long ctab_n = 0;
 
     Geo geo() const
     {
@@ -286,6 +299,10 @@ void free_fields(emf_solver *s)
         dfree(s->res[sl][0]);   // one block per slot; [1..5] point into it
         for (int c = 1; c < 6; ++c) s->res[sl][c] = nullptr;
     }
+    dfree(s->cidx); dfree(s->ctabE); dfree(s->ctabB);
+    s->cidx_valid = false;
+    s->cidx_bytes = 0;
+    s->ctab_n = 0;
     s->mat_set = false;
     s->coef_dirty = true;
     s->cur = 0;
@@ -340,8 +357,84 @@ int refresh_coeffs(emf_solver *s)
                                s->mu[c], s->mH[c], s->muR[c], s->n, s->dt);
         }
     }
+    s->cidx_bytes = 0;
+    if (s->compress && s->cidx_valid && !s->ceps && s->ctab_n > 0) {
+        hipLaunchKernelGGL(k_b_table, dim3(grid1d(3 * s->ctab_n, 256)), dim3(256), 0, s->stream,
+                           s->ctabE, s->ctabB, 3 * s->ctab_n, s->dt);
+        s->cidx_bytes = s->ctab_n <= 65535 ? 2 : 4;
+    }
     HIPCHK(hipGetLastError());
     s->coef_dirty = false;
+    return EMF_OK;
+}
+
+// Index of distinct real eps triples, built on the host from the arrays
+// being uploaded.  Runs of equal cells along x are the common case, so the
+// previous cell's triple is checked before the hash map.
+struct EpsKey {
+    uint64_t a, b, c;
+    bool operator==(const EpsKey &o) const { return a == o.a && b == o.b && c == o.c; }
+};
+struct EpsKeyHash {
+    size_t operator()(const EpsKey &k) const
+    {
+        uint64_t h = k.a * 0x9E3779B97F4A7C15ULL;
+        h ^= k.b + 0x9E3779B97F4A7C15ULL + (h << 6) + (h >> 2);
+        h ^= k.c + 0x9E3779B97F4A7C15ULL + (h << 6) + (h >> 2);
+        return (size_t)h;
+    }
+};
+
+int build_coeff_index(emf_solver *s, const double *ex, const double *ey, const double *ez)
+{
+    dfree(s->cidx); dfree(s->ctabE); dfree(s->ctabB);
+    s->cidx_valid = false;
+    s->ctab_n = 0;
+    const long long n = s->n, limit = 1LL << 20;
+    std::unordered_map<EpsKey, uint32_t, EpsKeyHash> map;
+    std::vector<double> tab;
+    std::vector<uint32_t> idx((size_t)n);
+    EpsKey last{ ~0ULL, ~0ULL, ~0ULL };
+    uint32_t last_i = 0;
+    for (long long g = 0; g < n; ++g) {
+        EpsKey k;
+        memcpy(&k.a, &ex[2 * g], 8); memcpy(&k.b, &ey[2 * g], 8); memcpy(&k.c, &ez[2 * g], 8);
+        if (!(k == last)) {
+            auto it = map.find(k);
+            if (it == map.end()) {
+                if ((long long)map.size() >= limit) return EMF_OK;   // too many: stay off
+                last_i = (uint32_t)map.size();
+                map.emplace(k, last_i);
+                tab.push_back(ex[2 * g]); tab.push_back(ey[2 * g]); tab.push_back(ez[2 * g]);
+            } else {
+                last_i = it->second;
+            }
+            last = k;
+        }
+        idx[(size_t)g] = last_i;
+    }
+    const long long nt = (long long)map.size();
+    CHECK(dalloc(&s->ctabE, 3 * (size_t)nt));
+    CHECK(dalloc(&s->ctabB, 3 * (size_t)nt));
+    HIPCHK(hipMemcpyAsync(s->ctabE, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice,
+                          s->stream));
+    if (nt <= 65535) {
+        std::vector<uint16_t> i16((size_t)n);
+        for (long long g = 0; g < n; ++g) i16[(size_t)g] = (uint16_t)idx[(size_t)g];
+        uint16_t *d = nullptr;
+        CHECK(dalloc(&d, (size_t)n));
+        s->cidx = d;
+        HIPCHK(hipMemcpyAsync(d, i16.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    } else {
+        uint32_t *d = nullptr;
+        CHECK(dalloc(&d, (size_t)n));
+        s->cidx = d;
+        HIPCHK(hipMemcpyAsync(d, idx.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    s->ctab_n = nt;
+    s->cidx_valid = true;
     return EMF_OK;
 }
 
@@ -539,6 +632,10 @@ FusedArgs make_args(const emf_solver *s, int from, int to)
     A.cs = (unsigned)(s->n * 8);
     A.m_s = s->dt / s->mu_re;
     A.mu_s = s->mu_re;
+    const bool comp = s->cidx_bytes != 0 && s->compress && s->cidx_valid && !s->ceps;
+    A.bI = comp ? s->cidx : nullptr;
+    A.bT = s->ctabB;
+    A.bIw = s->cidx_bytes;
     return A;
 }
 
@@ -952,6 +1049,7 @@ int emf_set_mat_arrays(emf_solver *s, const double *ex, const double *ey, const 
                                   s->stream));
         }
     HIPCHK(hipStreamSynchronize(s->stream));
+    if (s->compress) CHECK(build_coeff_index(s, ex, ey, ez));
     s->mat_set = true;
     s->coef_dirty = true;
     return EMF_OK;
@@ -974,7 +1072,16 @@ int emf_update_eps_box(emf_solver *s, int c, const double *vals, int k1, int k2,
                        s->scratch, g, k1, k2 - k1, j1, j2 - j1, i1, i2 - i1);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
+    s->cidx_valid = false;         // the index no longer describes eps: full coefficients
     s->coef_dirty = true;
+    return EMF_OK;
+}
+
+int emf_set_compression(emf_solver *s, int on)
+{
+    LOCK(s);
+    s->compress = on ? 1 : 0;
+    s->coef_dirty = true;          // the next refresh switches the E pass over
     return EMF_OK;
 }
 

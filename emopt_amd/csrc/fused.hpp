@@ -42,6 +42,9 @@ struct FusedArgs {
     const double *tab;       // CPML tables
     unsigned cs;             // byte stride between components
     double m_s, mu_s;        // uniform dt/mu.re and mu.re
+    const void *bI;          // real eps only, optional: per-cell index into bT
+    const double *bT;        // (b_x, b_y, b_z) per distinct eps triple
+    int bIw;                 // index width in bytes (2 or 4)
 };
 
 // Byte offsets of the psi slab arrays of a set ([face][component], faces

@@ -306,6 +306,15 @@ __global__ void k_eps_coeffs(const double2 *__restrict__ eps, double *__restrict
     }
 }
 
+// b = dt/eps for a table of distinct real eps values (the same division as
+// k_eps_coeffs, so the same doubles)
+__global__ void k_b_table(const double *__restrict__ eps, double *__restrict__ b, long long n,
+                          double dt)
+{
+    long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; g < n; g += (long long)gridDim.x * blockDim.x) b[g] = dt / eps[g];
+}
+
 // dt/mu.re and mu.re from raw complex mu
 __global__ void k_mu_coeffs(const double2 *__restrict__ mu, double *__restrict__ m,
                             double *__restrict__ mr, long long n, double dt)

@@ -2,7 +2,10 @@
 // place on one buffer set, with the same argument layout as the fused step.
 //
 // HBM traffic per cell: H pass reads E(3) H(3) and writes H(3) = 72 B; E pass
-// reads H(3) E(3) b_eps(3) and writes E(3) = 96 B; plus the psi slabs.
+// reads H(3) E(3) b_eps(3) and writes E(3) = 96 B; plus the psi slabs.  With
+// real eps the E pass can read a 2- or 4-byte index per cell into a small
+// table of distinct (b_x, b_y, b_z) instead of the three coefficients (the
+// same doubles, so the same results): 74 or 76 B per cell.
 //
 // A 64 x TY work-group owns a tile of (x, y) columns and sweeps zc planes
 // (zc = 1 is one cell per thread).  Every load a plane needs -- fields,
@@ -137,7 +140,14 @@ __device__ __forceinline__ void e_column(const FusedArgs &A, const Geo &g, const
         if (j > 0) { hz_j = ldo(A.Ho, o - oy + cs2); hx_j = ldo(A.Ho, o - oy); }
         if (k > 0) { hz_k = ldo(A.Ho, o - 8u + cs2); hy_k = ldo(A.Ho, o - 8u + cs); }
         const double ex = ldo(A.Eo, o), ey = ldo(A.Eo, o + cs), ez = ldo(A.Eo, o + cs2);
-        const double bx = ldo(A.bE, o), by = ldo(A.bE, o + cs), bz = ldo(A.bE, o + cs2);
+        double bx, by, bz;
+        if (!CEPS && A.bI) {       // compressed coefficients: index, then table
+            const unsigned t = A.bIw == 2 ? (unsigned)((const unsigned short *)A.bI)[o >> 3]
+                                          : ((const unsigned *)A.bI)[o >> 3];
+            bx = A.bT[3 * t]; by = A.bT[3 * t + 1]; bz = A.bT[3 * t + 2];
+        } else {
+            bx = ldo(A.bE, o); by = ldo(A.bE, o + cs); bz = ldo(A.bE, o + cs2);
+        }
         double ax = 1, ay = 1, az = 1;
         if (CEPS) { ax = ldo(A.aE, o); ay = ldo(A.aE, o + cs); az = ldo(A.aE, o + cs2); }
         const Slab sx = slab_x(g, i, j, k), sy = slab_y(g, i, j, k), sz = slab_z(g, i, j, k);

@@ -34,12 +34,13 @@ BATCH_MAX = 128
 
 def state_bytes(Nx, Ny, Nz, w, slots=1):
     """Device bytes of one solver: E and H (48 B/cell), update coefficients
-    (24 B), the eps copy kept for box updates (48 B), result slots (96 B
-    each) and the CPML psi slabs (E and H sets, 2 components each)."""
+    (24 B) and their compressed index (up to 4 B), the eps copy kept for box
+    updates (48 B), result slots (96 B each) and the CPML psi slabs (E and H
+    sets, 2 components each)."""
     n = Nx * Ny * Nz
     slab = 2 * (Ny * Nz * w[0] + Ny * Nz * w[1] + Nx * Nz * w[2] + Nx * Nz * w[3]
                 + Nx * Ny * w[4] + Nx * Ny * w[5])
-    return n * (48 + 24 + 48 + 96 * slots) + 2 * slab * 8
+    return n * (48 + 24 + 4 + 48 + 96 * slots) + 2 * slab * 8
 
 
 class Request:

@@ -113,6 +113,15 @@ EMF_API int emf_set_mat_arrays(emf_solver *s, const double *eps_x, const double 
 EMF_API int emf_update_eps_box(emf_solver *s, int c, const double *vals, int k1, int k2,
                        int j1, int j2, int i1, int i2);
 
+/* Coefficient compression for real eps (no counterpart in the reference):
+ * on (default), emf_set_mat_arrays also builds a per-cell index into the
+ * table of distinct (eps_x, eps_y, eps_z) triples, and the E pass reads that
+ * 2- or 4-byte index instead of three 8-byte coefficients b = dt/eps (the
+ * table holds the same doubles, so results are identical).  It stays off
+ * after emf_update_eps_box until the next emf_set_mat_arrays, and when there
+ * are more than 2^20 distinct triples. */
+EMF_API int emf_set_compression(emf_solver *s, int on);
+
 /* Copies the device materials back (six complex arrays, any may be NULL).
  * Supports get_A_diag (emopt/fdtd.py:1669-1692). */
 EMF_API int emf_get_mat_arrays(emf_solver *s, double *eps_x, double *eps_y, double *eps_z,
@@ -241,7 +250,7 @@ EMF_API int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64
  * launch sharing H through LDS; tyg = rows (wavefronts) per work-group (1, 2,
  * 4 or 8; 4, 8 or 16 for kind 2); zc = z planes per work-group; order =
  * work-group order of the two passes (0 x-fastest, 1 z-fastest, 2 z-fastest
- * grouped by XCD).  Defaults kind 1, tyg 4, zc 2, order 0.  Results are
+ * grouped by XCD).  Defaults kind 1, tyg 2, zc 2, order 0.  Results are
  * identical; only speed differs. */
 EMF_API int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc, int order);
 

@@ -139,7 +139,7 @@ def test_gpu_matches_reference_golden_solve():
     assert max(errs) <= TOL, errs
 
 
-def _run_steps(case, nsteps, kind, tyg=4, zc=8, order=0):
+def _run_steps(case, nsteps, kind, tyg=4, zc=8, order=0, compress=1):
     """nsteps time steps through emf_time_steps with the given step kernel."""
     import ctypes
     import warnings
@@ -156,6 +156,7 @@ def _run_steps(case, nsteps, kind, tyg=4, zc=8, order=0):
     sim._register(sim._sources)
     L, h = sim._lib, sim._h
     L.emf_set_tuning(h, kind, tyg, zc, order)
+    L.emf_set_compression(h, compress)
     L.emf_reset_pml(h)
     L.emf_zero_fields(h)
     out = np.zeros(3)
@@ -216,3 +217,21 @@ def test_tuned_solve_matches_default_solve(tuning):
     b = [sim.get_field(c, full) for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")]
     errs = [rel_l2(x, y) for x, y in zip(a["fields"], b)]
     assert max(errs) < 1e-12, (tuning, errs)
+
+
+@pytest.mark.parametrize("which", ["strip", "random", "random_ceps"])
+def test_coefficient_compression_is_exact(which):
+    """The E pass reading a per-cell index into the table of distinct eps
+    triples gives exactly the fields of the full coefficient arrays: few
+    distinct values (strip, 2-byte index), every cell distinct (random,
+    4-byte index) and complex eps (compression not applicable)."""
+    if which == "strip":
+        case = strip_case(40, 28, 22, w=5)
+    else:
+        case = random_case(70, 37, 23, w=(5, 4, 3, 6, 4, 5), seed=31, nsrc=2, mu_rand=False,
+                           ceps=(which == "random_ceps"))
+    for kind in (1, 3):
+        a = _run_steps(case, 41, kind=kind, tyg=4, zc=2, compress=1)
+        b = _run_steps(case, 41, kind=kind, tyg=4, zc=2, compress=0)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), (which, kind)

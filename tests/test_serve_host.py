@@ -41,11 +41,11 @@ class FakeBatch:
 
 
 def test_state_bytes():
-    # 10x10x10 grid, no CPML: 216 B per cell with one result slot
-    assert state_bytes(10, 10, 10, [0] * 6) == 1000 * 216
-    assert state_bytes(10, 10, 10, [0] * 6, slots=2) == 1000 * 312
+    # 10x10x10 grid, no CPML: 220 B per cell with one result slot
+    assert state_bytes(10, 10, 10, [0] * 6) == 1000 * 220
+    assert state_bytes(10, 10, 10, [0] * 6, slots=2) == 1000 * 316
     # psi: 2 components x (E and H) x 8 B per slab cell
-    assert state_bytes(10, 10, 10, [1, 0, 0, 0, 0, 0]) == 1000 * 216 + 2 * 100 * 2 * 8
+    assert state_bytes(10, 10, 10, [1, 0, 0, 0, 0, 0]) == 1000 * 220 + 2 * 100 * 2 * 8
 
 
 def test_admission_budget_and_order():
