@@ -148,7 +148,7 @@ struct emf_solver {
     int bc[3] = { '0', '0', '0' };
     int nconv = 10;
     int zc = 2;
-    int order = 0;                 // two-pass work-group order (pass_tile)
+    int order = 3;                 // two-pass work-group order (pass_tile)
 
     // device state.  E/H (and the H-pass psi set) exist twice for the fused
     // step, which reads one set and writes the other; E, H and psi[0] always
@@ -1682,7 +1682,7 @@ int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc, int order)
                               : (tyg == 1 || tyg == 2 || tyg == 4 || tyg == 8);
     if (!ok) EMF_FAIL(EMF_ERR_ARG, "tyg must be 1, 2, 4 or 8 (kinds 0, 1) or 4, 8, 16 (kind 2)");
     if (zc < 1) EMF_FAIL(EMF_ERR_ARG, "zc must be >= 1");
-    if (order < 0 || order > 2) EMF_FAIL(EMF_ERR_ARG, "order must be 0, 1 or 2");
+    if (order < 0 || order > 3) EMF_FAIL(EMF_ERR_ARG, "order must be 0, 1, 2 or 3");
     s->kind = kind;
     s->tyg = tyg;
     s->zc = zc;

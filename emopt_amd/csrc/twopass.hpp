@@ -23,13 +23,18 @@ constexpr int PX = 64;
 // z fastest, then x, then y.  order 2: as 1, after regrouping the linear id
 // by XCD (the dispatcher deals consecutive work-groups to the 8 XCDs in turn),
 // so that z-neighbouring tiles, which share a halo plane, run on one XCD and
-// share its L2.  The mapping is a bijection for any work-group count.
-__device__ __forceinline__ void pass_tile(const Geo &g, int ty, int &k, int &j, int &z0, int &z1)
+// share its L2.  order 3: as 0, but the E pass (rev) walks the tiles
+// backwards, so that each pass starts where the previous one ended and
+// first reads what that pass wrote last, while it is still in the Infinity
+// Cache.  The mapping is a bijection for any work-group count.
+__device__ __forceinline__ void pass_tile(const Geo &g, int ty, int &k, int &j, int &z0, int &z1,
+                                          bool rev = false)
 {
     const unsigned nbx = gridDim.x, nby = gridDim.y, nbz = gridDim.z;
     unsigned l = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    if (rev && g.order == 3) l = nbx * nby * nbz - 1 - l;
     unsigned bx, by, bz;
-    if (g.order == 0) {
+    if (g.order == 0 || g.order == 3) {
         bx = l % nbx; by = (l / nbx) % nby; bz = l / (nbx * nby);
     } else {
         if (g.order == 2) {
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_e(FusedArgs A, Geo g, Pas
 {
     if (done && *done) return;
     int k, j, z0, z1;
-    pass_tile(g, TY, k, j, z0, z1);
+    pass_tile(g, TY, k, j, z0, z1, true);
     if (k >= g.Nx || j >= g.Ny) return;
     e_column<CEPS>(A, g, se, k, j, z0, z1);
 }

@@ -190,7 +190,7 @@ def test_step_kernels_agree(variant):
     for kind in (1, 3):
         for tyg in (1, 2, 4, 8):
             for zc in (1, 5):
-                for order in (0, 1, 2):
+                for order in (0, 1, 2, 3):
                     if kind == 1 and tyg == 4 and zc == 8 and order == 0:
                         continue
                     got = _run_steps(case, 33, kind=kind, tyg=tyg, zc=zc, order=order)
