@@ -146,6 +146,19 @@ __device__ __forceinline__ void batch_advance(const BTable *__restrict__ T,
     }
 }
 
+// backwards: the entry holding tile t when t only decreases
+__device__ __forceinline__ void batch_retreat(const BTable *__restrict__ T,
+                                              const BState *__restrict__ S, int t, BCursor &c)
+{
+    while (t < T->off[c.e]) {
+        --c.e;
+        c.next = T->off[c.e + 1];
+        c.m = T->mem[c.e];
+        c.mode = S[c.m].mode;
+        c.step = S[c.m].step;
+    }
+}
+
 __global__ __launch_bounds__(PX * BTY) void k_batch_h(const BTable *__restrict__ T,
                                                       const BMember *__restrict__ M,
                                                       const BState *__restrict__ S)
@@ -171,6 +184,9 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_h(const BTable *__restrict__
     }
 }
 
+// The E pass walks each chunk backwards, so it starts on the tiles the H
+// pass wrote last (still in the Infinity Cache), as the single-solve E pass
+// does with work-group order 3.
 __global__ __launch_bounds__(PX * BTY) void k_batch_e(const BTable *__restrict__ T,
                                                       const BMember *__restrict__ M,
                                                       const BState *__restrict__ S)
@@ -180,10 +196,10 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_e(const BTable *__restrict__
     const int t0 = (int)blockIdx.x * chunk, t1 = min(t0 + chunk, total);
     if (t0 >= t1) return;
     BCursor c;
-    batch_seek(T, S, t0, c);
-    for (int t = t0; t < t1; ++t) {
-        batch_advance(T, S, t, c);
-        if (c.mode == BM_IDLE || c.mode == BM_PHASOR) { t = c.next - 1; continue; }
+    batch_seek(T, S, t1 - 1, c);
+    for (int t = t1 - 1; t >= t0; --t) {
+        batch_retreat(T, S, t, c);
+        if (c.mode == BM_IDLE || c.mode == BM_PHASOR) { t = T->off[c.e]; continue; }
         const BMember &B = M[c.m];
         int k, j, z0, z1;
         if (!batch_column(B, t - T->off[c.e], k, j, z0, z1)) continue;
