@@ -11,7 +11,8 @@ in a JSON file.
 
 Batches: steps per graph replay, z planes per work-group and persistent
 work-groups per CU are timed on a sample of the batch's members (a fixed
-number of replays from the start of their solves).
+number of replays from the start of their solves), stepped as one group so
+that every replay advances every member.
 """
 from __future__ import annotations
 
@@ -99,7 +100,7 @@ def tune_batch(sims, candidates=None, graphs=3, device=0, verbose=False):
     rates = {}
     for cand in candidates or BATCH_CANDIDATES:
         spg, zc, bpc = cand
-        b = Batch(device, spg, zc, bpc)
+        b = Batch(device, spg, zc, bpc, group_bytes=0)
         ids = [b.add(s, postprocess=False) for s in sims]
         b.run(until="graphs", max_graphs=1)          # warm-up (graph capture)
         t = time.perf_counter()

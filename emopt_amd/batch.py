@@ -22,11 +22,18 @@ import ctypes
 from . import _lib
 from ._lib import SolveInfo
 
-__all__ = ["Batch"]
+__all__ = ["Batch", "GROUP_BYTES"]
+
+# Members are stepped in groups whose hot state (~80 B/cell) stays within
+# this many bytes -- about MI355X's 256 MB Infinity Cache -- so each group's
+# E pass re-reads from cache what its H pass just wrote (batch of 32 mixed
+# solves: 0.98x the one-after-the-other time, against 0.91x with one group).
+GROUP_BYTES = 260 * 10**6
 
 
 class Batch:
-    def __init__(self, device=0, steps_per_graph=64, zc=2, blocks_per_cu=8):
+    def __init__(self, device=0, steps_per_graph=64, zc=2, blocks_per_cu=8,
+                 group_bytes=GROUP_BYTES):
         self._lib = _lib.load()
         h = ctypes.c_void_p()
         self._lib.emf_batch_new(device, ctypes.byref(h))
@@ -34,6 +41,13 @@ class Batch:
         self._members = {}          # id -> (sim, slot, postprocess)
         self.finished = {}          # id -> solve info dict of finished members
         self.set_tuning(steps_per_graph, zc, blocks_per_cu)
+        self.set_grouping(group_bytes)
+
+    def set_grouping(self, group_bytes):
+        """Step members in groups whose hot state fits `group_bytes` (one
+        group per graph replay, in turns); 0 steps all members together."""
+        self._lib.emf_batch_set_grouping(self._h, int(group_bytes))
+        self.group_bytes = group_bytes
 
     def set_tuning(self, steps_per_graph, zc, blocks_per_cu):
         self._lib.emf_batch_set_tuning(self._h, int(steps_per_graph), int(zc), int(blocks_per_cu))

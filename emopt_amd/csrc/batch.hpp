@@ -69,7 +69,28 @@ struct BTable {                   // the tiles of the solves in the batch
     int total;                    // tiles over all entries
     int mem[BATCH_MAX];           // member slot of entry e
     int off[BATCH_MAX + 1];       // first tile of entry e
+    // Entries are grouped (consecutive runs) so that a group's hot state fits
+    // the Infinity Cache; a graph replay steps the group `cur` only.
+    int ngroups;
+    int cur;
+    int gent[BATCH_MAX + 1];      // first entry of group g
 };
+
+// the tile range [t_lo, t_hi) and entry range [e_lo, e_hi) of the current group
+__device__ __forceinline__ void batch_group(const BTable *__restrict__ T, int &t_lo, int &t_hi,
+                                            int &e_lo, int &e_hi)
+{
+    const int g = T->cur;
+    e_lo = T->gent[g];
+    e_hi = T->gent[g + 1];
+    t_lo = T->off[e_lo];
+    t_hi = T->off[e_hi];
+}
+
+__global__ void k_batch_set_group(BTable *T, int g)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) T->cur = g;
+}
 
 // entry e with off[e] <= t < off[e + 1] (t < total)
 __device__ __forceinline__ int batch_entry(const BTable *__restrict__ T, int t)
@@ -163,9 +184,10 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_h(const BTable *__restrict__
                                                       const BMember *__restrict__ M,
                                                       const BState *__restrict__ S)
 {
-    const int total = T->total;
-    const int chunk = (total + (int)gridDim.x - 1) / (int)gridDim.x;
-    const int t0 = (int)blockIdx.x * chunk, t1 = min(t0 + chunk, total);
+    int t_lo, t_hi, e_lo, e_hi;
+    batch_group(T, t_lo, t_hi, e_lo, e_hi);
+    const int chunk = (t_hi - t_lo + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t0 = t_lo + (int)blockIdx.x * chunk, t1 = min(t0 + chunk, t_hi);
     if (t0 >= t1) return;
     BCursor c;
     batch_seek(T, S, t0, c);
@@ -191,9 +213,10 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_e(const BTable *__restrict__
                                                       const BMember *__restrict__ M,
                                                       const BState *__restrict__ S)
 {
-    const int total = T->total;
-    const int chunk = (total + (int)gridDim.x - 1) / (int)gridDim.x;
-    const int t0 = (int)blockIdx.x * chunk, t1 = min(t0 + chunk, total);
+    int t_lo, t_hi, e_lo, e_hi;
+    batch_group(T, t_lo, t_hi, e_lo, e_hi);
+    const int chunk = (t_hi - t_lo + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t0 = t_lo + (int)blockIdx.x * chunk, t1 = min(t0 + chunk, t_hi);
     if (t0 >= t1) return;
     BCursor c;
     batch_seek(T, S, t1 - 1, c);
@@ -207,7 +230,8 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_e(const BTable *__restrict__
         PassSrc ps;
         ps.n = B.nsrc; ps.comp0 = 0; ps.blk = B.blk;
         ps.st = f[3]; ps.ct = f[4]; ps.env = f[5];
-        e_column<false>(B.A, B.g, ps, k, j, z0, z1);
+        if (B.A.bI) e_column<CM_INDEX, 0>(B.A, B.g, ps, k, j, z0, z1);
+        else e_column<CM_REAL, 0>(B.A, B.g, ps, k, j, z0, z1);
     }
 }
 
@@ -218,9 +242,10 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_phasor(const BTable *__restr
                                                            const BMember *__restrict__ M,
                                                            const BState *__restrict__ S)
 {
-    const int total = T->total;
-    const int chunk = (total + (int)gridDim.x - 1) / (int)gridDim.x;
-    const int t0 = (int)blockIdx.x * chunk, t1 = min(t0 + chunk, total);
+    int t_lo, t_hi, e_lo, e_hi;
+    batch_group(T, t_lo, t_hi, e_lo, e_hi);
+    const int chunk = (t_hi - t_lo + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t0 = t_lo + (int)blockIdx.x * chunk, t1 = min(t0 + chunk, t_hi);
     if (t0 >= t1) return;
     BCursor c;
     batch_seek(T, S, t0, c);
@@ -240,8 +265,10 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_phasor(const BTable *__restr
 __global__ void k_batch_tick(const BTable *__restrict__ T, const BMember *__restrict__ M,
                              BState *__restrict__ S)
 {
-    const int e = blockIdx.x;
-    if (e >= T->nact || threadIdx.x != 0) return;
+    int t_lo, t_hi, e_lo, e_hi;
+    batch_group(T, t_lo, t_hi, e_lo, e_hi);
+    const int e = e_lo + (int)blockIdx.x;
+    if (e >= e_hi || threadIdx.x != 0) return;
     const int m = T->mem[e];
     const BMember &B = M[m];
     BState st = S[m];

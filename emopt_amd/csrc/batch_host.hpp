@@ -30,6 +30,8 @@ struct emf_batch {
     int spg = 64;                       // steps per graph replay
     int zc = 2;                         // z planes per work-group
     int bpc = 8;                        // persistent work-groups per CU
+    long long group_bytes = 0;          // hot-state bytes per group (0: one group)
+    int gnext = 0;                      // group the next replay steps
     int ncu = 0;
     BTable *T = nullptr;
     BMember *M = nullptr;
@@ -61,6 +63,22 @@ int batch_upload_table(emf_batch *b)
         ++t.nact;
     }
     t.off[t.nact] = t.total;
+    // groups: consecutive entries whose hot state (E, H, coefficients: ~80
+    // B/cell) stays within group_bytes
+    t.ngroups = 0;
+    long long acc = 0;
+    for (int e = 0; e < t.nact; ++e) {
+        const long long hot = 80LL * b->hM[t.mem[e]].n;
+        if (e == 0 || (b->group_bytes > 0 && acc + hot > b->group_bytes)) {
+            t.gent[t.ngroups++] = e;
+            acc = 0;
+        }
+        acc += hot;
+    }
+    if (t.ngroups == 0) t.gent[t.ngroups++] = 0;
+    t.gent[t.ngroups] = t.nact;
+    t.cur = 0;
+    b->gnext = 0;
     HIPCHK(hipMemcpyAsync(b->T, &t, sizeof(BTable), hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));   // t is rewritten by the next call
     return EMF_OK;
@@ -193,6 +211,15 @@ int emf_batch_set_tuning(emf_batch *b, int steps_per_graph, int zc, int blocks_p
     b->zc = zc;
     b->bpc = blocks_per_cu;
     return EMF_OK;
+}
+
+int emf_batch_set_grouping(emf_batch *b, int64_t group_bytes)
+{
+    BLOCK(b);
+    if (group_bytes < 0) EMF_FAIL(EMF_ERR_ARG, "group_bytes must be >= 0");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->group_bytes = group_bytes;
+    return batch_upload_table(b);
 }
 
 int emf_batch_add(emf_batch *b, emf_solver *s, int slot, double rtol, double nmax, int64_t Tn,
@@ -355,7 +382,10 @@ int emf_batch_run(emf_batch *b, int max_graphs, int until, int *finished)
             while (!stop && rc == EMF_OK && (int)pend.size() < 2 &&
                    (until == 2 || launched < max_graphs)) {
                 hipEvent_t ev;
-                if (hipGraphLaunch(b->exec, b->stream) != hipSuccess ||
+                hipLaunchKernelGGL(k_batch_set_group, dim3(1), dim3(64), 0, b->stream, b->T,
+                                   b->gnext);
+                b->gnext = (b->gnext + 1) % std::max(1, b->hT.ngroups);
+                if (hipGetLastError() != hipSuccess || hipGraphLaunch(b->exec, b->stream) != hipSuccess ||
                     hipMemcpyAsync(b->hS + (size_t)b->ring * BATCH_MAX, b->S,
                                    BATCH_MAX * sizeof(BState), hipMemcpyDeviceToHost,
                                    b->stream) != hipSuccess ||

@@ -422,7 +422,7 @@ int build_coeff_index(emf_solver *s, const double *ex, const double *ey, const d
         std::vector<uint16_t> i16((size_t)n);
         for (long long g = 0; g < n; ++g) i16[(size_t)g] = (uint16_t)idx[(size_t)g];
         uint16_t *d = nullptr;
-        CHECK(dalloc(&d, (size_t)n));
+        CHECK(dalloc(&d, (size_t)n + 2));   // the E pass reads aligned dwords
         s->cidx = d;
         HIPCHK(hipMemcpyAsync(d, i16.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
@@ -664,17 +664,26 @@ void launch_h_occ(const emf_solver *s, const FusedArgs &A, const Geo &g, const P
     }
 }
 
-template <bool CE, int OCC>
-void launch_e_occ(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
-                  const int *done)
+template <int CM, int OCC, int ZU>
+void launch_e_zu(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
+                 const int *done)
 {
     const dim3 gr = pass_grid(s);
     switch (pass_ty(s)) {
-    case 1: hipLaunchKernelGGL((k_pass_e<CE, 1, OCC>), gr, dim3(PX, 1), 0, s->stream, A, g, ps, done); break;
-    case 2: hipLaunchKernelGGL((k_pass_e<CE, 2, OCC>), gr, dim3(PX, 2), 0, s->stream, A, g, ps, done); break;
-    case 8: hipLaunchKernelGGL((k_pass_e<CE, 8, OCC>), gr, dim3(PX, 8), 0, s->stream, A, g, ps, done); break;
-    default: hipLaunchKernelGGL((k_pass_e<CE, 4, OCC>), gr, dim3(PX, 4), 0, s->stream, A, g, ps, done);
+    case 1: hipLaunchKernelGGL((k_pass_e<CM, 1, OCC, ZU>), gr, dim3(PX, 1), 0, s->stream, A, g, ps, done); break;
+    case 2: hipLaunchKernelGGL((k_pass_e<CM, 2, OCC, ZU>), gr, dim3(PX, 2), 0, s->stream, A, g, ps, done); break;
+    case 8: hipLaunchKernelGGL((k_pass_e<CM, 8, OCC, ZU>), gr, dim3(PX, 8), 0, s->stream, A, g, ps, done); break;
+    default: hipLaunchKernelGGL((k_pass_e<CM, 4, OCC, ZU>), gr, dim3(PX, 4), 0, s->stream, A, g, ps, done);
     }
+}
+
+// columns of at most 2 planes take the unrolled E pass (indices loaded first)
+template <int CM, int OCC>
+void launch_e_occ(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
+                  const int *done)
+{
+    if (CM == CM_INDEX && s->zc <= 2) launch_e_zu<CM, OCC, 2>(s, A, g, ps, done);
+    else launch_e_zu<CM, OCC, 0>(s, A, g, ps, done);
 }
 
 // kind 3 = the two passes compiled for 5 waves per SIMD
@@ -686,12 +695,12 @@ void launch_h_t(const emf_solver *s, const FusedArgs &A, const Geo &g, const Pas
     else launch_h_occ<MU, 4>(s, A, g, ps, done);
 }
 
-template <bool CE>
+template <int CM>
 void launch_e_t(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
                 const int *done)
 {
-    if (s->kind == 3) launch_e_occ<CE, 5>(s, A, g, ps, done);
-    else launch_e_occ<CE, 4>(s, A, g, ps, done);
+    if (s->kind == 3) launch_e_occ<CM, 5>(s, A, g, ps, done);
+    else launch_e_occ<CM, 4>(s, A, g, ps, done);
 }
 
 // H pass in place on the current set (update_H, src/fdtd.cpp:121-367)
@@ -718,8 +727,9 @@ int launch_e(emf_solver *s, double t, const int *done)
     PassSrc ps = pass_sources(s, 0, t);
     FusedArgs A = make_args(s, s->cur, s->cur);
     Geo g = s->geo();
-    if (s->ceps) launch_e_t<true>(s, A, g, ps, done);
-    else launch_e_t<false>(s, A, g, ps, done);
+    if (s->ceps) launch_e_t<CM_COMPLEX>(s, A, g, ps, done);
+    else if (A.bI) launch_e_t<CM_INDEX>(s, A, g, ps, done);
+    else launch_e_t<CM_REAL>(s, A, g, ps, done);
     HIPCHK(hipGetLastError());
     return EMF_OK;
 }

@@ -5,7 +5,8 @@
 // reads H(3) E(3) b_eps(3) and writes E(3) = 96 B; plus the psi slabs.  With
 // real eps the E pass can read a 2- or 4-byte index per cell into a small
 // table of distinct (b_x, b_y, b_z) instead of the three coefficients (the
-// same doubles, so the same results): 74 or 76 B per cell.
+// same doubles, so the same results): 74 or 76 B per cell.  The index loads
+// go out before a plane's other loads and the table gather after them.
 //
 // A 64 x TY work-group owns a tile of (x, y) columns and sweeps zc planes
 // (zc = 1 is one cell per thread).  Every load a plane needs -- fields,
@@ -115,117 +116,166 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_h(FusedArgs A, Geo g, Pas
 }
 
 // ------------------------------------------------------------------ E pass
-template <bool CEPS>
+// Coefficient modes of the E pass: real eps with full b arrays, complex eps
+// (a and b arrays), real eps with b gathered through a per-cell index.
+enum { CM_REAL = 0, CM_COMPLEX = 1, CM_INDEX = 2 };
+
+// Entry of the per-cell coefficient index at byte offset o of a field
+// component: one aligned dword holds the 2- or 4-byte entry (the 2-byte index
+// is allocated with two spare entries for the last dword).
+__device__ __forceinline__ unsigned coef_index(const FusedArgs &A, unsigned o)
+{
+    const unsigned e = o >> 3, w2 = A.bIw == 2;
+    const unsigned v = *(const unsigned *)((const char *)A.bI + (w2 ? (e << 1) & ~3u : e << 2));
+    return w2 ? (v >> ((e & 1u) << 4)) & 0xffffu : v;
+}
+
+// One cell (i, j, k) at byte offset o; hxm/hym hold H_x, H_y at i-1 and are
+// advanced to plane i.  With CM_INDEX, ci is the cell's coefficient index,
+// loaded by the caller ahead of the plane's other loads: the table gather is
+// issued last, after every other load of the plane (psi included).
+template <int CM>
+__device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const PassSrc &se,
+                                        const Layout &L, int k, int j, int i, unsigned o,
+                                        unsigned ci, bool col_src, double &hxm, double &hym)
+{
+    const unsigned cs = A.cs, cs2 = 2 * A.cs;
+    const unsigned oy = (unsigned)g.Nx * 8u;
+    // min-face ghost rule: '0' -> 0, 'E' -> -H(first), 'H' -> +H(first)
+    const double sgx = g.bc[0] == 'E' ? -1.0 : (g.bc[0] == 'H' ? 1.0 : 0.0);
+    const double sgy = g.bc[1] == 'E' ? -1.0 : (g.bc[1] == 'H' ? 1.0 : 0.0);
+    const bool bx0 = g.bc[0] == '0', by0 = g.bc[1] == '0';
+    constexpr bool CEPS = CM == CM_COMPLEX;
+    // ---- all loads of this plane
+    const double hx = ldo(A.Ho, o), hy = ldo(A.Ho, o + cs), hz = ldo(A.Ho, o + cs2);
+    // row j-1 and column k-1; at j == 0 / k == 0 the own cell stands in
+    // (unconditional loads keep the count of loads behind the index fixed)
+    // and the ghost rule below replaces it
+    const unsigned dj = j > 0 ? oy : 0u, dk = k > 0 ? 8u : 0u;
+    double hz_j = ldo(A.Ho, o - dj + cs2), hx_j = ldo(A.Ho, o - dj);
+    double hz_k = ldo(A.Ho, o - dk + cs2), hy_k = ldo(A.Ho, o - dk + cs);
+    const double ex = ldo(A.Eo, o), ey = ldo(A.Eo, o + cs), ez = ldo(A.Eo, o + cs2);
+    double bx = 0, by = 0, bz = 0;
+    if (CM != CM_INDEX) { bx = ldo(A.bE, o); by = ldo(A.bE, o + cs); bz = ldo(A.bE, o + cs2); }
+    double ax = 1, ay = 1, az = 1;
+    if (CEPS) { ax = ldo(A.aE, o); ay = ldo(A.aE, o + cs); az = ldo(A.aE, o + cs2); }
+    const Slab sx = slab_x(g, i, j, k), sy = slab_y(g, i, j, k), sz = slab_z(g, i, j, k);
+    const unsigned fx = sx.lo ? 0 : 1, fy = sy.lo ? 2 : 3, fz = sz.lo ? 4 : 5;
+    double qx0 = 0, qx1 = 0, qy0 = 0, qy1 = 0, qz0 = 0, qz1 = 0;
+    double kx = 1, tbx = 0, tcx = 0, ky = 1, tby = 0, tcy = 0, kz = 1, tbz = 0, tcz = 0;
+    if (sx.in) {
+        qx0 = ldo(A.pe, L.pso[fx][0] + sx.io); qx1 = ldo(A.pe, L.pso[fx][1] + sx.io);
+        const unsigned t = L.tbo[0] + 3 * L.tbl[0] + (unsigned)sx.par * 8u;
+        kx = ldo(A.tab, t); tbx = ldo(A.tab, t + L.tbl[0]); tcx = ldo(A.tab, t + 2 * L.tbl[0]);
+    }
+    if (sy.in) {
+        qy0 = ldo(A.pe, L.pso[fy][0] + sy.io); qy1 = ldo(A.pe, L.pso[fy][1] + sy.io);
+        const unsigned t = L.tbo[1] + 3 * L.tbl[1] + (unsigned)sy.par * 8u;
+        ky = ldo(A.tab, t); tby = ldo(A.tab, t + L.tbl[1]); tcy = ldo(A.tab, t + 2 * L.tbl[1]);
+    }
+    if (sz.in) {
+        qz0 = ldo(A.pe, L.pso[fz][0] + sz.io); qz1 = ldo(A.pe, L.pso[fz][1] + sz.io);
+        const unsigned t = L.tbo[2] + 3 * L.tbl[2] + (unsigned)sz.par * 8u;
+        kz = ldo(A.tab, t); tbz = ldo(A.tab, t + L.tbl[2]); tcz = ldo(A.tab, t + 2 * L.tbl[2]);
+    }
+    if (CM == CM_INDEX) { bx = A.bT[3 * ci]; by = A.bT[3 * ci + 1]; bz = A.bT[3 * ci + 2]; }
+    // ---- boundary ghosts at the min faces
+    if (j == 0) { hz_j = by0 ? 0.0 : sgy * hz; hx_j = by0 ? 0.0 : sgy * hx; }
+    if (k == 0) { hz_k = bx0 ? 0.0 : sgx * hz; hy_k = bx0 ? 0.0 : sgx * hy; }
+    // ---- update (src/fdtd.cpp:554-666)
+    const double dHzdy = g.ry * (hz - hz_j);
+    const double dHydz = g.rz * (hy - hym);
+    double nex = CEPS ? ax * ex + (dHzdy - dHydz) * bx : ex + (dHzdy - dHydz) * bx;
+    const double dHxdz = g.rz * (hx - hxm);
+    const double dHzdx = g.rx * (hz - hz_k);
+    double ney = CEPS ? ay * ey + (dHxdz - dHzdx) * by : ey + (dHxdz - dHzdx) * by;
+    const double dHydx = g.rx * (hy - hy_k);
+    const double dHxdy = g.ry * (hx - hx_j);
+    double nez = CEPS ? az * ez + (dHydx - dHxdy) * bz : ez + (dHydx - dHxdy) * bz;
+    if (sx.in) {
+        qx0 = tcx * dHydx + tbx * qx0;
+        qx1 = tcx * dHzdx + tbx * qx1;
+        nez = nez + (qx0 - dHydx + dHydx / kx) * bz;
+        ney = ney - (qx1 - dHzdx + dHzdx / kx) * by;
+    }
+    if (sy.in) {
+        qy0 = tcy * dHxdy + tby * qy0;
+        qy1 = tcy * dHzdy + tby * qy1;
+        nez = nez - (qy0 - dHxdy + dHxdy / ky) * bz;
+        nex = nex + (qy1 - dHzdy + dHzdy / ky) * bx;
+    }
+    if (sz.in) {
+        qz0 = tcz * dHxdz + tbz * qz0;
+        qz1 = tcz * dHydz + tbz * qz1;
+        nex = nex - (qz1 - dHydz + dHydz / kz) * bx;
+        ney = ney + (qz0 - dHxdz + dHxdz / kz) * by;
+    }
+    if (col_src) {
+        for (int q = 0; q < se.n; ++q) {
+            long long m;
+            const SrcBlk &sb = se.blk[q];
+            if (in_box(sb, i, j, k, m)) {
+                const double2 vx = sb.f[0][m], vy = sb.f[1][m], vz = sb.f[2][m];
+                nex = nex - src_val(se, vx) * bx;
+                ney = ney - src_val(se, vy) * by;
+                nez = nez - src_val(se, vz) * bz;
+            }
+        }
+    }
+    sto(A.En, o, nex); sto(A.En, o + cs, ney); sto(A.En, o + cs2, nez);
+    if (sx.in) { sto(A.pe, L.pso[fx][0] + sx.io, qx0); sto(A.pe, L.pso[fx][1] + sx.io, qx1); }
+    if (sy.in) { sto(A.pe, L.pso[fy][0] + sy.io, qy0); sto(A.pe, L.pso[fy][1] + sy.io, qy1); }
+    if (sz.in) { sto(A.pe, L.pso[fz][0] + sz.io, qz0); sto(A.pe, L.pso[fz][1] + sz.io, qz1); }
+    hxm = hx; hym = hy;
+}
+
+// Cells z0 <= i < z1 of one (k, j) column.  ZU > 0: the column has at most
+// ZU planes and the loop is unrolled, so the coefficient indices of all its
+// planes are loaded first, ahead of everything else -- a plane's gather then
+// never waits for its own index.  ZU = 0: any number of planes, each plane
+// loading its index first.
+template <int CM, int ZU>
 __device__ __forceinline__ void e_column(const FusedArgs &A, const Geo &g, const PassSrc &se,
                                          int k, int j, int z0, int z1)
 {
     const Layout L = make_layout(g);
-    const int Nx = g.Nx, Ny = g.Ny;
-    const unsigned cs = A.cs, cs2 = 2 * A.cs;
-    const unsigned oz = (unsigned)g.nxy * 8u, oy = (unsigned)Nx * 8u;
-    // min-face ghost rule: '0' -> 0, 'E' -> -H(first), 'H' -> +H(first)
-    const double sgx = g.bc[0] == 'E' ? -1.0 : (g.bc[0] == 'H' ? 1.0 : 0.0);
-    const double sgy = g.bc[1] == 'E' ? -1.0 : (g.bc[1] == 'H' ? 1.0 : 0.0);
+    const unsigned cs = A.cs;
+    const unsigned oz = (unsigned)g.nxy * 8u;
     const double sgz = g.bc[2] == 'E' ? -1.0 : (g.bc[2] == 'H' ? 1.0 : 0.0);
-    const bool bx0 = g.bc[0] == '0', by0 = g.bc[1] == '0', bz0 = g.bc[2] == '0';
+    const bool bz0 = g.bc[2] == '0';
     bool col_src = false;
     for (int q = 0; q < se.n; ++q) {
         const SrcBlk &s = se.blk[q];
         if (k >= s.k0 && k < s.k0 + s.K && j >= s.j0 && j < s.j0 + s.J && s.i0 < z1 && s.i0 + s.I > z0)
             col_src = true;
     }
-    unsigned o = ((unsigned)z0 * (unsigned)Ny + (unsigned)j) * (unsigned)Nx * 8u + (unsigned)k * 8u;
-    double hxm, hym;   // H at i-1
-    if (z0 > 0) { hxm = ldo(A.Ho, o - oz); hym = ldo(A.Ho, o - oz + cs); }
-    else { hxm = bz0 ? 0.0 : sgz * ldo(A.Ho, o); hym = bz0 ? 0.0 : sgz * ldo(A.Ho, o + cs); }
-    for (int i = z0; i < z1; ++i, o += oz) {
-        // ---- all loads of this plane
-        const double hx = ldo(A.Ho, o), hy = ldo(A.Ho, o + cs), hz = ldo(A.Ho, o + cs2);
-        double hz_j = 0, hx_j = 0, hz_k = 0, hy_k = 0;
-        if (j > 0) { hz_j = ldo(A.Ho, o - oy + cs2); hx_j = ldo(A.Ho, o - oy); }
-        if (k > 0) { hz_k = ldo(A.Ho, o - 8u + cs2); hy_k = ldo(A.Ho, o - 8u + cs); }
-        const double ex = ldo(A.Eo, o), ey = ldo(A.Eo, o + cs), ez = ldo(A.Eo, o + cs2);
-        double bx, by, bz;
-        if (!CEPS && A.bI) {       // compressed coefficients: index, then table
-            const unsigned t = A.bIw == 2 ? (unsigned)((const unsigned short *)A.bI)[o >> 3]
-                                          : ((const unsigned *)A.bI)[o >> 3];
-            bx = A.bT[3 * t]; by = A.bT[3 * t + 1]; bz = A.bT[3 * t + 2];
-        } else {
-            bx = ldo(A.bE, o); by = ldo(A.bE, o + cs); bz = ldo(A.bE, o + cs2);
+    unsigned o = ((unsigned)z0 * (unsigned)g.Ny + (unsigned)j) * (unsigned)g.Nx * 8u + (unsigned)k * 8u;
+    if (ZU > 0) {
+        unsigned ci[ZU > 0 ? ZU : 1];
+#pragma unroll
+        for (int p = 0; p < ZU; ++p)   // a missing plane re-reads the last one
+            ci[p] = CM == CM_INDEX ? coef_index(A, o + (unsigned)min(p, z1 - z0 - 1) * oz) : 0u;
+        double hxm, hym;   // H at i-1
+        if (z0 > 0) { hxm = ldo(A.Ho, o - oz); hym = ldo(A.Ho, o - oz + cs); }
+        else { hxm = bz0 ? 0.0 : sgz * ldo(A.Ho, o); hym = bz0 ? 0.0 : sgz * ldo(A.Ho, o + cs); }
+#pragma unroll
+        for (int p = 0; p < ZU; ++p) {
+            if (z0 + p >= z1) break;
+            e_plane<CM>(A, g, se, L, k, j, z0 + p, o + (unsigned)p * oz, ci[p], col_src, hxm, hym);
         }
-        double ax = 1, ay = 1, az = 1;
-        if (CEPS) { ax = ldo(A.aE, o); ay = ldo(A.aE, o + cs); az = ldo(A.aE, o + cs2); }
-        const Slab sx = slab_x(g, i, j, k), sy = slab_y(g, i, j, k), sz = slab_z(g, i, j, k);
-        const unsigned fx = sx.lo ? 0 : 1, fy = sy.lo ? 2 : 3, fz = sz.lo ? 4 : 5;
-        double qx0 = 0, qx1 = 0, qy0 = 0, qy1 = 0, qz0 = 0, qz1 = 0;
-        double kx = 1, tbx = 0, tcx = 0, ky = 1, tby = 0, tcy = 0, kz = 1, tbz = 0, tcz = 0;
-        if (sx.in) {
-            qx0 = ldo(A.pe, L.pso[fx][0] + sx.io); qx1 = ldo(A.pe, L.pso[fx][1] + sx.io);
-            const unsigned t = L.tbo[0] + 3 * L.tbl[0] + (unsigned)sx.par * 8u;
-            kx = ldo(A.tab, t); tbx = ldo(A.tab, t + L.tbl[0]); tcx = ldo(A.tab, t + 2 * L.tbl[0]);
+    } else {
+        double hxm, hym;
+        if (z0 > 0) { hxm = ldo(A.Ho, o - oz); hym = ldo(A.Ho, o - oz + cs); }
+        else { hxm = bz0 ? 0.0 : sgz * ldo(A.Ho, o); hym = bz0 ? 0.0 : sgz * ldo(A.Ho, o + cs); }
+        for (int i = z0; i < z1; ++i, o += oz) {
+            const unsigned ci = CM == CM_INDEX ? coef_index(A, o) : 0u;
+            e_plane<CM>(A, g, se, L, k, j, i, o, ci, col_src, hxm, hym);
         }
-        if (sy.in) {
-            qy0 = ldo(A.pe, L.pso[fy][0] + sy.io); qy1 = ldo(A.pe, L.pso[fy][1] + sy.io);
-            const unsigned t = L.tbo[1] + 3 * L.tbl[1] + (unsigned)sy.par * 8u;
-            ky = ldo(A.tab, t); tby = ldo(A.tab, t + L.tbl[1]); tcy = ldo(A.tab, t + 2 * L.tbl[1]);
-        }
-        if (sz.in) {
-            qz0 = ldo(A.pe, L.pso[fz][0] + sz.io); qz1 = ldo(A.pe, L.pso[fz][1] + sz.io);
-            const unsigned t = L.tbo[2] + 3 * L.tbl[2] + (unsigned)sz.par * 8u;
-            kz = ldo(A.tab, t); tbz = ldo(A.tab, t + L.tbl[2]); tcz = ldo(A.tab, t + 2 * L.tbl[2]);
-        }
-        // ---- boundary ghosts at the min faces
-        if (j == 0) { hz_j = by0 ? 0.0 : sgy * hz; hx_j = by0 ? 0.0 : sgy * hx; }
-        if (k == 0) { hz_k = bx0 ? 0.0 : sgx * hz; hy_k = bx0 ? 0.0 : sgx * hy; }
-        // ---- update (src/fdtd.cpp:554-666)
-        const double dHzdy = g.ry * (hz - hz_j);
-        const double dHydz = g.rz * (hy - hym);
-        double nex = CEPS ? ax * ex + (dHzdy - dHydz) * bx : ex + (dHzdy - dHydz) * bx;
-        const double dHxdz = g.rz * (hx - hxm);
-        const double dHzdx = g.rx * (hz - hz_k);
-        double ney = CEPS ? ay * ey + (dHxdz - dHzdx) * by : ey + (dHxdz - dHzdx) * by;
-        const double dHydx = g.rx * (hy - hy_k);
-        const double dHxdy = g.ry * (hx - hx_j);
-        double nez = CEPS ? az * ez + (dHydx - dHxdy) * bz : ez + (dHydx - dHxdy) * bz;
-        if (sx.in) {
-            qx0 = tcx * dHydx + tbx * qx0;
-            qx1 = tcx * dHzdx + tbx * qx1;
-            nez = nez + (qx0 - dHydx + dHydx / kx) * bz;
-            ney = ney - (qx1 - dHzdx + dHzdx / kx) * by;
-        }
-        if (sy.in) {
-            qy0 = tcy * dHxdy + tby * qy0;
-            qy1 = tcy * dHzdy + tby * qy1;
-            nez = nez - (qy0 - dHxdy + dHxdy / ky) * bz;
-            nex = nex + (qy1 - dHzdy + dHzdy / ky) * bx;
-        }
-        if (sz.in) {
-            qz0 = tcz * dHxdz + tbz * qz0;
-            qz1 = tcz * dHydz + tbz * qz1;
-            nex = nex - (qz1 - dHydz + dHydz / kz) * bx;
-            ney = ney + (qz0 - dHxdz + dHxdz / kz) * by;
-        }
-        if (col_src) {
-            for (int q = 0; q < se.n; ++q) {
-                long long m;
-                const SrcBlk &sb = se.blk[q];
-                if (in_box(sb, i, j, k, m)) {
-                    const double2 vx = sb.f[0][m], vy = sb.f[1][m], vz = sb.f[2][m];
-                    nex = nex - src_val(se, vx) * bx;
-                    ney = ney - src_val(se, vy) * by;
-                    nez = nez - src_val(se, vz) * bz;
-                }
-            }
-        }
-        sto(A.En, o, nex); sto(A.En, o + cs, ney); sto(A.En, o + cs2, nez);
-        if (sx.in) { sto(A.pe, L.pso[fx][0] + sx.io, qx0); sto(A.pe, L.pso[fx][1] + sx.io, qx1); }
-        if (sy.in) { sto(A.pe, L.pso[fy][0] + sy.io, qy0); sto(A.pe, L.pso[fy][1] + sy.io, qy1); }
-        if (sz.in) { sto(A.pe, L.pso[fz][0] + sz.io, qz0); sto(A.pe, L.pso[fz][1] + sz.io, qz1); }
-        hxm = hx; hym = hy;
     }
 }
 
-template <bool CEPS, int TY, int OCC = 4>
+// ZU: see e_column (the launch picks ZU = 2 when zc <= 2)
+template <int CM, int TY, int OCC = 4, int ZU = 0>
 __global__ __launch_bounds__(PX * TY, OCC) void k_pass_e(FusedArgs A, Geo g, PassSrc se,
                                                         const int *__restrict__ done)
 {
@@ -233,7 +283,7 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_e(FusedArgs A, Geo g, Pas
     int k, j, z0, z1;
     pass_tile(g, TY, k, j, z0, z1, true);
     if (k >= g.Nx || j >= g.Ny) return;
-    e_column<CEPS>(A, g, se, k, j, z0, z1);
+    e_column<CM, ZU>(A, g, se, k, j, z0, z1);
 }
 
 }  // namespace emf

@@ -282,6 +282,14 @@ EMF_API int emf_batch_delete(emf_batch *b);
  * persistent work-groups per CU (default 8).  Only while nothing runs. */
 EMF_API int emf_batch_set_tuning(emf_batch *b, int steps_per_graph, int zc, int blocks_per_cu);
 
+/* Cache-aware stepping: members are grouped (in admission order) so that a
+ * group's hot state -- E, H and coefficients, ~80 B/cell -- stays within
+ * group_bytes, and each graph replay steps one group, the groups taking turns.
+ * Sized to the Infinity Cache, a group's E pass re-reads from cache what its
+ * H pass just wrote.  0 (default) = one group: every replay steps all
+ * members. */
+EMF_API int emf_batch_set_grouping(emf_batch *b, int64_t group_bytes);
+
 /* Admits solver s: resets its fields and CPML as emf_solve does and queues
  * the solve (result slot, rtol, nmax, Tn >= 1, fixed_n0 as for emf_solve).
  * *member receives the member id.  The solver handle stays owned by the

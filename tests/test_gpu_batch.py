@@ -116,6 +116,39 @@ def test_batch_continuous_admission(cases, refs):
     b.close()
 
 
+def test_batch_grouped_stepping(cases, refs):
+    """Cache-sized groups (here about two members per group, so the groups
+    take turns replay by replay), with members admitted part-way: every
+    member still matches its single solve, and exactly matches the same
+    member stepped without groups."""
+    from emopt_amd.batch import Batch
+    one = max(80 * c["Nx"] * c["Ny"] * c["Nz"] for c in cases)
+    b = Batch(steps_per_graph=8, group_bytes=2 * one)
+    sims = {}
+    for q in (0, 1, 2):
+        sims[q] = build_sim(cases[q])
+        b.add(sims[q], adjoint=cases[q]["adjoint"], postprocess=False)
+    b.run(until="graphs", max_graphs=15)
+    for q in (3, 4, 5):
+        sims[q] = build_sim(cases[q])
+        b.add(sims[q], adjoint=cases[q]["adjoint"], postprocess=False)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        b.run()
+    assert b.running == 0
+    for q, s in sims.items():
+        _check(s, cases[q], refs[q], "grouped member %d" % q)
+    b.close()
+    flat = Batch(steps_per_graph=8, group_bytes=0)
+    s4 = build_sim(cases[4])
+    flat.add(s4, adjoint=cases[4]["adjoint"], postprocess=False)
+    flat.run()
+    for x, y in zip(all_fields(s4, cases[4], cases[4]["adjoint"]),
+                    all_fields(sims[4], cases[4], cases[4]["adjoint"])):
+        assert np.array_equal(x, y)
+    flat.close()
+
+
 def test_batch_fixed_n0_and_nmax(cases):
     """fixed_n0 members (status 'fixed') and a member stopped by Nmax
     (status 'nmax') against the single-solve path."""
