@@ -734,22 +734,31 @@ int launch_e(emf_solver *s, double t, const int *done)
     return EMF_OK;
 }
 
-template <bool MU, bool CE>
-void launch_lds_t(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &sh,
-                  const PassSrc &se, const int *done)
+template <bool MU, int CM>
+void launch_lds_cm(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &sh,
+                   const PassSrc &se, const int *done)
 {
     const int ty = s->tyg < 4 ? 4 : s->tyg;
     dim3 grid((s->Nx + FXO - 1) / FXO, (s->Ny + ty - 2) / (ty - 1), (s->Nz + s->zc - 1) / s->zc);
     switch (ty) {
     case 16:
-        hipLaunchKernelGGL((k_step_lds<MU, CE, 16>), grid, dim3(FX, 16), 0, s->stream, A, g, sh, se, done);
+        hipLaunchKernelGGL((k_step_lds<MU, CM, 16>), grid, dim3(FX, 16), 0, s->stream, A, g, sh, se, done);
         break;
     case 8:
-        hipLaunchKernelGGL((k_step_lds<MU, CE, 8>), grid, dim3(FX, 8), 0, s->stream, A, g, sh, se, done);
+        hipLaunchKernelGGL((k_step_lds<MU, CM, 8>), grid, dim3(FX, 8), 0, s->stream, A, g, sh, se, done);
         break;
     default:
-        hipLaunchKernelGGL((k_step_lds<MU, CE, 4>), grid, dim3(FX, 4), 0, s->stream, A, g, sh, se, done);
+        hipLaunchKernelGGL((k_step_lds<MU, CM, 4>), grid, dim3(FX, 4), 0, s->stream, A, g, sh, se, done);
     }
+}
+
+template <bool MU, bool CE>
+void launch_lds_t(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &sh,
+                  const PassSrc &se, const int *done)
+{
+    if (CE) launch_lds_cm<MU, CM_COMPLEX>(s, A, g, sh, se, done);
+    else if (A.bI) launch_lds_cm<MU, CM_INDEX>(s, A, g, sh, se, done);
+    else launch_lds_cm<MU, CM_REAL>(s, A, g, sh, se, done);
 }
 
 template <bool MU, bool CE>

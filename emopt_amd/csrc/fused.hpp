@@ -47,6 +47,20 @@ struct FusedArgs {
     int bIw;                 // index width in bytes (2 or 4)
 };
 
+// Coefficient modes of the E update: real eps with full b arrays, complex eps
+// (a and b arrays), real eps with b gathered through a per-cell index.
+enum { CM_REAL = 0, CM_COMPLEX = 1, CM_INDEX = 2 };
+
+// Entry of the per-cell coefficient index at byte offset o of a field
+// component: one aligned dword holds the 2- or 4-byte entry (the 2-byte index
+// is allocated with two spare entries for the last dword).
+__device__ __forceinline__ unsigned coef_index(const FusedArgs &A, unsigned o)
+{
+    const unsigned e = o >> 3, w2 = A.bIw == 2;
+    const unsigned v = *(const unsigned *)((const char *)A.bI + (w2 ? (e << 1) & ~3u : e << 2));
+    return w2 ? (v >> ((e & 1u) << 4)) & 0xffffu : v;
+}
+
 // Byte offsets of the psi slab arrays of a set ([face][component], faces
 // xmin xmax ymin ymax zmin zmax, each face's two arrays consecutive) and of
 // the CPML tables (per axis: kH bH cH kE bE cE, w_min + w_max entries each).
@@ -76,6 +90,14 @@ __device__ __forceinline__ Layout make_layout(const Geo &g)
         t += 6u * (unsigned)len * 8u;
     }
     return L;
+}
+
+// Byte offset of psi array c (0, 1) of the min (lo) or max face of an axis.
+// The select between two constant-indexed entries keeps the Layout in
+// registers (a runtime face index would send the array to memory).
+__device__ __forceinline__ unsigned psof(const Layout &L, int axis, bool lo, int c)
+{
+    return lo ? L.pso[2 * axis][c] : L.pso[2 * axis + 1][c];
 }
 
 // Accesses by 32-bit byte offset from a uniform base pointer: the backend then
@@ -164,23 +186,20 @@ __device__ __forceinline__ void h_load_aux(const Geo &g, const Layout &L, const 
     d.px0 = d.px1 = d.py0 = d.py1 = d.pz0 = d.pz1 = 0;
     d.kx = d.bx = d.cx = d.ky = d.by = d.cy = d.kz = d.bz = d.cz = 0;
     if (d.sx.in) {
-        const unsigned f = d.sx.lo ? 0 : 1;
-        d.px0 = ldo(A.pho, L.pso[f][0] + d.sx.io);
-        d.px1 = ldo(A.pho, L.pso[f][1] + d.sx.io);
+        d.px0 = ldo(A.pho, psof(L, 0, d.sx.lo, 0) + d.sx.io);
+        d.px1 = ldo(A.pho, psof(L, 0, d.sx.lo, 1) + d.sx.io);
         const unsigned t = L.tbo[0] + (unsigned)d.sx.par * 8u;
         d.kx = ldo(A.tab, t); d.bx = ldo(A.tab, t + L.tbl[0]); d.cx = ldo(A.tab, t + 2 * L.tbl[0]);
     }
     if (d.sy.in) {
-        const unsigned f = d.sy.lo ? 2 : 3;
-        d.py0 = ldo(A.pho, L.pso[f][0] + d.sy.io);
-        d.py1 = ldo(A.pho, L.pso[f][1] + d.sy.io);
+        d.py0 = ldo(A.pho, psof(L, 1, d.sy.lo, 0) + d.sy.io);
+        d.py1 = ldo(A.pho, psof(L, 1, d.sy.lo, 1) + d.sy.io);
         const unsigned t = L.tbo[1] + (unsigned)d.sy.par * 8u;
         d.ky = ldo(A.tab, t); d.by = ldo(A.tab, t + L.tbl[1]); d.cy = ldo(A.tab, t + 2 * L.tbl[1]);
     }
     if (d.sz.in) {
-        const unsigned f = d.sz.lo ? 4 : 5;
-        d.pz0 = ldo(A.pho, L.pso[f][0] + d.sz.io);
-        d.pz1 = ldo(A.pho, L.pso[f][1] + d.sz.io);
+        d.pz0 = ldo(A.pho, psof(L, 2, d.sz.lo, 0) + d.sz.io);
+        d.pz1 = ldo(A.pho, psof(L, 2, d.sz.lo, 1) + d.sz.io);
         const unsigned t = L.tbo[2] + (unsigned)d.sz.par * 8u;
         d.kz = ldo(A.tab, t); d.bz = ldo(A.tab, t + L.tbl[2]); d.cz = ldo(A.tab, t + 2 * L.tbl[2]);
     }
@@ -363,11 +382,10 @@ __global__ __launch_bounds__(FX * TY) void k_step_fused(FusedArgs A, Geo g, Pass
             }
             // E-side psi and tables of the owned cell
             const Slab &sx = a.sx, &sy = a.sy, &sz = a.sz;
-            const unsigned fx = sx.lo ? 0 : 1, fy = sy.lo ? 2 : 3, fz = sz.lo ? 4 : 5;
             double qx0 = 0, qx1 = 0, qy0 = 0, qy1 = 0, qz0 = 0, qz1 = 0;
-            if (sx.in) { qx0 = ldo(A.pe, L.pso[fx][0] + sx.io); qx1 = ldo(A.pe, L.pso[fx][1] + sx.io); }
-            if (sy.in) { qy0 = ldo(A.pe, L.pso[fy][0] + sy.io); qy1 = ldo(A.pe, L.pso[fy][1] + sy.io); }
-            if (sz.in) { qz0 = ldo(A.pe, L.pso[fz][0] + sz.io); qz1 = ldo(A.pe, L.pso[fz][1] + sz.io); }
+            if (sx.in) { qx0 = ldo(A.pe, psof(L, 0, sx.lo, 0) + sx.io); qx1 = ldo(A.pe, psof(L, 0, sx.lo, 1) + sx.io); }
+            if (sy.in) { qy0 = ldo(A.pe, psof(L, 1, sy.lo, 0) + sy.io); qy1 = ldo(A.pe, psof(L, 1, sy.lo, 1) + sy.io); }
+            if (sz.in) { qz0 = ldo(A.pe, psof(L, 2, sz.lo, 0) + sz.io); qz1 = ldo(A.pe, psof(L, 2, sz.lo, 1) + sz.io); }
 
             const double dHzdy = g.ry * (hz - hz_j);
             const double dHydz = g.rz * (hy - hy_m);
@@ -420,12 +438,12 @@ __global__ __launch_bounds__(FX * TY) void k_step_fused(FusedArgs A, Geo g, Pass
             }
             sto(A.Hn, o, hx); sto(A.Hn, o + cs, hy); sto(A.Hn, o + cs2, hz);
             sto(A.En, o, nex); sto(A.En, o + cs, ney); sto(A.En, o + cs2, nez);
-            if (sx.in) { sto(A.phn, L.pso[fx][0] + sx.io, npa[0]); sto(A.phn, L.pso[fx][1] + sx.io, npa[1]);
-                         sto(A.pe, L.pso[fx][0] + sx.io, qx0); sto(A.pe, L.pso[fx][1] + sx.io, qx1); }
-            if (sy.in) { sto(A.phn, L.pso[fy][0] + sy.io, npa[2]); sto(A.phn, L.pso[fy][1] + sy.io, npa[3]);
-                         sto(A.pe, L.pso[fy][0] + sy.io, qy0); sto(A.pe, L.pso[fy][1] + sy.io, qy1); }
-            if (sz.in) { sto(A.phn, L.pso[fz][0] + sz.io, npa[4]); sto(A.phn, L.pso[fz][1] + sz.io, npa[5]);
-                         sto(A.pe, L.pso[fz][0] + sz.io, qz0); sto(A.pe, L.pso[fz][1] + sz.io, qz1); }
+            if (sx.in) { sto(A.phn, psof(L, 0, sx.lo, 0) + sx.io, npa[0]); sto(A.phn, psof(L, 0, sx.lo, 1) + sx.io, npa[1]);
+                         sto(A.pe, psof(L, 0, sx.lo, 0) + sx.io, qx0); sto(A.pe, psof(L, 0, sx.lo, 1) + sx.io, qx1); }
+            if (sy.in) { sto(A.phn, psof(L, 1, sy.lo, 0) + sy.io, npa[2]); sto(A.phn, psof(L, 1, sy.lo, 1) + sy.io, npa[3]);
+                         sto(A.pe, psof(L, 1, sy.lo, 0) + sy.io, qy0); sto(A.pe, psof(L, 1, sy.lo, 1) + sy.io, qy1); }
+            if (sz.in) { sto(A.phn, psof(L, 2, sz.lo, 0) + sz.io, npa[4]); sto(A.phn, psof(L, 2, sz.lo, 1) + sz.io, npa[5]);
+                         sto(A.pe, psof(L, 2, sz.lo, 0) + sz.io, qz0); sto(A.pe, psof(L, 2, sz.lo, 1) + sz.io, qz1); }
         }
         hxm = hx; hym = hy;
         ex = a.exi; ey = a.eyi; ez = ezi;
@@ -440,15 +458,24 @@ namespace emf {
 // ---------------------------------------------------------------------------
 // Fused step, LDS variant: a 64 x TY work-group owns 63 x (TY-1) cells of a
 // tile (lane 0 = halo column k0-1, wave 0 = halo row j0-1; both compute H_new
-// only).  H_new of row j-1 comes from LDS (one barrier per plane, buffers
-// alternate by plane parity), H_new of k-1 from the neighbouring lane, H_new
-// of i-1 from registers.  CPML coefficients and psi are loaded inside the slab
-// branches, which keeps the live register set small (higher occupancy).
-template <bool MU_ARRAY, bool CEPS, int TY>
+// only) and sweeps a chunk of z planes, starting one plane below the chunk
+// (H_new of plane z0-1 is recomputed, nothing written).  H_new of row j-1
+// comes from LDS (one barrier per plane, buffers alternate by plane parity),
+// H_new of k-1 from the neighbouring lane, H_new of i-1 from registers.
+//
+// Per plane every load goes out before any compute: the coefficient index
+// first, then the E and H inputs of H_new, the CPML psi of both updates, the
+// E coefficients, and the coefficient-table gather last; the thread then
+// waits for memory once per plane.  Neighbour loads past the grid's max faces
+// read the cell itself and are replaced by the zero ghost, so the loads need
+// no branches.  Cells outside the grid (halo lanes at the edges) load cell
+// (i, 0, 0) and hold zeros.
+template <bool MU_ARRAY, int CM, int TY>
 __global__ __launch_bounds__(FX * TY) void k_step_lds(FusedArgs A, Geo g, PassSrc sh, PassSrc se,
                                                      const int *__restrict__ done)
 {
     if (done && *done) return;
+    constexpr bool CEPS = CM == CM_COMPLEX;
     __shared__ double lds[2][TY][FX][2];   // (Hx, Hz) of H_new per row
     const Layout L = make_layout(g);
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
@@ -460,11 +487,12 @@ __global__ __launch_bounds__(FX * TY) void k_step_lds(FusedArgs A, Geo g, PassSr
     const int z0 = bz * g.zc, z1 = min(z0 + g.zc, Nz);
     const bool hcell = k >= 0 && k < Nx && j >= 0 && j < Ny;
     const bool own = hcell && lane >= 1 && row >= 1;
-    const bool jn = j + 1 < Ny, kn = k + 1 < Nx;
+    const bool jn = hcell && j + 1 < Ny, kn = hcell && k + 1 < Nx;
     const unsigned cs = A.cs, cs2 = 2 * A.cs;
     const unsigned oz = (unsigned)g.nxy * 8u, oy = (unsigned)Nx * 8u;
     const unsigned o0 = hcell ? ((unsigned)j * (unsigned)Nx + (unsigned)k) * 8u : 0u;
-    bool tile_src = false;
+    const unsigned djp = jn ? oy : 0u, dkp = kn ? 8u : 0u;
+    bool tsh = false, tse = false;   // does the tile meet an H / E source block?
     {
         const int tk0 = bx * FXO - 1, tk1 = tk0 + FX, tj0 = by * (TY - 1) - 1, tj1 = tj0 + TY;
         const int tz0 = z0 > 0 ? z0 - 1 : z0;
@@ -472,90 +500,82 @@ __global__ __launch_bounds__(FX * TY) void k_step_lds(FusedArgs A, Geo g, PassSr
             const SrcBlk &s = sh.blk[q];
             if (s.k0 < tk1 && s.k0 + s.K > tk0 && s.j0 < tj1 && s.j0 + s.J > tj0 && s.i0 < z1 &&
                 s.i0 + s.I > tz0)
-                tile_src = true;
+                tsh = true;
+        }
+        for (int q = 0; q < se.n; ++q) {
+            const SrcBlk &s = se.blk[q];
+            if (s.k0 < tk1 && s.k0 + s.K > tk0 && s.j0 < tj1 && s.j0 + s.J > tj0 && s.i0 < z1 &&
+                s.i0 + s.I > z0)
+                tse = true;
         }
     }
+    const double sgx = g.bc[0] == 'E' ? -1.0 : (g.bc[0] == 'H' ? 1.0 : 0.0);
+    const double sgy = g.bc[1] == 'E' ? -1.0 : (g.bc[1] == 'H' ? 1.0 : 0.0);
+    const double sgz = g.bc[2] == 'E' ? -1.0 : (g.bc[2] == 'H' ? 1.0 : 0.0);
     const int zs = z0 > 0 ? z0 - 1 : z0;
-    double ex = 0, ey = 0, ez = 0;
-    if (hcell) {
+    double ex, ey, ez;   // E_old(i)
+    {
         const unsigned o = (unsigned)zs * oz + o0;
         ex = ldo(A.Eo, o); ey = ldo(A.Eo, o + cs); ez = ldo(A.Eo, o + cs2);
+        if (!hcell) ex = ey = ez = 0;
     }
-    double hxm = 0, hym = 0;
+    double hxm = 0, hym = 0;   // H_new(i-1)
     for (int i = zs; i < z1; ++i) {
         const unsigned o = (unsigned)i * oz + o0;
         const bool doE = own && i >= z0;
-        // ---- loads of this plane (PML data is loaded in the slab branches)
-        double exi = 0, eyi = 0, ezi = 0, ezj = 0, exj = 0, hox = 0, hoy = 0, hoz = 0;
-        double bex = 0, bey = 0, bez = 0;
+        const bool up = hcell && i + 1 < Nz;
+        const unsigned dip = up ? oz : 0u;
+        // ---------------- all loads of this plane
+        unsigned ci = 0;
+        if (CM == CM_INDEX) ci = coef_index(A, o);
+        HIn d;
+        d.ex = ex; d.ey = ey; d.ez = ez;
+        d.exi = ldo(A.Eo, o + dip); d.eyi = ldo(A.Eo, o + dip + cs);
+        double ezi = ldo(A.Eo, o + dip + cs2);
+        d.ezj = ldo(A.Eo, o + djp + cs2); d.exj = ldo(A.Eo, o + djp);
+        d.hx = ldo(A.Ho, o); d.hy = ldo(A.Ho, o + cs); d.hz = ldo(A.Ho, o + cs2);
+        double ezk = 0, eyk = 0;   // E_old(k+1) of lane 63 (the others shuffle)
+        if (lane == FX - 1) { ezk = ldo(A.Eo, o + dkp + cs2); eyk = ldo(A.Eo, o + dkp + cs); }
         if (hcell) {
-            if (i + 1 < Nz) { exi = ldo(A.Eo, o + oz); eyi = ldo(A.Eo, o + oz + cs); ezi = ldo(A.Eo, o + oz + cs2); }
-            if (jn) { ezj = ldo(A.Eo, o + oy + cs2); exj = ldo(A.Eo, o + oy); }
-            hox = ldo(A.Ho, o); hoy = ldo(A.Ho, o + cs); hoz = ldo(A.Ho, o + cs2);
+            h_load_aux<MU_ARRAY>(g, L, A, i, j, k, o, d);
+        } else {
+            d.mx = d.my = d.mz = 0;
+            d.sx.in = d.sy.in = d.sz.in = false;
         }
-        if (doE) { bex = ldo(A.bE, o); bey = ldo(A.bE, o + cs); bez = ldo(A.bE, o + cs2); }
-        double ezk = __shfl_down(ez, 1), eyk = __shfl_down(ey, 1);
-        if (lane == FX - 1) {
-            ezk = 0; eyk = 0;
-            if (hcell && kn) { ezk = ldo(A.Eo, o + 8u + cs2); eyk = ldo(A.Eo, o + 8u + cs); }
+        double bex = 0, bey = 0, bez = 0, aex = 1, aey = 1, aez = 1;
+        if (CM != CM_INDEX) { bex = ldo(A.bE, o); bey = ldo(A.bE, o + cs); bez = ldo(A.bE, o + cs2); }
+        if (CEPS) { aex = ldo(A.aE, o); aey = ldo(A.aE, o + cs); aez = ldo(A.aE, o + cs2); }
+        double qx0 = 0, qx1 = 0, qy0 = 0, qy1 = 0, qz0 = 0, qz1 = 0;
+        if (doE) {
+            if (d.sx.in) { qx0 = ldo(A.pe, psof(L, 0, d.sx.lo, 0) + d.sx.io); qx1 = ldo(A.pe, psof(L, 0, d.sx.lo, 1) + d.sx.io); }
+            if (d.sy.in) { qy0 = ldo(A.pe, psof(L, 1, d.sy.lo, 0) + d.sy.io); qy1 = ldo(A.pe, psof(L, 1, d.sy.lo, 1) + d.sy.io); }
+            if (d.sz.in) { qz0 = ldo(A.pe, psof(L, 2, d.sz.lo, 0) + d.sz.io); qz1 = ldo(A.pe, psof(L, 2, d.sz.lo, 1) + d.sz.io); }
         }
-        // ---- H_new(i, j, k)
-        double hx = 0, hy = 0, hz = 0;
+        if (CM == CM_INDEX) { bex = A.bT[3 * ci]; bey = A.bT[3 * ci + 1]; bez = A.bT[3 * ci + 2]; }
+        // ---------------- H_new(i, j, k)
+        if (!up) { d.exi = d.eyi = ezi = 0; }
+        if (!jn) { d.ezj = d.exj = 0; }
+        {
+            const double zk = __shfl_down(d.ez, 1), yk = __shfl_down(d.ey, 1);
+            if (lane == FX - 1) {
+                d.ezk = kn ? ezk : 0.0; d.eyk = kn ? eyk : 0.0;
+            } else {
+                d.ezk = zk; d.eyk = yk;   // zero from a lane outside the grid
+            }
+        }
+        double hx = 0, hy = 0, hz = 0, np[6];
         if (hcell) {
-            const double mx = MU_ARRAY ? ldo(A.mH, o) : A.m_s;
-            const double my = MU_ARRAY ? ldo(A.mH, o + cs) : A.m_s;
-            const double mz = MU_ARRAY ? ldo(A.mH, o + cs2) : A.m_s;
-            const double dEzdy = g.ry * (ezj - ez);
-            const double dEydz = g.rz * (eyi - ey);
-            hx = hox + mx * (dEydz - dEzdy);
-            const double dExdz = g.rz * (exi - ex);
-            const double dEzdx = g.rx * (ezk - ez);
-            hy = hoy + my * (dEzdx - dExdz);
-            const double dEydx = g.rx * (eyk - ey);
-            const double dExdy = g.ry * (exj - ex);
-            hz = hoz + mz * (dExdy - dEydx);
-            const bool wpsi = doE;
-            const Slab sx = slab_x(g, i, j, k);
-            if (sx.in) {
-                const unsigned f = sx.lo ? 0 : 1, t = L.tbo[0] + (unsigned)sx.par * 8u;
-                const double kap = ldo(A.tab, t), b = ldo(A.tab, t + L.tbl[0]), c = ldo(A.tab, t + 2 * L.tbl[0]);
-                const double p0 = c * dEydx + b * ldo(A.pho, L.pso[f][0] + sx.io);
-                const double p1 = c * dEzdx + b * ldo(A.pho, L.pso[f][1] + sx.io);
-                if (wpsi) { sto(A.phn, L.pso[f][0] + sx.io, p0); sto(A.phn, L.pso[f][1] + sx.io, p1); }
-                hz = hz - mz * (p0 - dEydx + dEydx / kap);
-                hy = hy + my * (p1 - dEzdx + dEzdx / kap);
-            }
-            const Slab sy = slab_y(g, i, j, k);
-            if (sy.in) {
-                const unsigned f = sy.lo ? 2 : 3, t = L.tbo[1] + (unsigned)sy.par * 8u;
-                const double kap = ldo(A.tab, t), b = ldo(A.tab, t + L.tbl[1]), c = ldo(A.tab, t + 2 * L.tbl[1]);
-                const double p0 = c * dExdy + b * ldo(A.pho, L.pso[f][0] + sy.io);
-                const double p1 = c * dEzdy + b * ldo(A.pho, L.pso[f][1] + sy.io);
-                if (wpsi) { sto(A.phn, L.pso[f][0] + sy.io, p0); sto(A.phn, L.pso[f][1] + sy.io, p1); }
-                hz = hz + mz * (p0 - dExdy + dExdy / kap);
-                hx = hx - mx * (p1 - dEzdy + dEzdy / kap);
-            }
-            const Slab sz = slab_z(g, i, j, k);
-            if (sz.in) {
-                const unsigned f = sz.lo ? 4 : 5, t = L.tbo[2] + (unsigned)sz.par * 8u;
-                const double kap = ldo(A.tab, t), b = ldo(A.tab, t + L.tbl[2]), c = ldo(A.tab, t + 2 * L.tbl[2]);
-                const double p0 = c * dExdz + b * ldo(A.pho, L.pso[f][0] + sz.io);
-                const double p1 = c * dEydz + b * ldo(A.pho, L.pso[f][1] + sz.io);
-                if (wpsi) { sto(A.phn, L.pso[f][0] + sz.io, p0); sto(A.phn, L.pso[f][1] + sz.io, p1); }
-                hx = hx + mx * (p1 - dEydz + dEydz / kap);
-                hy = hy - my * (p0 - dExdz + dExdz / kap);
-            }
-            if (tile_src) h_sources<MU_ARRAY>(g, sh, A, i, j, k, o, hx, hy, hz);
+            h_compute(g, d, hx, hy, hz, np);
+            if (tsh) h_sources<MU_ARRAY>(g, sh, A, i, j, k, o, hx, hy, hz);
         }
+        // ---------------- exchange H_new with the neighbours
         const int par = i & 1;
         lds[par][row][lane][0] = hx;
         lds[par][row][lane][1] = hz;
         __syncthreads();
         double hz_k = __shfl_up(hz, 1), hy_k = __shfl_up(hy, 1);
+        // ---------------- E_new and all stores (owner only)
         if (doE) {
-            const double sgx = g.bc[0] == 'E' ? -1.0 : (g.bc[0] == 'H' ? 1.0 : 0.0);
-            const double sgy = g.bc[1] == 'E' ? -1.0 : (g.bc[1] == 'H' ? 1.0 : 0.0);
-            const double sgz = g.bc[2] == 'E' ? -1.0 : (g.bc[2] == 'H' ? 1.0 : 0.0);
             double hx_j = lds[par][row - 1][lane][0], hz_j = lds[par][row - 1][lane][1];
             if (j == 0) { hx_j = g.bc[1] == '0' ? 0.0 : sgy * hx; hz_j = g.bc[1] == '0' ? 0.0 : sgy * hz; }
             if (k == 0) { hz_k = g.bc[0] == '0' ? 0.0 : sgx * hz; hy_k = g.bc[0] == '0' ? 0.0 : sgx * hy; }
@@ -563,44 +583,38 @@ __global__ __launch_bounds__(FX * TY) void k_step_lds(FusedArgs A, Geo g, PassSr
             if (i == 0) { hx_m = g.bc[2] == '0' ? 0.0 : sgz * hx; hy_m = g.bc[2] == '0' ? 0.0 : sgz * hy; }
             const double dHzdy = g.ry * (hz - hz_j);
             const double dHydz = g.rz * (hy - hy_m);
-            double nex = CEPS ? ldo(A.aE, o) * ex + (dHzdy - dHydz) * bex : ex + (dHzdy - dHydz) * bex;
+            double nex = CEPS ? aex * d.ex + (dHzdy - dHydz) * bex : d.ex + (dHzdy - dHydz) * bex;
             const double dHxdz = g.rz * (hx - hx_m);
             const double dHzdx = g.rx * (hz - hz_k);
-            double ney = CEPS ? ldo(A.aE, o + cs) * ey + (dHxdz - dHzdx) * bey : ey + (dHxdz - dHzdx) * bey;
+            double ney = CEPS ? aey * d.ey + (dHxdz - dHzdx) * bey : d.ey + (dHxdz - dHzdx) * bey;
             const double dHydx = g.rx * (hy - hy_k);
             const double dHxdy = g.ry * (hx - hx_j);
-            double nez = CEPS ? ldo(A.aE, o + cs2) * ez + (dHydx - dHxdy) * bez : ez + (dHydx - dHxdy) * bez;
-            const Slab sx = slab_x(g, i, j, k);
-            if (sx.in) {
-                const unsigned f = sx.lo ? 0 : 1, t = L.tbo[0] + 3 * L.tbl[0] + (unsigned)sx.par * 8u;
-                const double kap = ldo(A.tab, t), b = ldo(A.tab, t + L.tbl[0]), c = ldo(A.tab, t + 2 * L.tbl[0]);
-                const double q0 = c * dHydx + b * ldo(A.pe, L.pso[f][0] + sx.io);
-                const double q1 = c * dHzdx + b * ldo(A.pe, L.pso[f][1] + sx.io);
-                sto(A.pe, L.pso[f][0] + sx.io, q0); sto(A.pe, L.pso[f][1] + sx.io, q1);
-                nez = nez + (q0 - dHydx + dHydx / kap) * bez;
-                ney = ney - (q1 - dHzdx + dHzdx / kap) * bey;
+            double nez = CEPS ? aez * d.ez + (dHydx - dHxdy) * bez : d.ez + (dHydx - dHxdy) * bez;
+            if (d.sx.in) {
+                const unsigned t = L.tbo[0] + 3 * L.tbl[0] + (unsigned)d.sx.par * 8u;
+                const double kap = ldo(A.tab, t), bb = ldo(A.tab, t + L.tbl[0]), cc = ldo(A.tab, t + 2 * L.tbl[0]);
+                qx0 = cc * dHydx + bb * qx0;
+                qx1 = cc * dHzdx + bb * qx1;
+                nez = nez + (qx0 - dHydx + dHydx / kap) * bez;
+                ney = ney - (qx1 - dHzdx + dHzdx / kap) * bey;
             }
-            const Slab sy = slab_y(g, i, j, k);
-            if (sy.in) {
-                const unsigned f = sy.lo ? 2 : 3, t = L.tbo[1] + 3 * L.tbl[1] + (unsigned)sy.par * 8u;
-                const double kap = ldo(A.tab, t), b = ldo(A.tab, t + L.tbl[1]), c = ldo(A.tab, t + 2 * L.tbl[1]);
-                const double q0 = c * dHxdy + b * ldo(A.pe, L.pso[f][0] + sy.io);
-                const double q1 = c * dHzdy + b * ldo(A.pe, L.pso[f][1] + sy.io);
-                sto(A.pe, L.pso[f][0] + sy.io, q0); sto(A.pe, L.pso[f][1] + sy.io, q1);
-                nez = nez - (q0 - dHxdy + dHxdy / kap) * bez;
-                nex = nex + (q1 - dHzdy + dHzdy / kap) * bex;
+            if (d.sy.in) {
+                const unsigned t = L.tbo[1] + 3 * L.tbl[1] + (unsigned)d.sy.par * 8u;
+                const double kap = ldo(A.tab, t), bb = ldo(A.tab, t + L.tbl[1]), cc = ldo(A.tab, t + 2 * L.tbl[1]);
+                qy0 = cc * dHxdy + bb * qy0;
+                qy1 = cc * dHzdy + bb * qy1;
+                nez = nez - (qy0 - dHxdy + dHxdy / kap) * bez;
+                nex = nex + (qy1 - dHzdy + dHzdy / kap) * bex;
             }
-            const Slab sz = slab_z(g, i, j, k);
-            if (sz.in) {
-                const unsigned f = sz.lo ? 4 : 5, t = L.tbo[2] + 3 * L.tbl[2] + (unsigned)sz.par * 8u;
-                const double kap = ldo(A.tab, t), b = ldo(A.tab, t + L.tbl[2]), c = ldo(A.tab, t + 2 * L.tbl[2]);
-                const double q0 = c * dHxdz + b * ldo(A.pe, L.pso[f][0] + sz.io);
-                const double q1 = c * dHydz + b * ldo(A.pe, L.pso[f][1] + sz.io);
-                sto(A.pe, L.pso[f][0] + sz.io, q0); sto(A.pe, L.pso[f][1] + sz.io, q1);
-                nex = nex - (q1 - dHydz + dHydz / kap) * bex;
-                ney = ney + (q0 - dHxdz + dHxdz / kap) * bey;
+            if (d.sz.in) {
+                const unsigned t = L.tbo[2] + 3 * L.tbl[2] + (unsigned)d.sz.par * 8u;
+                const double kap = ldo(A.tab, t), bb = ldo(A.tab, t + L.tbl[2]), cc = ldo(A.tab, t + 2 * L.tbl[2]);
+                qz0 = cc * dHxdz + bb * qz0;
+                qz1 = cc * dHydz + bb * qz1;
+                nex = nex - (qz1 - dHydz + dHydz / kap) * bex;
+                ney = ney + (qz0 - dHxdz + dHxdz / kap) * bey;
             }
-            if (tile_src) {
+            if (tse) {
                 for (int q = 0; q < se.n; ++q) {
                     long long m;
                     const SrcBlk &sb = se.blk[q];
@@ -614,9 +628,15 @@ __global__ __launch_bounds__(FX * TY) void k_step_lds(FusedArgs A, Geo g, PassSr
             }
             sto(A.Hn, o, hx); sto(A.Hn, o + cs, hy); sto(A.Hn, o + cs2, hz);
             sto(A.En, o, nex); sto(A.En, o + cs, ney); sto(A.En, o + cs2, nez);
+            if (d.sx.in) { sto(A.phn, psof(L, 0, d.sx.lo, 0) + d.sx.io, np[0]); sto(A.phn, psof(L, 0, d.sx.lo, 1) + d.sx.io, np[1]);
+                           sto(A.pe, psof(L, 0, d.sx.lo, 0) + d.sx.io, qx0); sto(A.pe, psof(L, 0, d.sx.lo, 1) + d.sx.io, qx1); }
+            if (d.sy.in) { sto(A.phn, psof(L, 1, d.sy.lo, 0) + d.sy.io, np[2]); sto(A.phn, psof(L, 1, d.sy.lo, 1) + d.sy.io, np[3]);
+                           sto(A.pe, psof(L, 1, d.sy.lo, 0) + d.sy.io, qy0); sto(A.pe, psof(L, 1, d.sy.lo, 1) + d.sy.io, qy1); }
+            if (d.sz.in) { sto(A.phn, psof(L, 2, d.sz.lo, 0) + d.sz.io, np[4]); sto(A.phn, psof(L, 2, d.sz.lo, 1) + d.sz.io, np[5]);
+                           sto(A.pe, psof(L, 2, d.sz.lo, 0) + d.sz.io, qz0); sto(A.pe, psof(L, 2, d.sz.lo, 1) + d.sz.io, qz1); }
         }
         hxm = hx; hym = hy;
-        ex = exi; ey = eyi; ez = ezi;
+        ex = d.exi; ey = d.eyi; ez = ezi;
     }
 }
 

@@ -94,9 +94,9 @@ __device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const
         h_compute(g, d, hx, hy, hz, np);
         if (col_src) h_sources<MU_ARRAY>(g, sh, A, i, j, k, o, hx, hy, hz);
         sto(A.Hn, o, hx); sto(A.Hn, o + cs, hy); sto(A.Hn, o + cs2, hz);
-        if (d.sx.in) { const unsigned f = d.sx.lo ? 0 : 1; sto(A.phn, L.pso[f][0] + d.sx.io, np[0]); sto(A.phn, L.pso[f][1] + d.sx.io, np[1]); }
-        if (d.sy.in) { const unsigned f = d.sy.lo ? 2 : 3; sto(A.phn, L.pso[f][0] + d.sy.io, np[2]); sto(A.phn, L.pso[f][1] + d.sy.io, np[3]); }
-        if (d.sz.in) { const unsigned f = d.sz.lo ? 4 : 5; sto(A.phn, L.pso[f][0] + d.sz.io, np[4]); sto(A.phn, L.pso[f][1] + d.sz.io, np[5]); }
+        if (d.sx.in) { sto(A.phn, psof(L, 0, d.sx.lo, 0) + d.sx.io, np[0]); sto(A.phn, psof(L, 0, d.sx.lo, 1) + d.sx.io, np[1]); }
+        if (d.sy.in) { sto(A.phn, psof(L, 1, d.sy.lo, 0) + d.sy.io, np[2]); sto(A.phn, psof(L, 1, d.sy.lo, 1) + d.sy.io, np[3]); }
+        if (d.sz.in) { sto(A.phn, psof(L, 2, d.sz.lo, 0) + d.sz.io, np[4]); sto(A.phn, psof(L, 2, d.sz.lo, 1) + d.sz.io, np[5]); }
         ex = d.exi; ey = d.eyi;
     }
 }
@@ -116,20 +116,6 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_h(FusedArgs A, Geo g, Pas
 }
 
 // ------------------------------------------------------------------ E pass
-// Coefficient modes of the E pass: real eps with full b arrays, complex eps
-// (a and b arrays), real eps with b gathered through a per-cell index.
-enum { CM_REAL = 0, CM_COMPLEX = 1, CM_INDEX = 2 };
-
-// Entry of the per-cell coefficient index at byte offset o of a field
-// component: one aligned dword holds the 2- or 4-byte entry (the 2-byte index
-// is allocated with two spare entries for the last dword).
-__device__ __forceinline__ unsigned coef_index(const FusedArgs &A, unsigned o)
-{
-    const unsigned e = o >> 3, w2 = A.bIw == 2;
-    const unsigned v = *(const unsigned *)((const char *)A.bI + (w2 ? (e << 1) & ~3u : e << 2));
-    return w2 ? (v >> ((e & 1u) << 4)) & 0xffffu : v;
-}
-
 // One cell (i, j, k) at byte offset o; hxm/hym hold H_x, H_y at i-1 and are
 // advanced to plane i.  With CM_INDEX, ci is the cell's coefficient index,
 // loaded by the caller ahead of the plane's other loads: the table gather is
@@ -160,21 +146,20 @@ __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const 
     double ax = 1, ay = 1, az = 1;
     if (CEPS) { ax = ldo(A.aE, o); ay = ldo(A.aE, o + cs); az = ldo(A.aE, o + cs2); }
     const Slab sx = slab_x(g, i, j, k), sy = slab_y(g, i, j, k), sz = slab_z(g, i, j, k);
-    const unsigned fx = sx.lo ? 0 : 1, fy = sy.lo ? 2 : 3, fz = sz.lo ? 4 : 5;
     double qx0 = 0, qx1 = 0, qy0 = 0, qy1 = 0, qz0 = 0, qz1 = 0;
     double kx = 1, tbx = 0, tcx = 0, ky = 1, tby = 0, tcy = 0, kz = 1, tbz = 0, tcz = 0;
     if (sx.in) {
-        qx0 = ldo(A.pe, L.pso[fx][0] + sx.io); qx1 = ldo(A.pe, L.pso[fx][1] + sx.io);
+        qx0 = ldo(A.pe, psof(L, 0, sx.lo, 0) + sx.io); qx1 = ldo(A.pe, psof(L, 0, sx.lo, 1) + sx.io);
         const unsigned t = L.tbo[0] + 3 * L.tbl[0] + (unsigned)sx.par * 8u;
         kx = ldo(A.tab, t); tbx = ldo(A.tab, t + L.tbl[0]); tcx = ldo(A.tab, t + 2 * L.tbl[0]);
     }
     if (sy.in) {
-        qy0 = ldo(A.pe, L.pso[fy][0] + sy.io); qy1 = ldo(A.pe, L.pso[fy][1] + sy.io);
+        qy0 = ldo(A.pe, psof(L, 1, sy.lo, 0) + sy.io); qy1 = ldo(A.pe, psof(L, 1, sy.lo, 1) + sy.io);
         const unsigned t = L.tbo[1] + 3 * L.tbl[1] + (unsigned)sy.par * 8u;
         ky = ldo(A.tab, t); tby = ldo(A.tab, t + L.tbl[1]); tcy = ldo(A.tab, t + 2 * L.tbl[1]);
     }
     if (sz.in) {
-        qz0 = ldo(A.pe, L.pso[fz][0] + sz.io); qz1 = ldo(A.pe, L.pso[fz][1] + sz.io);
+        qz0 = ldo(A.pe, psof(L, 2, sz.lo, 0) + sz.io); qz1 = ldo(A.pe, psof(L, 2, sz.lo, 1) + sz.io);
         const unsigned t = L.tbo[2] + 3 * L.tbl[2] + (unsigned)sz.par * 8u;
         kz = ldo(A.tab, t); tbz = ldo(A.tab, t + L.tbl[2]); tcz = ldo(A.tab, t + 2 * L.tbl[2]);
     }
@@ -223,9 +208,9 @@ __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const 
         }
     }
     sto(A.En, o, nex); sto(A.En, o + cs, ney); sto(A.En, o + cs2, nez);
-    if (sx.in) { sto(A.pe, L.pso[fx][0] + sx.io, qx0); sto(A.pe, L.pso[fx][1] + sx.io, qx1); }
-    if (sy.in) { sto(A.pe, L.pso[fy][0] + sy.io, qy0); sto(A.pe, L.pso[fy][1] + sy.io, qy1); }
-    if (sz.in) { sto(A.pe, L.pso[fz][0] + sz.io, qz0); sto(A.pe, L.pso[fz][1] + sz.io, qz1); }
+    if (sx.in) { sto(A.pe, psof(L, 0, sx.lo, 0) + sx.io, qx0); sto(A.pe, psof(L, 0, sx.lo, 1) + sx.io, qx1); }
+    if (sy.in) { sto(A.pe, psof(L, 1, sy.lo, 0) + sy.io, qy0); sto(A.pe, psof(L, 1, sy.lo, 1) + sy.io, qy1); }
+    if (sz.in) { sto(A.pe, psof(L, 2, sz.lo, 0) + sz.io, qz0); sto(A.pe, psof(L, 2, sz.lo, 1) + sz.io, qz1); }
     hxm = hx; hym = hy;
 }
 

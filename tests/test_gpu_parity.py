@@ -224,14 +224,15 @@ def test_coefficient_compression_is_exact(which):
     """The E pass reading a per-cell index into the table of distinct eps
     triples gives exactly the fields of the full coefficient arrays: few
     distinct values (strip, 2-byte index), every cell distinct (random,
-    4-byte index) and complex eps (compression not applicable)."""
+    4-byte index) and complex eps (compression not applicable); for the two
+    passes and for the fused LDS step."""
     if which == "strip":
         case = strip_case(40, 28, 22, w=5)
     else:
         case = random_case(70, 37, 23, w=(5, 4, 3, 6, 4, 5), seed=31, nsrc=2, mu_rand=False,
                            ceps=(which == "random_ceps"))
-    for kind in (1, 3):
-        a = _run_steps(case, 41, kind=kind, tyg=4, zc=2, compress=1)
-        b = _run_steps(case, 41, kind=kind, tyg=4, zc=2, compress=0)
+    for kind, tyg, zc in ((1, 4, 2), (3, 4, 2), (2, 8, 5)):
+        a = _run_steps(case, 41, kind=kind, tyg=tyg, zc=zc, compress=1)
+        b = _run_steps(case, 41, kind=kind, tyg=tyg, zc=zc, compress=0)
         for x, y in zip(a, b):
             assert np.array_equal(x, y), (which, kind)
