@@ -212,7 +212,7 @@ struct emf_solver {
     int cidx_bytes = 0;            // width of the index in use (0: off)
     void *cidx = nullptr;          // device index, n entries
     double *ctabE = nullptr;       // device table of eps triples
-    double *ctabB = nullptr;       // device table of b = dt/eps triples
+    double *ctabB = nullptr;       // device table of b = dt/eps triples (4 doubles each)
     long long ctab_n = 0;
 
     Geo geo() const
@@ -359,8 +359,8 @@ int refresh_coeffs(emf_solver *s)
     }
     s->cidx_bytes = 0;
     if (s->compress && s->cidx_valid && !s->ceps && s->ctab_n > 0) {
-        hipLaunchKernelGGL(k_b_table, dim3(grid1d(3 * s->ctab_n, 256)), dim3(256), 0, s->stream,
-                           s->ctabE, s->ctabB, 3 * s->ctab_n, s->dt);
+        hipLaunchKernelGGL(k_b_table, dim3(grid1d(4 * s->ctab_n, 256)), dim3(256), 0, s->stream,
+                           s->ctabE, s->ctabB, s->ctab_n, s->dt);
         s->cidx_bytes = s->ctab_n <= 65535 ? 2 : 4;
     }
     HIPCHK(hipGetLastError());
@@ -415,7 +415,7 @@ int build_coeff_index(emf_solver *s, const double *ex, const double *ey, const d
     }
     const long long nt = (long long)map.size();
     CHECK(dalloc(&s->ctabE, 3 * (size_t)nt));
-    CHECK(dalloc(&s->ctabB, 3 * (size_t)nt));
+    CHECK(dalloc(&s->ctabB, 4 * (size_t)nt));
     HIPCHK(hipMemcpyAsync(s->ctabE, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice,
                           s->stream));
     if (nt <= 65535) {
@@ -677,12 +677,14 @@ void launch_e_zu(const emf_solver *s, const FusedArgs &A, const Geo &g, const Pa
     }
 }
 
-// columns of at most 2 planes take the unrolled E pass (indices loaded first)
+// columns of at most 8 planes take the unrolled E pass (indices loaded first)
 template <int CM, int OCC>
 void launch_e_occ(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
                   const int *done)
 {
     if (CM == CM_INDEX && s->zc <= 2) launch_e_zu<CM, OCC, 2>(s, A, g, ps, done);
+    else if (CM == CM_INDEX && s->zc <= 4) launch_e_zu<CM, OCC, 4>(s, A, g, ps, done);
+    else if (CM == CM_INDEX && s->zc <= 8) launch_e_zu<CM, OCC, 8>(s, A, g, ps, done);
     else launch_e_zu<CM, OCC, 0>(s, A, g, ps, done);
 }
 

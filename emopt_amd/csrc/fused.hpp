@@ -43,7 +43,7 @@ struct FusedArgs {
     unsigned cs;             // byte stride between components
     double m_s, mu_s;        // uniform dt/mu.re and mu.re
     const void *bI;          // real eps only, optional: per-cell index into bT
-    const double *bT;        // (b_x, b_y, b_z) per distinct eps triple
+    const double *bT;        // (b_x, b_y, b_z, 0) per distinct eps triple
     int bIw;                 // index width in bytes (2 or 4)
 };
 
@@ -59,6 +59,16 @@ __device__ __forceinline__ unsigned coef_index(const FusedArgs &A, unsigned o)
     const unsigned e = o >> 3, w2 = A.bIw == 2;
     const unsigned v = *(const unsigned *)((const char *)A.bI + (w2 ? (e << 1) & ~3u : e << 2));
     return w2 ? (v >> ((e & 1u) << 4)) & 0xffffu : v;
+}
+
+// (b_x, b_y, b_z) of table entry ci: 32-byte entries, one 16-byte and one
+// 8-byte load
+__device__ __forceinline__ void coef_gather(const FusedArgs &A, unsigned ci, double &bx,
+                                            double &by, double &bz)
+{
+    const double2 xy = *reinterpret_cast<const double2 *>(A.bT + 4 * ci);
+    bx = xy.x; by = xy.y;
+    bz = A.bT[4 * ci + 2];
 }
 
 // Byte offsets of the psi slab arrays of a set ([face][component], faces
@@ -551,7 +561,7 @@ __global__ __launch_bounds__(FX * TY) void k_step_lds(FusedArgs A, Geo g, PassSr
             if (d.sy.in) { qy0 = ldo(A.pe, psof(L, 1, d.sy.lo, 0) + d.sy.io); qy1 = ldo(A.pe, psof(L, 1, d.sy.lo, 1) + d.sy.io); }
             if (d.sz.in) { qz0 = ldo(A.pe, psof(L, 2, d.sz.lo, 0) + d.sz.io); qz1 = ldo(A.pe, psof(L, 2, d.sz.lo, 1) + d.sz.io); }
         }
-        if (CM == CM_INDEX) { bex = A.bT[3 * ci]; bey = A.bT[3 * ci + 1]; bez = A.bT[3 * ci + 2]; }
+        if (CM == CM_INDEX) coef_gather(A, ci, bex, bey, bez);
         // ---------------- H_new(i, j, k)
         if (!up) { d.exi = d.eyi = ezi = 0; }
         if (!jn) { d.ezj = d.exj = 0; }

@@ -306,13 +306,19 @@ __global__ void k_eps_coeffs(const double2 *__restrict__ eps, double *__restrict
     }
 }
 
-// b = dt/eps for a table of distinct real eps values (the same division as
-// k_eps_coeffs, so the same doubles)
-__global__ void k_b_table(const double *__restrict__ eps, double *__restrict__ b, long long n,
+// b = dt/eps for a table of nt distinct real eps triples (the same division
+// as k_eps_coeffs, so the same doubles).  eps holds 3 values per triple, b 4
+// (b_x, b_y, b_z, 0): a 32-byte entry is gathered with one 16-byte and one
+// 8-byte load.
+__global__ void k_b_table(const double *__restrict__ eps, double *__restrict__ b, long long nt,
                           double dt)
 {
     long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    for (; g < n; g += (long long)gridDim.x * blockDim.x) b[g] = dt / eps[g];
+    for (; g < 4 * nt; g += (long long)gridDim.x * blockDim.x) {
+        const long long t = g >> 2;
+        const int c = (int)(g & 3);
+        b[g] = c < 3 ? dt / eps[3 * t + c] : 0.0;
+    }
 }
 
 // dt/mu.re and mu.re from raw complex mu

@@ -81,9 +81,15 @@ __device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const
         if (i + 1 < Nz) { d.exi = ldo(A.Eo, o + oz); d.eyi = ldo(A.Eo, o + oz + cs); }
         d.ez = ldo(A.Eo, o + cs2);
         if (jn) { d.ezj = ldo(A.Eo, o + oy + cs2); d.exj = ldo(A.Eo, o + oy); }
-        if (kn) { d.ezk = ldo(A.Eo, o + 8u + cs2); d.eyk = ldo(A.Eo, o + 8u + cs); }
         d.hx = ldo(A.Ho, o); d.hy = ldo(A.Ho, o + cs); d.hz = ldo(A.Ho, o + cs2);
+        // E(k+1) comes from the next lane; the last lane loads it
+        double ezk = 0, eyk = 0;
+        if (threadIdx.x == PX - 1 && kn) { ezk = ldo(A.Eo, o + 8u + cs2); eyk = ldo(A.Eo, o + 8u + cs); }
         h_load_aux<MU_ARRAY>(g, L, A, i, j, k, o, d);
+        d.ezk = __shfl_down(d.ez, 1);
+        d.eyk = __shfl_down(d.ey, 1);
+        if (threadIdx.x == PX - 1) { d.ezk = ezk; d.eyk = eyk; }
+        if (!kn) d.ezk = d.eyk = 0;   // zero ghost past the max face
         if (cap) {
             const long long gc = (long long)(o >> 3);
             cap[2 * gc] = d.ex; cap[2 * (capn + gc)] = d.ey; cap[2 * (2 * capn + gc)] = d.ez;
@@ -134,12 +140,11 @@ __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const 
     constexpr bool CEPS = CM == CM_COMPLEX;
     // ---- all loads of this plane
     const double hx = ldo(A.Ho, o), hy = ldo(A.Ho, o + cs), hz = ldo(A.Ho, o + cs2);
-    // row j-1 and column k-1; at j == 0 / k == 0 the own cell stands in
-    // (unconditional loads keep the count of loads behind the index fixed)
-    // and the ghost rule below replaces it
-    const unsigned dj = j > 0 ? oy : 0u, dk = k > 0 ? 8u : 0u;
+    // row j-1; at j == 0 the own cell stands in (an unconditional load keeps
+    // the count of loads behind the index fixed) and the ghost rule below
+    // replaces it
+    const unsigned dj = j > 0 ? oy : 0u;
     double hz_j = ldo(A.Ho, o - dj + cs2), hx_j = ldo(A.Ho, o - dj);
-    double hz_k = ldo(A.Ho, o - dk + cs2), hy_k = ldo(A.Ho, o - dk + cs);
     const double ex = ldo(A.Eo, o), ey = ldo(A.Eo, o + cs), ez = ldo(A.Eo, o + cs2);
     double bx = 0, by = 0, bz = 0;
     if (CM != CM_INDEX) { bx = ldo(A.bE, o); by = ldo(A.bE, o + cs); bz = ldo(A.bE, o + cs2); }
@@ -163,7 +168,12 @@ __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const 
         const unsigned t = L.tbo[2] + 3 * L.tbl[2] + (unsigned)sz.par * 8u;
         kz = ldo(A.tab, t); tbz = ldo(A.tab, t + L.tbl[2]); tcz = ldo(A.tab, t + 2 * L.tbl[2]);
     }
-    if (CM == CM_INDEX) { bx = A.bT[3 * ci]; by = A.bT[3 * ci + 1]; bz = A.bT[3 * ci + 2]; }
+    // H(k-1) comes from the previous lane; lane 0 loads it
+    double hz_k0 = 0, hy_k0 = 0;
+    if (threadIdx.x == 0 && k > 0) { hz_k0 = ldo(A.Ho, o - 8u + cs2); hy_k0 = ldo(A.Ho, o - 8u + cs); }
+    if (CM == CM_INDEX) coef_gather(A, ci, bx, by, bz);
+    double hz_k = __shfl_up(hz, 1), hy_k = __shfl_up(hy, 1);
+    if (threadIdx.x == 0) { hz_k = hz_k0; hy_k = hy_k0; }
     // ---- boundary ghosts at the min faces
     if (j == 0) { hz_j = by0 ? 0.0 : sgy * hz; hx_j = by0 ? 0.0 : sgy * hx; }
     if (k == 0) { hz_k = bx0 ? 0.0 : sgx * hz; hy_k = bx0 ? 0.0 : sgx * hy; }
@@ -259,7 +269,7 @@ __device__ __forceinline__ void e_column(const FusedArgs &A, const Geo &g, const
     }
 }
 
-// ZU: see e_column (the launch picks ZU = 2 when zc <= 2)
+// ZU: see e_column (the launch picks the smallest ZU of 2, 4, 8 that holds zc)
 template <int CM, int TY, int OCC = 4, int ZU = 0>
 __global__ __launch_bounds__(PX * TY, OCC) void k_pass_e(FusedArgs A, Geo g, PassSrc se,
                                                         const int *__restrict__ done)
