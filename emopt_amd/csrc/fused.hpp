@@ -119,6 +119,16 @@ __device__ __forceinline__ double ldo(const double *base, unsigned off)
     return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + off);
 }
 
+// Load from data that no kernel writes while a step runs (the CPML tables),
+// through the constant address space: where the offset is the same for the
+// whole wave the backend then loads it with a scalar load, which keeps it off
+// the vector memory path.
+__device__ __forceinline__ double ldc(const double *base, unsigned off)
+{
+    typedef const __attribute__((address_space(4))) double cdouble;
+    return *(cdouble *)(reinterpret_cast<const char *>(base) + off);
+}
+
 __device__ __forceinline__ void sto(double *base, unsigned off, double v)
 {
     *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + off) = v;
@@ -199,19 +209,19 @@ __device__ __forceinline__ void h_load_aux(const Geo &g, const Layout &L, const 
         d.px0 = ldo(A.pho, psof(L, 0, d.sx.lo, 0) + d.sx.io);
         d.px1 = ldo(A.pho, psof(L, 0, d.sx.lo, 1) + d.sx.io);
         const unsigned t = L.tbo[0] + (unsigned)d.sx.par * 8u;
-        d.kx = ldo(A.tab, t); d.bx = ldo(A.tab, t + L.tbl[0]); d.cx = ldo(A.tab, t + 2 * L.tbl[0]);
+        d.kx = ldc(A.tab, t); d.bx = ldc(A.tab, t + L.tbl[0]); d.cx = ldc(A.tab, t + 2 * L.tbl[0]);
     }
     if (d.sy.in) {
         d.py0 = ldo(A.pho, psof(L, 1, d.sy.lo, 0) + d.sy.io);
         d.py1 = ldo(A.pho, psof(L, 1, d.sy.lo, 1) + d.sy.io);
         const unsigned t = L.tbo[1] + (unsigned)d.sy.par * 8u;
-        d.ky = ldo(A.tab, t); d.by = ldo(A.tab, t + L.tbl[1]); d.cy = ldo(A.tab, t + 2 * L.tbl[1]);
+        d.ky = ldc(A.tab, t); d.by = ldc(A.tab, t + L.tbl[1]); d.cy = ldc(A.tab, t + 2 * L.tbl[1]);
     }
     if (d.sz.in) {
         d.pz0 = ldo(A.pho, psof(L, 2, d.sz.lo, 0) + d.sz.io);
         d.pz1 = ldo(A.pho, psof(L, 2, d.sz.lo, 1) + d.sz.io);
         const unsigned t = L.tbo[2] + (unsigned)d.sz.par * 8u;
-        d.kz = ldo(A.tab, t); d.bz = ldo(A.tab, t + L.tbl[2]); d.cz = ldo(A.tab, t + 2 * L.tbl[2]);
+        d.kz = ldc(A.tab, t); d.bz = ldc(A.tab, t + L.tbl[2]); d.cz = ldc(A.tab, t + 2 * L.tbl[2]);
     }
 }
 
@@ -412,7 +422,7 @@ __global__ __launch_bounds__(FX * TY) void k_step_fused(FusedArgs A, Geo g, Pass
             else nez = ez + (dHydx - dHxdy) * bez;
             if (sx.in) {
                 const unsigned t = L.tbo[0] + 3 * L.tbl[0] + (unsigned)sx.par * 8u;
-                const double kap = ldo(A.tab, t), bb = ldo(A.tab, t + L.tbl[0]), cc = ldo(A.tab, t + 2 * L.tbl[0]);
+                const double kap = ldc(A.tab, t), bb = ldc(A.tab, t + L.tbl[0]), cc = ldc(A.tab, t + 2 * L.tbl[0]);
                 qx0 = cc * dHydx + bb * qx0;
                 qx1 = cc * dHzdx + bb * qx1;
                 nez = nez + (qx0 - dHydx + dHydx / kap) * bez;
@@ -420,7 +430,7 @@ __global__ __launch_bounds__(FX * TY) void k_step_fused(FusedArgs A, Geo g, Pass
             }
             if (sy.in) {
                 const unsigned t = L.tbo[1] + 3 * L.tbl[1] + (unsigned)sy.par * 8u;
-                const double kap = ldo(A.tab, t), bb = ldo(A.tab, t + L.tbl[1]), cc = ldo(A.tab, t + 2 * L.tbl[1]);
+                const double kap = ldc(A.tab, t), bb = ldc(A.tab, t + L.tbl[1]), cc = ldc(A.tab, t + 2 * L.tbl[1]);
                 qy0 = cc * dHxdy + bb * qy0;
                 qy1 = cc * dHzdy + bb * qy1;
                 nez = nez - (qy0 - dHxdy + dHxdy / kap) * bez;
@@ -428,7 +438,7 @@ __global__ __launch_bounds__(FX * TY) void k_step_fused(FusedArgs A, Geo g, Pass
             }
             if (sz.in) {
                 const unsigned t = L.tbo[2] + 3 * L.tbl[2] + (unsigned)sz.par * 8u;
-                const double kap = ldo(A.tab, t), bb = ldo(A.tab, t + L.tbl[2]), cc = ldo(A.tab, t + 2 * L.tbl[2]);
+                const double kap = ldc(A.tab, t), bb = ldc(A.tab, t + L.tbl[2]), cc = ldc(A.tab, t + 2 * L.tbl[2]);
                 qz0 = cc * dHxdz + bb * qz0;
                 qz1 = cc * dHydz + bb * qz1;
                 nex = nex - (qz1 - dHydz + dHydz / kap) * bex;
@@ -602,7 +612,7 @@ __global__ __launch_bounds__(FX * TY) void k_step_lds(FusedArgs A, Geo g, PassSr
             double nez = CEPS ? aez * d.ez + (dHydx - dHxdy) * bez : d.ez + (dHydx - dHxdy) * bez;
             if (d.sx.in) {
                 const unsigned t = L.tbo[0] + 3 * L.tbl[0] + (unsigned)d.sx.par * 8u;
-                const double kap = ldo(A.tab, t), bb = ldo(A.tab, t + L.tbl[0]), cc = ldo(A.tab, t + 2 * L.tbl[0]);
+                const double kap = ldc(A.tab, t), bb = ldc(A.tab, t + L.tbl[0]), cc = ldc(A.tab, t + 2 * L.tbl[0]);
                 qx0 = cc * dHydx + bb * qx0;
                 qx1 = cc * dHzdx + bb * qx1;
                 nez = nez + (qx0 - dHydx + dHydx / kap) * bez;
@@ -610,7 +620,7 @@ __global__ __launch_bounds__(FX * TY) void k_step_lds(FusedArgs A, Geo g, PassSr
             }
             if (d.sy.in) {
                 const unsigned t = L.tbo[1] + 3 * L.tbl[1] + (unsigned)d.sy.par * 8u;
-                const double kap = ldo(A.tab, t), bb = ldo(A.tab, t + L.tbl[1]), cc = ldo(A.tab, t + 2 * L.tbl[1]);
+                const double kap = ldc(A.tab, t), bb = ldc(A.tab, t + L.tbl[1]), cc = ldc(A.tab, t + 2 * L.tbl[1]);
                 qy0 = cc * dHxdy + bb * qy0;
                 qy1 = cc * dHzdy + bb * qy1;
                 nez = nez - (qy0 - dHxdy + dHxdy / kap) * bez;
@@ -618,7 +628,7 @@ __global__ __launch_bounds__(FX * TY) void k_step_lds(FusedArgs A, Geo g, PassSr
             }
             if (d.sz.in) {
                 const unsigned t = L.tbo[2] + 3 * L.tbl[2] + (unsigned)d.sz.par * 8u;
-                const double kap = ldo(A.tab, t), bb = ldo(A.tab, t + L.tbl[2]), cc = ldo(A.tab, t + 2 * L.tbl[2]);
+                const double kap = ldc(A.tab, t), bb = ldc(A.tab, t + L.tbl[2]), cc = ldc(A.tab, t + 2 * L.tbl[2]);
                 qz0 = cc * dHxdz + bb * qz0;
                 qz1 = cc * dHydz + bb * qz1;
                 nex = nex - (qz1 - dHydz + dHydz / kap) * bex;

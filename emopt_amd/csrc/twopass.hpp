@@ -117,6 +117,7 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_h(FusedArgs A, Geo g, Pas
     if (done && *done) return;
     int k, j, z0, z1;
     pass_tile(g, TY, k, j, z0, z1);
+    j = __builtin_amdgcn_readfirstlane(j);   // one row per wave
     if (k >= g.Nx || j >= g.Ny) return;
     h_column<MU_ARRAY>(A, g, sh, k, j, z0, z1, nullptr, 0);
 }
@@ -156,17 +157,17 @@ __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const 
     if (sx.in) {
         qx0 = ldo(A.pe, psof(L, 0, sx.lo, 0) + sx.io); qx1 = ldo(A.pe, psof(L, 0, sx.lo, 1) + sx.io);
         const unsigned t = L.tbo[0] + 3 * L.tbl[0] + (unsigned)sx.par * 8u;
-        kx = ldo(A.tab, t); tbx = ldo(A.tab, t + L.tbl[0]); tcx = ldo(A.tab, t + 2 * L.tbl[0]);
+        kx = ldc(A.tab, t); tbx = ldc(A.tab, t + L.tbl[0]); tcx = ldc(A.tab, t + 2 * L.tbl[0]);
     }
     if (sy.in) {
         qy0 = ldo(A.pe, psof(L, 1, sy.lo, 0) + sy.io); qy1 = ldo(A.pe, psof(L, 1, sy.lo, 1) + sy.io);
         const unsigned t = L.tbo[1] + 3 * L.tbl[1] + (unsigned)sy.par * 8u;
-        ky = ldo(A.tab, t); tby = ldo(A.tab, t + L.tbl[1]); tcy = ldo(A.tab, t + 2 * L.tbl[1]);
+        ky = ldc(A.tab, t); tby = ldc(A.tab, t + L.tbl[1]); tcy = ldc(A.tab, t + 2 * L.tbl[1]);
     }
     if (sz.in) {
         qz0 = ldo(A.pe, psof(L, 2, sz.lo, 0) + sz.io); qz1 = ldo(A.pe, psof(L, 2, sz.lo, 1) + sz.io);
         const unsigned t = L.tbo[2] + 3 * L.tbl[2] + (unsigned)sz.par * 8u;
-        kz = ldo(A.tab, t); tbz = ldo(A.tab, t + L.tbl[2]); tcz = ldo(A.tab, t + 2 * L.tbl[2]);
+        kz = ldc(A.tab, t); tbz = ldc(A.tab, t + L.tbl[2]); tcz = ldc(A.tab, t + 2 * L.tbl[2]);
     }
     // H(k-1) comes from the previous lane; lane 0 loads it
     double hz_k0 = 0, hy_k0 = 0;
@@ -277,6 +278,7 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_e(FusedArgs A, Geo g, Pas
     if (done && *done) return;
     int k, j, z0, z1;
     pass_tile(g, TY, k, j, z0, z1, true);
+    j = __builtin_amdgcn_readfirstlane(j);   // one row per wave
     if (k >= g.Nx || j >= g.Ny) return;
     e_column<CM, ZU>(A, g, se, k, j, z0, z1);
 }
