@@ -110,7 +110,7 @@ __device__ __forceinline__ bool batch_column(const BMember &B, int lt, int &k, i
 {
     const int bx = lt % B.ntx, by = (lt / B.ntx) % B.nty, bz = lt / (B.ntx * B.nty);
     k = bx * PX + threadIdx.x;
-    j = by * BTY + threadIdx.y;
+    j = __builtin_amdgcn_readfirstlane(by * BTY + (int)threadIdx.y);   // one row per wave
     z0 = bz * B.g.zc;
     z1 = min(z0 + B.g.zc, B.g.Nz);
     return k < B.g.Nx && j < B.g.Ny;
@@ -230,7 +230,8 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_e(const BTable *__restrict__
         PassSrc ps;
         ps.n = B.nsrc; ps.comp0 = 0; ps.blk = B.blk;
         ps.st = f[3]; ps.ct = f[4]; ps.env = f[5];
-        if (B.A.bI) e_column<CM_INDEX, 0>(B.A, B.g, ps, k, j, z0, z1);
+        if (B.A.bI && z1 - z0 <= 2) e_column<CM_INDEX, 2>(B.A, B.g, ps, k, j, z0, z1);
+        else if (B.A.bI) e_column<CM_INDEX, 0>(B.A, B.g, ps, k, j, z0, z1);
         else e_column<CM_REAL, 0>(B.A, B.g, ps, k, j, z0, z1);
     }
 }
