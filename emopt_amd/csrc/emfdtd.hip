@@ -149,6 +149,7 @@ struct emf_solver {
     int nconv = 10;
     int zc = 2;
     int order = 3;                 // two-pass work-group order (pass_tile)
+    int psi_stream = -1;           // streaming psi access: -1 by size, 0 off, 1 on
 
     // device state.  E/H (and the H-pass psi set) exist twice for the fused
     // step, which reads one set and writes the other; E, H and psi[0] always
@@ -639,6 +640,18 @@ FusedArgs make_args(const emf_solver *s, int from, int to)
     return A;
 }
 
+// Whether the passes stream the psi arrays past the caches (non-temporal):
+// when a step's data -- fields, coefficients and psi -- exceeds the 256 MB
+// Infinity Cache, nothing of it survives to the next step and psi lines would
+// only displace the fields the next pass re-reads; a smaller solve keeps its
+// psi cached from step to step.  emf_set_streaming overrides.
+inline bool stream_psi(const emf_solver *s)
+{
+    if (s->psi_stream >= 0) return s->psi_stream == 1;
+    const double bytes = (double)s->n * 100.0 + 2.0 * 2.0 * 8.0 * (double)s->psi_tot;
+    return bytes > 256e6;
+}
+
 // rows per work-group of the two passes (tyg of a fused kind may be 16)
 int pass_ty(const emf_solver *s)
 {
@@ -651,41 +664,41 @@ dim3 pass_grid(const emf_solver *s)
     return dim3((s->Nx + PX - 1) / PX, (s->Ny + ty - 1) / ty, (s->Nz + s->zc - 1) / s->zc);
 }
 
-template <bool MU, int OCC>
+template <bool MU, int OCC, bool NT>
 void launch_h_occ(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
                   const int *done)
 {
     const dim3 gr = pass_grid(s);
     switch (pass_ty(s)) {
-    case 1: hipLaunchKernelGGL((k_pass_h<MU, 1, OCC>), gr, dim3(PX, 1), 0, s->stream, A, g, ps, done); break;
-    case 2: hipLaunchKernelGGL((k_pass_h<MU, 2, OCC>), gr, dim3(PX, 2), 0, s->stream, A, g, ps, done); break;
-    case 8: hipLaunchKernelGGL((k_pass_h<MU, 8, OCC>), gr, dim3(PX, 8), 0, s->stream, A, g, ps, done); break;
-    default: hipLaunchKernelGGL((k_pass_h<MU, 4, OCC>), gr, dim3(PX, 4), 0, s->stream, A, g, ps, done);
+    case 1: hipLaunchKernelGGL((k_pass_h<MU, 1, OCC, NT>), gr, dim3(PX, 1), 0, s->stream, A, g, ps, done); break;
+    case 2: hipLaunchKernelGGL((k_pass_h<MU, 2, OCC, NT>), gr, dim3(PX, 2), 0, s->stream, A, g, ps, done); break;
+    case 8: hipLaunchKernelGGL((k_pass_h<MU, 8, OCC, NT>), gr, dim3(PX, 8), 0, s->stream, A, g, ps, done); break;
+    default: hipLaunchKernelGGL((k_pass_h<MU, 4, OCC, NT>), gr, dim3(PX, 4), 0, s->stream, A, g, ps, done);
     }
 }
 
-template <int CM, int OCC, int ZU>
+template <int CM, int OCC, int ZU, bool NT>
 void launch_e_zu(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
                  const int *done)
 {
     const dim3 gr = pass_grid(s);
     switch (pass_ty(s)) {
-    case 1: hipLaunchKernelGGL((k_pass_e<CM, 1, OCC, ZU>), gr, dim3(PX, 1), 0, s->stream, A, g, ps, done); break;
-    case 2: hipLaunchKernelGGL((k_pass_e<CM, 2, OCC, ZU>), gr, dim3(PX, 2), 0, s->stream, A, g, ps, done); break;
-    case 8: hipLaunchKernelGGL((k_pass_e<CM, 8, OCC, ZU>), gr, dim3(PX, 8), 0, s->stream, A, g, ps, done); break;
-    default: hipLaunchKernelGGL((k_pass_e<CM, 4, OCC, ZU>), gr, dim3(PX, 4), 0, s->stream, A, g, ps, done);
+    case 1: hipLaunchKernelGGL((k_pass_e<CM, 1, OCC, ZU, NT>), gr, dim3(PX, 1), 0, s->stream, A, g, ps, done); break;
+    case 2: hipLaunchKernelGGL((k_pass_e<CM, 2, OCC, ZU, NT>), gr, dim3(PX, 2), 0, s->stream, A, g, ps, done); break;
+    case 8: hipLaunchKernelGGL((k_pass_e<CM, 8, OCC, ZU, NT>), gr, dim3(PX, 8), 0, s->stream, A, g, ps, done); break;
+    default: hipLaunchKernelGGL((k_pass_e<CM, 4, OCC, ZU, NT>), gr, dim3(PX, 4), 0, s->stream, A, g, ps, done);
     }
 }
 
 // columns of at most 8 planes take the unrolled E pass (indices loaded first)
-template <int CM, int OCC>
+template <int CM, int OCC, bool NT>
 void launch_e_occ(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
                   const int *done)
 {
-    if (CM == CM_INDEX && s->zc <= 2) launch_e_zu<CM, OCC, 2>(s, A, g, ps, done);
-    else if (CM == CM_INDEX && s->zc <= 4) launch_e_zu<CM, OCC, 4>(s, A, g, ps, done);
-    else if (CM == CM_INDEX && s->zc <= 8) launch_e_zu<CM, OCC, 8>(s, A, g, ps, done);
-    else launch_e_zu<CM, OCC, 0>(s, A, g, ps, done);
+    if (CM == CM_INDEX && s->zc <= 2) launch_e_zu<CM, OCC, 2, NT>(s, A, g, ps, done);
+    else if (CM == CM_INDEX && s->zc <= 4) launch_e_zu<CM, OCC, 4, NT>(s, A, g, ps, done);
+    else if (CM == CM_INDEX && s->zc <= 8) launch_e_zu<CM, OCC, 8, NT>(s, A, g, ps, done);
+    else launch_e_zu<CM, OCC, 0, NT>(s, A, g, ps, done);
 }
 
 // kind 3 = the two passes compiled for 5 waves per SIMD
@@ -693,16 +706,18 @@ template <bool MU>
 void launch_h_t(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
                 const int *done)
 {
-    if (s->kind == 3) launch_h_occ<MU, 5>(s, A, g, ps, done);
-    else launch_h_occ<MU, 4>(s, A, g, ps, done);
+    if (s->kind == 3) launch_h_occ<MU, 5, false>(s, A, g, ps, done);
+    else if (stream_psi(s)) launch_h_occ<MU, 4, true>(s, A, g, ps, done);
+    else launch_h_occ<MU, 4, false>(s, A, g, ps, done);
 }
 
 template <int CM>
 void launch_e_t(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
                 const int *done)
 {
-    if (s->kind == 3) launch_e_occ<CM, 5>(s, A, g, ps, done);
-    else launch_e_occ<CM, 4>(s, A, g, ps, done);
+    if (s->kind == 3) launch_e_occ<CM, 5, false>(s, A, g, ps, done);
+    else if (stream_psi(s)) launch_e_occ<CM, 4, true>(s, A, g, ps, done);
+    else launch_e_occ<CM, 4, false>(s, A, g, ps, done);
 }
 
 // H pass in place on the current set (update_H, src/fdtd.cpp:121-367)
@@ -1690,6 +1705,14 @@ int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check
     for (auto &e : ev) (void)hipEventDestroy(e);
     if (rc != EMF_OK) EMF_FAIL(rc, "timed stepping failed");
     out[0] = tot; out[1] = th; out[2] = te;
+    return EMF_OK;
+}
+
+int emf_set_streaming(emf_solver *s, int mode)
+{
+    LOCK(s);
+    if (mode < -1 || mode > 1) EMF_FAIL(EMF_ERR_ARG, "mode must be -1, 0 or 1");
+    s->psi_stream = mode;
     return EMF_OK;
 }
 

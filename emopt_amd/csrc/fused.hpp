@@ -129,9 +129,39 @@ __device__ __forceinline__ double ldc(const double *base, unsigned off)
     return *(cdouble *)(reinterpret_cast<const char *>(base) + off);
 }
 
+// Streaming access for data that is touched once per step (the CPML psi
+// arrays): non-temporal loads and stores keep these lines from displacing the
+// fields that the next pass re-reads from L2 and the Infinity Cache.  Worth
+// it only when a step's data is larger than that cache; a smaller solve keeps
+// its psi in the cache from step to step.
+__device__ __forceinline__ double ldn(const double *base, unsigned off)
+{
+    return __builtin_nontemporal_load(
+        reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + off));
+}
+
+__device__ __forceinline__ void stn(double *base, unsigned off, double v)
+{
+    __builtin_nontemporal_store(v, reinterpret_cast<double *>(reinterpret_cast<char *>(base) + off));
+}
+
 __device__ __forceinline__ void sto(double *base, unsigned off, double v)
 {
     *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
+// psi access, streaming (NT) or plain
+template <bool NT>
+__device__ __forceinline__ double ldp(const double *base, unsigned off)
+{
+    return NT ? ldn(base, off) : ldo(base, off);
+}
+
+template <bool NT>
+__device__ __forceinline__ void stp(double *base, unsigned off, double v)
+{
+    if (NT) stn(base, off, v);
+    else sto(base, off, v);
 }
 
 // CPML slab of one cell along one axis: which face, psi byte offset and
@@ -193,7 +223,7 @@ struct HIn {
     Slab sx, sy, sz;
 };
 
-template <bool MU_ARRAY>
+template <bool MU_ARRAY, bool NT = false>
 __device__ __forceinline__ void h_load_aux(const Geo &g, const Layout &L, const FusedArgs &A,
                                            int i, int j, int k, unsigned o, HIn &d)
 {
@@ -206,20 +236,20 @@ __device__ __forceinline__ void h_load_aux(const Geo &g, const Layout &L, const 
     d.px0 = d.px1 = d.py0 = d.py1 = d.pz0 = d.pz1 = 0;
     d.kx = d.bx = d.cx = d.ky = d.by = d.cy = d.kz = d.bz = d.cz = 0;
     if (d.sx.in) {
-        d.px0 = ldo(A.pho, psof(L, 0, d.sx.lo, 0) + d.sx.io);
-        d.px1 = ldo(A.pho, psof(L, 0, d.sx.lo, 1) + d.sx.io);
+        d.px0 = ldp<NT>(A.pho, psof(L, 0, d.sx.lo, 0) + d.sx.io);
+        d.px1 = ldp<NT>(A.pho, psof(L, 0, d.sx.lo, 1) + d.sx.io);
         const unsigned t = L.tbo[0] + (unsigned)d.sx.par * 8u;
         d.kx = ldc(A.tab, t); d.bx = ldc(A.tab, t + L.tbl[0]); d.cx = ldc(A.tab, t + 2 * L.tbl[0]);
     }
     if (d.sy.in) {
-        d.py0 = ldo(A.pho, psof(L, 1, d.sy.lo, 0) + d.sy.io);
-        d.py1 = ldo(A.pho, psof(L, 1, d.sy.lo, 1) + d.sy.io);
+        d.py0 = ldp<NT>(A.pho, psof(L, 1, d.sy.lo, 0) + d.sy.io);
+        d.py1 = ldp<NT>(A.pho, psof(L, 1, d.sy.lo, 1) + d.sy.io);
         const unsigned t = L.tbo[1] + (unsigned)d.sy.par * 8u;
         d.ky = ldc(A.tab, t); d.by = ldc(A.tab, t + L.tbl[1]); d.cy = ldc(A.tab, t + 2 * L.tbl[1]);
     }
     if (d.sz.in) {
-        d.pz0 = ldo(A.pho, psof(L, 2, d.sz.lo, 0) + d.sz.io);
-        d.pz1 = ldo(A.pho, psof(L, 2, d.sz.lo, 1) + d.sz.io);
+        d.pz0 = ldp<NT>(A.pho, psof(L, 2, d.sz.lo, 0) + d.sz.io);
+        d.pz1 = ldp<NT>(A.pho, psof(L, 2, d.sz.lo, 1) + d.sz.io);
         const unsigned t = L.tbo[2] + (unsigned)d.sz.par * 8u;
         d.kz = ldc(A.tab, t); d.bz = ldc(A.tab, t + L.tbl[2]); d.cz = ldc(A.tab, t + 2 * L.tbl[2]);
     }

@@ -56,7 +56,7 @@ __device__ __forceinline__ void pass_tile(const Geo &g, int ty, int &k, int &j, 
 // the pre-update E and H (the state after the steps taken so far) are also
 // stored as a snapshot: component c of cell g goes to cap[2 * (c * n + g)]
 // (cap points at the .x or .y half of a six-component result block).
-template <bool MU_ARRAY>
+template <bool MU_ARRAY, bool NT = false>
 __device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const PassSrc &sh,
                                          int k, int j, int z0, int z1, double *cap,
                                          long long capn)
@@ -85,7 +85,7 @@ __device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const
         // E(k+1) comes from the next lane; the last lane loads it
         double ezk = 0, eyk = 0;
         if (threadIdx.x == PX - 1 && kn) { ezk = ldo(A.Eo, o + 8u + cs2); eyk = ldo(A.Eo, o + 8u + cs); }
-        h_load_aux<MU_ARRAY>(g, L, A, i, j, k, o, d);
+        h_load_aux<MU_ARRAY, NT>(g, L, A, i, j, k, o, d);
         d.ezk = __shfl_down(d.ez, 1);
         d.eyk = __shfl_down(d.ey, 1);
         if (threadIdx.x == PX - 1) { d.ezk = ezk; d.eyk = eyk; }
@@ -100,17 +100,17 @@ __device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const
         h_compute(g, d, hx, hy, hz, np);
         if (col_src) h_sources<MU_ARRAY>(g, sh, A, i, j, k, o, hx, hy, hz);
         sto(A.Hn, o, hx); sto(A.Hn, o + cs, hy); sto(A.Hn, o + cs2, hz);
-        if (d.sx.in) { sto(A.phn, psof(L, 0, d.sx.lo, 0) + d.sx.io, np[0]); sto(A.phn, psof(L, 0, d.sx.lo, 1) + d.sx.io, np[1]); }
-        if (d.sy.in) { sto(A.phn, psof(L, 1, d.sy.lo, 0) + d.sy.io, np[2]); sto(A.phn, psof(L, 1, d.sy.lo, 1) + d.sy.io, np[3]); }
-        if (d.sz.in) { sto(A.phn, psof(L, 2, d.sz.lo, 0) + d.sz.io, np[4]); sto(A.phn, psof(L, 2, d.sz.lo, 1) + d.sz.io, np[5]); }
+        if (d.sx.in) { stp<NT>(A.phn, psof(L, 0, d.sx.lo, 0) + d.sx.io, np[0]); stp<NT>(A.phn, psof(L, 0, d.sx.lo, 1) + d.sx.io, np[1]); }
+        if (d.sy.in) { stp<NT>(A.phn, psof(L, 1, d.sy.lo, 0) + d.sy.io, np[2]); stp<NT>(A.phn, psof(L, 1, d.sy.lo, 1) + d.sy.io, np[3]); }
+        if (d.sz.in) { stp<NT>(A.phn, psof(L, 2, d.sz.lo, 0) + d.sz.io, np[4]); stp<NT>(A.phn, psof(L, 2, d.sz.lo, 1) + d.sz.io, np[5]); }
         ex = d.exi; ey = d.eyi;
     }
 }
 
 // OCC = minimum waves per SIMD the register allocation must allow (4 fits
 // the passes without spilling; 5 trades a few spilled registers for 25 %
-// more waves in flight).
-template <bool MU_ARRAY, int TY, int OCC = 4>
+// more waves in flight).  NT: streaming psi access (see ldn).
+template <bool MU_ARRAY, int TY, int OCC = 4, bool NT = false>
 __global__ __launch_bounds__(PX * TY, OCC) void k_pass_h(FusedArgs A, Geo g, PassSrc sh,
                                                         const int *__restrict__ done)
 {
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_h(FusedArgs A, Geo g, Pas
     pass_tile(g, TY, k, j, z0, z1);
     j = __builtin_amdgcn_readfirstlane(j);   // one row per wave
     if (k >= g.Nx || j >= g.Ny) return;
-    h_column<MU_ARRAY>(A, g, sh, k, j, z0, z1, nullptr, 0);
+    h_column<MU_ARRAY, NT>(A, g, sh, k, j, z0, z1, nullptr, 0);
 }
 
 // ------------------------------------------------------------------ E pass
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_h(FusedArgs A, Geo g, Pas
 // advanced to plane i.  With CM_INDEX, ci is the cell's coefficient index,
 // loaded by the caller ahead of the plane's other loads: the table gather is
 // issued last, after every other load of the plane (psi included).
-template <int CM>
+template <int CM, bool NT = false>
 __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const PassSrc &se,
                                         const Layout &L, int k, int j, int i, unsigned o,
                                         unsigned ci, bool col_src, double &hxm, double &hym)
@@ -155,17 +155,17 @@ __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const 
     double qx0 = 0, qx1 = 0, qy0 = 0, qy1 = 0, qz0 = 0, qz1 = 0;
     double kx = 1, tbx = 0, tcx = 0, ky = 1, tby = 0, tcy = 0, kz = 1, tbz = 0, tcz = 0;
     if (sx.in) {
-        qx0 = ldo(A.pe, psof(L, 0, sx.lo, 0) + sx.io); qx1 = ldo(A.pe, psof(L, 0, sx.lo, 1) + sx.io);
+        qx0 = ldp<NT>(A.pe, psof(L, 0, sx.lo, 0) + sx.io); qx1 = ldp<NT>(A.pe, psof(L, 0, sx.lo, 1) + sx.io);
         const unsigned t = L.tbo[0] + 3 * L.tbl[0] + (unsigned)sx.par * 8u;
         kx = ldc(A.tab, t); tbx = ldc(A.tab, t + L.tbl[0]); tcx = ldc(A.tab, t + 2 * L.tbl[0]);
     }
     if (sy.in) {
-        qy0 = ldo(A.pe, psof(L, 1, sy.lo, 0) + sy.io); qy1 = ldo(A.pe, psof(L, 1, sy.lo, 1) + sy.io);
+        qy0 = ldp<NT>(A.pe, psof(L, 1, sy.lo, 0) + sy.io); qy1 = ldp<NT>(A.pe, psof(L, 1, sy.lo, 1) + sy.io);
         const unsigned t = L.tbo[1] + 3 * L.tbl[1] + (unsigned)sy.par * 8u;
         ky = ldc(A.tab, t); tby = ldc(A.tab, t + L.tbl[1]); tcy = ldc(A.tab, t + 2 * L.tbl[1]);
     }
     if (sz.in) {
-        qz0 = ldo(A.pe, psof(L, 2, sz.lo, 0) + sz.io); qz1 = ldo(A.pe, psof(L, 2, sz.lo, 1) + sz.io);
+        qz0 = ldp<NT>(A.pe, psof(L, 2, sz.lo, 0) + sz.io); qz1 = ldp<NT>(A.pe, psof(L, 2, sz.lo, 1) + sz.io);
         const unsigned t = L.tbo[2] + 3 * L.tbl[2] + (unsigned)sz.par * 8u;
         kz = ldc(A.tab, t); tbz = ldc(A.tab, t + L.tbl[2]); tcz = ldc(A.tab, t + 2 * L.tbl[2]);
     }
@@ -219,9 +219,9 @@ __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const 
         }
     }
     sto(A.En, o, nex); sto(A.En, o + cs, ney); sto(A.En, o + cs2, nez);
-    if (sx.in) { sto(A.pe, psof(L, 0, sx.lo, 0) + sx.io, qx0); sto(A.pe, psof(L, 0, sx.lo, 1) + sx.io, qx1); }
-    if (sy.in) { sto(A.pe, psof(L, 1, sy.lo, 0) + sy.io, qy0); sto(A.pe, psof(L, 1, sy.lo, 1) + sy.io, qy1); }
-    if (sz.in) { sto(A.pe, psof(L, 2, sz.lo, 0) + sz.io, qz0); sto(A.pe, psof(L, 2, sz.lo, 1) + sz.io, qz1); }
+    if (sx.in) { stp<NT>(A.pe, psof(L, 0, sx.lo, 0) + sx.io, qx0); stp<NT>(A.pe, psof(L, 0, sx.lo, 1) + sx.io, qx1); }
+    if (sy.in) { stp<NT>(A.pe, psof(L, 1, sy.lo, 0) + sy.io, qy0); stp<NT>(A.pe, psof(L, 1, sy.lo, 1) + sy.io, qy1); }
+    if (sz.in) { stp<NT>(A.pe, psof(L, 2, sz.lo, 0) + sz.io, qz0); stp<NT>(A.pe, psof(L, 2, sz.lo, 1) + sz.io, qz1); }
     hxm = hx; hym = hy;
 }
 
@@ -230,7 +230,7 @@ __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const 
 // planes are loaded first, ahead of everything else -- a plane's gather then
 // never waits for its own index.  ZU = 0: any number of planes, each plane
 // loading its index first.
-template <int CM, int ZU>
+template <int CM, int ZU, bool NT = false>
 __device__ __forceinline__ void e_column(const FusedArgs &A, const Geo &g, const PassSrc &se,
                                          int k, int j, int z0, int z1)
 {
@@ -257,7 +257,8 @@ __device__ __forceinline__ void e_column(const FusedArgs &A, const Geo &g, const
 #pragma unroll
         for (int p = 0; p < ZU; ++p) {
             if (z0 + p >= z1) break;
-            e_plane<CM>(A, g, se, L, k, j, z0 + p, o + (unsigned)p * oz, ci[p], col_src, hxm, hym);
+            e_plane<CM, NT>(A, g, se, L, k, j, z0 + p, o + (unsigned)p * oz, ci[p], col_src, hxm,
+                            hym);
         }
     } else {
         double hxm, hym;
@@ -265,13 +266,13 @@ __device__ __forceinline__ void e_column(const FusedArgs &A, const Geo &g, const
         else { hxm = bz0 ? 0.0 : sgz * ldo(A.Ho, o); hym = bz0 ? 0.0 : sgz * ldo(A.Ho, o + cs); }
         for (int i = z0; i < z1; ++i, o += oz) {
             const unsigned ci = CM == CM_INDEX ? coef_index(A, o) : 0u;
-            e_plane<CM>(A, g, se, L, k, j, i, o, ci, col_src, hxm, hym);
+            e_plane<CM, NT>(A, g, se, L, k, j, i, o, ci, col_src, hxm, hym);
         }
     }
 }
 
 // ZU: see e_column (the launch picks the smallest ZU of 2, 4, 8 that holds zc)
-template <int CM, int TY, int OCC = 4, int ZU = 0>
+template <int CM, int TY, int OCC = 4, int ZU = 0, bool NT = false>
 __global__ __launch_bounds__(PX * TY, OCC) void k_pass_e(FusedArgs A, Geo g, PassSrc se,
                                                         const int *__restrict__ done)
 {
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_e(FusedArgs A, Geo g, Pas
     pass_tile(g, TY, k, j, z0, z1, true);
     j = __builtin_amdgcn_readfirstlane(j);   // one row per wave
     if (k >= g.Nx || j >= g.Ny) return;
-    e_column<CM, ZU>(A, g, se, k, j, z0, z1);
+    e_column<CM, ZU, NT>(A, g, se, k, j, z0, z1);
 }
 
 }  // namespace emf

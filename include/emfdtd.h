@@ -254,6 +254,12 @@ EMF_API int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64
  * identical; only speed differs. */
 EMF_API int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc, int order);
 
+/* Streaming (non-temporal) access to the CPML psi arrays in the two passes:
+ * -1 (default) on when a step's data -- fields, coefficients and psi --
+ * exceeds the 256 MB Infinity Cache, 0 off, 1 on.  A cache policy only:
+ * results are the same either way.  No counterpart in the reference. */
+EMF_API int emf_set_streaming(emf_solver *s, int mode);
+
 /* Device copy bandwidth (GB/s, read + write bytes) of a plain vector copy of
  * `bytes` bytes, median of `reps` launches; a roofline reference point. */
 EMF_API int emf_copy_bandwidth(int device, int64_t bytes, int reps, double *gbps);

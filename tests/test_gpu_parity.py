@@ -139,7 +139,7 @@ def test_gpu_matches_reference_golden_solve():
     assert max(errs) <= TOL, errs
 
 
-def _run_steps(case, nsteps, kind, tyg=4, zc=8, order=0, compress=1):
+def _run_steps(case, nsteps, kind, tyg=4, zc=8, order=0, compress=1, stream=-1):
     """nsteps time steps through emf_time_steps with the given step kernel."""
     import ctypes
     import warnings
@@ -157,6 +157,7 @@ def _run_steps(case, nsteps, kind, tyg=4, zc=8, order=0, compress=1):
     L, h = sim._lib, sim._h
     L.emf_set_tuning(h, kind, tyg, zc, order)
     L.emf_set_compression(h, compress)
+    L.emf_set_streaming(h, stream)
     L.emf_reset_pml(h)
     L.emf_zero_fields(h)
     out = np.zeros(3)
@@ -236,3 +237,14 @@ def test_coefficient_compression_is_exact(which):
         b = _run_steps(case, 41, kind=kind, tyg=tyg, zc=zc, compress=0)
         for x, y in zip(a, b):
             assert np.array_equal(x, y), (which, kind)
+
+
+def test_psi_streaming_is_exact():
+    """Streaming (non-temporal) psi access is a cache policy only: the fields
+    after 41 steps are bit-identical with it forced on and off."""
+    case = random_case(70, 37, 23, w=(5, 4, 3, 6, 4, 5), seed=31, nsrc=2, mu_rand=True)
+    for tyg, zc in ((2, 2), (4, 5)):
+        a = _run_steps(case, 41, kind=1, tyg=tyg, zc=zc, order=3, stream=1)
+        b = _run_steps(case, 41, kind=1, tyg=tyg, zc=zc, order=3, stream=0)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), (tyg, zc)
