@@ -10,9 +10,12 @@
 // Per step:  k_batch_h   H half step of every tile (storing the t0 / t1
 //                        snapshot for solves at those points of their loop)
 //            k_batch_e   E half step of every tile
-//            k_batch_phasor  phasor pass of the solves that reached it
 //            k_batch_tick  per solve: step count, convergence check every Tn
 //                        steps, choice of what the next step does
+// and once per graph replay, after its steps:
+//            k_batch_phasor  phasor pass of the solves that reached it (they
+//                        wait, skipped by every kernel, until then)
+//            k_batch_finish  marks those solves finished
 //
 // Arithmetic per cell is that of k_pass_h / k_pass_e / k_phasor3 with the same
 // host-computed source factors, so a solve's result agrees with the
@@ -236,9 +239,10 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_e(const BTable *__restrict__
     }
 }
 
-// The phasor pass of the solves that reached it (after the E pass; the H and
-// E kernels skip them).  A kernel of its own so that its transcendental math
-// does not set the register budget, and so the occupancy, of the H pass.
+// The phasor pass of the solves that reached it (the H and E kernels skip
+// them), once at the end of a graph replay.  A kernel of its own so that its
+// transcendental math does not set the register budget, and so the
+// occupancy, of the H and E passes.
 __global__ __launch_bounds__(PX * BTY) void k_batch_phasor(const BTable *__restrict__ T,
                                                            const BMember *__restrict__ M,
                                                            const BState *__restrict__ S)
@@ -273,13 +277,7 @@ __global__ void k_batch_tick(const BTable *__restrict__ T, const BMember *__rest
     const int m = T->mem[e];
     const BMember &B = M[m];
     BState st = S[m];
-    if (st.mode == BM_IDLE) return;
-    if (st.mode == BM_PHASOR) {      // the phasor pass ran: the solve is finished
-        st.mode = BM_IDLE;
-        st.phase = BP_DONE;
-        S[m] = st;
-        return;
-    }
+    if (st.mode == BM_IDLE || st.mode == BM_PHASOR) return;   // PHASOR: waits for k_batch_finish
     const long long nd = ++st.step;  // steps completed
     int mode = BM_STEP;
     if (st.phase == BP_CONV) {
@@ -318,6 +316,20 @@ __global__ void k_batch_tick(const BTable *__restrict__ T, const BMember *__rest
     }
     st.mode = mode;
     S[m] = st;
+}
+
+// After the replay's phasor pass: the solves that took it are finished.
+__global__ void k_batch_finish(const BTable *__restrict__ T, BState *__restrict__ S)
+{
+    int t_lo, t_hi, e_lo, e_hi;
+    batch_group(T, t_lo, t_hi, e_lo, e_hi);
+    const int e = e_lo + (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
+    if (e >= e_hi) return;
+    const int m = T->mem[e];
+    if (S[m].mode == BM_PHASOR) {
+        S[m].mode = BM_IDLE;
+        S[m].phase = BP_DONE;
+    }
 }
 
 }  // namespace emf

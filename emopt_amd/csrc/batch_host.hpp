@@ -92,11 +92,11 @@ int batch_graph(emf_batch *b)
     for (int q = 0; q < b->spg; ++q) {
         hipLaunchKernelGGL(k_batch_h, dim3(blocks), dim3(PX, BTY), 0, b->stream, b->T, b->M, b->S);
         hipLaunchKernelGGL(k_batch_e, dim3(blocks), dim3(PX, BTY), 0, b->stream, b->T, b->M, b->S);
-        hipLaunchKernelGGL(k_batch_phasor, dim3(blocks), dim3(PX, BTY), 0, b->stream, b->T, b->M,
-                           b->S);
         hipLaunchKernelGGL(k_batch_tick, dim3(BATCH_MAX), dim3(64), 0, b->stream, b->T, b->M,
                            b->S);
     }
+    hipLaunchKernelGGL(k_batch_phasor, dim3(blocks), dim3(PX, BTY), 0, b->stream, b->T, b->M, b->S);
+    hipLaunchKernelGGL(k_batch_finish, dim3(BATCH_MAX / 64), dim3(64), 0, b->stream, b->T, b->S);
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(b->stream, &g);
     if (ec != hipSuccess || !g) EMF_FAIL(EMF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
