@@ -33,7 +33,7 @@ GROUP_BYTES = 260 * 10**6
 
 class Batch:
     def __init__(self, device=0, steps_per_graph=64, zc=2, blocks_per_cu=8,
-                 group_bytes=GROUP_BYTES):
+                 group_bytes=GROUP_BYTES, rotate_groups=False):
         self._lib = _lib.load()
         h = ctypes.c_void_p()
         self._lib.emf_batch_new(device, ctypes.byref(h))
@@ -41,13 +41,16 @@ class Batch:
         self._members = {}          # id -> (sim, slot, postprocess)
         self.finished = {}          # id -> solve info dict of finished members
         self.set_tuning(steps_per_graph, zc, blocks_per_cu)
-        self.set_grouping(group_bytes)
+        self.set_grouping(group_bytes, rotate_groups)
 
-    def set_grouping(self, group_bytes):
-        """Step members in groups whose hot state fits `group_bytes` (one
-        group per graph replay, in turns); 0 steps all members together."""
-        self._lib.emf_batch_set_grouping(self._h, int(group_bytes))
+    def set_grouping(self, group_bytes, rotate=False):
+        """Step members in groups whose hot state fits `group_bytes`, one
+        group per graph replay: the first group until its members finish
+        (rotate=False), or the groups in turns (rotate=True).  0 steps all
+        members together."""
+        self._lib.emf_batch_set_grouping(self._h, int(group_bytes), 1 if rotate else 0)
         self.group_bytes = group_bytes
+        self.rotate_groups = bool(rotate)
 
     def set_tuning(self, steps_per_graph, zc, blocks_per_cu):
         self._lib.emf_batch_set_tuning(self._h, int(steps_per_graph), int(zc), int(blocks_per_cu))

@@ -32,6 +32,7 @@ struct emf_batch {
     int bpc = 8;                        // persistent work-groups per CU
     long long group_bytes = 0;          // hot-state bytes per group (0: one group)
     int gnext = 0;                      // group the next replay steps
+    int rotate = 0;                     // 1: groups take turns; 0: first group until done
     int ncu = 0;
     BTable *T = nullptr;
     BMember *M = nullptr;
@@ -213,12 +214,14 @@ int emf_batch_set_tuning(emf_batch *b, int steps_per_graph, int zc, int blocks_p
     return EMF_OK;
 }
 
-int emf_batch_set_grouping(emf_batch *b, int64_t group_bytes)
+int emf_batch_set_grouping(emf_batch *b, int64_t group_bytes, int rotate)
 {
     BLOCK(b);
     if (group_bytes < 0) EMF_FAIL(EMF_ERR_ARG, "group_bytes must be >= 0");
+    if (rotate != 0 && rotate != 1) EMF_FAIL(EMF_ERR_ARG, "rotate must be 0 or 1");
     HIPCHK(hipStreamSynchronize(b->stream));
     b->group_bytes = group_bytes;
+    b->rotate = rotate;
     return batch_upload_table(b);
 }
 
@@ -384,7 +387,7 @@ int emf_batch_run(emf_batch *b, int max_graphs, int until, int *finished)
                 hipEvent_t ev;
                 hipLaunchKernelGGL(k_batch_set_group, dim3(1), dim3(64), 0, b->stream, b->T,
                                    b->gnext);
-                b->gnext = (b->gnext + 1) % std::max(1, b->hT.ngroups);
+                b->gnext = b->rotate ? (b->gnext + 1) % std::max(1, b->hT.ngroups) : 0;
                 if (hipGetLastError() != hipSuccess || hipGraphLaunch(b->exec, b->stream) != hipSuccess ||
                     hipMemcpyAsync(b->hS + (size_t)b->ring * BATCH_MAX, b->S,
                                    BATCH_MAX * sizeof(BState), hipMemcpyDeviceToHost,

@@ -290,11 +290,13 @@ EMF_API int emf_batch_set_tuning(emf_batch *b, int steps_per_graph, int zc, int 
 
 /* Cache-aware stepping: members are grouped (in admission order) so that a
  * group's hot state -- E, H and coefficients, ~80 B/cell -- stays within
- * group_bytes, and each graph replay steps one group, the groups taking turns.
- * Sized to the Infinity Cache, a group's E pass re-reads from cache what its
- * H pass just wrote.  0 (default) = one group: every replay steps all
- * members. */
-EMF_API int emf_batch_set_grouping(emf_batch *b, int64_t group_bytes);
+ * group_bytes, and each graph replay steps one group.  Sized to the Infinity
+ * Cache, a group's E pass re-reads from cache what its H pass just wrote.
+ * rotate = 1: the groups take turns, replay by replay; rotate = 0: the first
+ * group is stepped until its members finish, then the next (same throughput,
+ * earlier results for the first admitted).  group_bytes = 0 (default): one
+ * group, every replay steps all members.  Results never depend on either. */
+EMF_API int emf_batch_set_grouping(emf_batch *b, int64_t group_bytes, int rotate);
 
 /* Admits solver s: resets its fields and CPML as emf_solve does and queues
  * the solve (result slot, rtol, nmax, Tn >= 1, fixed_n0 as for emf_solve).

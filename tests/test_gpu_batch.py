@@ -116,14 +116,15 @@ def test_batch_continuous_admission(cases, refs):
     b.close()
 
 
-def test_batch_grouped_stepping(cases, refs):
-    """Cache-sized groups (here about two members per group, so the groups
-    take turns replay by replay), with members admitted part-way: every
-    member still matches its single solve, and exactly matches the same
-    member stepped without groups."""
+@pytest.mark.parametrize("rotate", [False, True])
+def test_batch_grouped_stepping(cases, refs, rotate):
+    """Cache-sized groups (here about two members per group), stepped first
+    group first or in turns replay by replay, with members admitted
+    part-way: every member still matches its single solve, and exactly
+    matches the same member stepped without groups."""
     from emopt_amd.batch import Batch
     one = max(80 * c["Nx"] * c["Ny"] * c["Nz"] for c in cases)
-    b = Batch(steps_per_graph=8, group_bytes=2 * one)
+    b = Batch(steps_per_graph=8, group_bytes=2 * one, rotate_groups=rotate)
     sims = {}
     for q in (0, 1, 2):
         sims[q] = build_sim(cases[q])
