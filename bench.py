@@ -234,7 +234,7 @@ def bench_batched(n, seed=3):
             "max_rel_l2_vs_plain_sample": err, "sample": sample}
 
 
-def bench_serving(rate, seconds, seed=5, budget=200e9, max_running=2):
+def bench_serving(rate, seconds, seed=5, budget=200e9, max_running=8):
     """Poisson arrivals at `rate` per second for `seconds` of requests of the
     serving workload (dims log-uniform 64^3..320x240x120, d 0.03-0.05 um,
     lambda 1.26-1.62 um, random ellipses), served by the continuous-batching
@@ -424,7 +424,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="arrival window of the serving section (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0)
-    ap.add_argument("--serve-running", type=int, default=2, help="solves stepping together")
+    ap.add_argument("--serve-running", type=int, default=8, help="solves admitted to the batch at once")
     ap.add_argument("--diff", action="store_true", help="also run the differentiation workload")
     ap.add_argument("--profile-summary", action="store_true")
     ap.add_argument("--stats")

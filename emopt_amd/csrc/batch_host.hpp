@@ -19,6 +19,7 @@ struct BatchSlot {
     double *fac = nullptr;              // device per-step factors
     emf_solve_info info{};
     std::chrono::steady_clock::time_point t_add;
+    long long seq = 0;                  // admission number (table order)
 };
 
 }  // namespace
@@ -33,6 +34,7 @@ struct emf_batch {
     long long group_bytes = 0;          // hot-state bytes per group (0: one group)
     int gnext = 0;                      // group the next replay steps
     int rotate = 0;                     // 1: groups take turns; 0: first group until done
+    long long next_seq = 0;             // admission counter
     int ncu = 0;
     BTable *T = nullptr;
     BMember *M = nullptr;
@@ -53,8 +55,14 @@ int batch_upload_table(emf_batch *b)
     BTable &t = b->hT;
     t.nact = 0;
     t.total = 0;
-    for (int m = 0; m < BATCH_MAX; ++m) {
-        if (!b->sl[m].s || b->sl[m].finished) continue;
+    // entries in admission order (slots are reused, so slot order is not
+    // arrival order): the first group then holds the oldest solves
+    int order[BATCH_MAX], na = 0;
+    for (int m = 0; m < BATCH_MAX; ++m)
+        if (b->sl[m].s && !b->sl[m].finished) order[na++] = m;
+    std::sort(order, order + na, [&](int x, int y) { return b->sl[x].seq < b->sl[y].seq; });
+    for (int q = 0; q < na; ++q) {
+        const int m = order[q];
         const BMember &B = b->hM[m];
         const long long tiles = (long long)B.ntx * B.nty * B.ntz;
         if ((long long)t.total + tiles > 0x7fffffffLL) EMF_FAIL(EMF_ERR_ARG, "batch too large");
@@ -329,6 +337,7 @@ int emf_batch_add(emf_batch *b, emf_solver *s, int slot, double rtol, double nma
     q.finished = false;
     q.info = emf_solve_info{};
     q.t_add = std::chrono::steady_clock::now();
+    q.seq = b->next_seq++;
     rc = batch_upload_table(b);           // synchronises the stream (B, st are locals)
     if (rc != EMF_OK) {
         q.s = nullptr;

@@ -9,13 +9,14 @@ batch at once, release their budget and hand their result to a callback.  The
 server records per-request latency (arrival to result) and throughput.
 
 Admission is first-come first-served: the queue head waits until it fits
-(no overtaking, so no request starves).  At most `max_running` solves step
-together; up to `prep_ahead` more are built ahead on the workers, so a
-finished solve is replaced at the next replay boundary.  A small
-`max_running` keeps latency close to first-come first-served order (one
-device's bandwidth is shared by fewer solves); a large one shares it among
-all admitted solves.  Results do not depend on admission order or on the
-other requests in the batch: each member's arithmetic is its own.
+(no overtaking, so no request starves).  At most `max_running` solves are
+members of the batch; up to `prep_ahead` more are built ahead on the
+workers, so a finished solve is replaced at the next replay boundary.  The
+batch steps its members in cache-sized groups in admission order, the
+oldest group first, so the device works through requests in arrival order
+while the host builds the next ones.  Results do not depend on admission
+order or on the other requests in the batch: each member's arithmetic is its
+own.
 """
 from __future__ import annotations
 
@@ -68,7 +69,7 @@ class Request:
 
 class Server:
     def __init__(self, device=0, hbm_budget=200e9, prep_workers=4, steps_per_graph=32, zc=2,
-                 blocks_per_cu=8, graphs_per_poll=4, on_done=None, batch=None, max_running=4,
+                 blocks_per_cu=8, graphs_per_poll=4, on_done=None, batch=None, max_running=8,
                  prep_ahead=None):
         self.batch = batch if batch is not None else Batch(device, steps_per_graph, zc,
                                                            blocks_per_cu)
