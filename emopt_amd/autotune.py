@@ -26,7 +26,7 @@ import numpy as np
 __all__ = ["SOLVE_CANDIDATES", "tune_solver", "BATCH_CANDIDATES", "tune_batch"]
 
 # (kind, rows per work-group, z planes per work-group, order)
-SOLVE_CANDIDATES = [(1, 2, 2, 3), (1, 2, 4, 3), (1, 4, 2, 3), (1, 2, 2, 0), (1, 4, 2, 0),
+SOLVE_CANDIDATES = [(1, 4, 2, 3), (1, 2, 2, 3), (1, 2, 4, 3), (1, 2, 2, 0), (1, 4, 2, 0),
                     (1, 8, 2, 3), (1, 2, 1, 3), (1, 4, 2, 1), (3, 2, 2, 3), (2, 8, 4, 0),
                     (2, 4, 4, 0), (0, 1, 8, 0)]
 _cache = {}

@@ -190,7 +190,7 @@ struct emf_solver {
     bool src_dirty = true;
 
     int kind = 1;                  // 0, 2 = fused step kernels; 1, 3 = separate H / E passes
-    int tyg = 2;                   // waves (rows) per work-group
+    int tyg = 4;                   // waves (rows) per work-group
 
     ConvState *conv = nullptr;
     long long *samples = nullptr;

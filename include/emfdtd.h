@@ -250,7 +250,7 @@ EMF_API int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64
  * launch sharing H through LDS; tyg = rows (wavefronts) per work-group (1, 2,
  * 4 or 8; 4, 8 or 16 for kind 2); zc = z planes per work-group; order =
  * work-group order of the two passes (0 x-fastest, 1 z-fastest, 2 z-fastest
- * grouped by XCD, 3 x-fastest with the E pass walking backwards).  Defaults kind 1, tyg 2, zc 2, order 3.  Results are
+ * grouped by XCD, 3 x-fastest with the E pass walking backwards).  Defaults kind 1, tyg 4, zc 2, order 3.  Results are
  * identical; only speed differs. */
 EMF_API int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc, int order);
 
