@@ -288,6 +288,31 @@ def bench_serving(rate, seconds, seed=5, budget=200e9, max_running=8):
                                     if st.get("solves_per_hr") else None)}
 
 
+def bench_materials(reps=3):
+    """Material sampling of the 400x200x120 workload (40 ellipses in a
+    240x120x10 design slab over cladding): the three eps components at their
+    Yee offsets on the host sampler and on the device (emf_sample_structured,
+    including the copy back to host memory), best of `reps`; the results are
+    compared for equality."""
+    from emopt_amd.fdtd import FDTD
+    from tests.cases import ellipse_design_eps
+    Nx, Ny, Nz, d = 400, 200, 120, 0.03
+    eps = ellipse_design_eps(Nx, Ny, Nz, d, seed=2)
+    shifts = FDTD.YEE_SHIFTS[:3]
+    eps.get_values_device(0, 8, 0, 8, 0, 8)    # warm-up
+    th, td, same = [], [], True
+    for _ in range(reps):
+        t = time.perf_counter()
+        h = [eps.get_values(0, Nx, 0, Ny, 0, Nz, sx=a, sy=b, sz=c) for a, b, c in shifts]
+        th.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        g = [eps.get_values_device(0, Nx, 0, Ny, 0, Nz, sx=a, sy=b, sz=c) for a, b, c in shifts]
+        td.append(time.perf_counter() - t)
+        same = same and all(np.array_equal(x, y) for x, y in zip(g, h))
+    return {"workload": "eps_xyz_400x200x120_ellipses40", "host_s": min(th), "device_s": min(td),
+            "speedup": min(th) / min(td), "identical": bool(same)}
+
+
 def bench_diff(n_fd=4, rtol=1e-5):
     """The differentiation workload at full size (360x60x120 grating, 64
     edge parameters): gradient through the autograd function (forward +
@@ -518,6 +543,10 @@ def main():
                         "cell_updates_per_s": cells * ls["steps"] / el}
         log("[bench] full solve: %d steps in %.2fs (%s)" % (ls["steps"], el, ls["status"]))
 
+    if not args.no_solve:
+        res["materials"] = bench_materials()
+        log("[bench] materials: host %.3fs, device %.3fs, identical %s" % (
+            res["materials"]["host_s"], res["materials"]["device_s"], res["materials"]["identical"]))
     if args.batch > 0:
         res["batched"] = bench_batched(args.batch)
     if args.serve_seconds > 0:

@@ -84,6 +84,9 @@ _SIGS = {
     "emf_time_steps": ([_P, c_int64, c_int64, c_int64, c_int, _P], c_int),
     "emf_set_tuning": ([_P, c_int, c_int, c_int, c_int], c_int),
     "emf_set_streaming": ([_P, c_int], c_int),
+    "emf_sample_structured": ([c_int, _P, c_int, _P, _P, c_int, c_double, c_double, c_double,
+                               c_int, c_double, c_double, c_int, c_int, c_int, c_int, c_int, c_int,
+                               c_double, c_double, c_double, _P], c_int),
     "emf_copy_bandwidth": ([c_int, c_int64, c_int, _P], c_int),
     "emf_calc_ydAx": ([_P, _P, _P, _P] + [c_int] * 6 + [_P], c_int),
     "emf_grad_assemble": ([_P, c_int64, _P, _P, _P, _P, c_int, c_double, _P], c_int),

@@ -23,6 +23,7 @@
 
 #include "../../include/emfdtd.h"
 #include "batch.hpp"
+#include "material.hpp"
 
 using namespace emf;
 
@@ -1731,6 +1732,95 @@ int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc, int order)
     s->tyg = tyg;
     s->zc = zc;
     s->order = order;
+    return EMF_OK;
+}
+
+int emf_sample_structured(int device, const emf_prim *prims, int nprims, const double *vx,
+                          const double *vy, int nverts, double dx, double dy, double dz,
+                          int supersample, double bg_re, double bg_im, int k1, int k2, int j1,
+                          int j2, int i1, int i2, double sx, double sy, double sz, double *out)
+{
+    static_assert(sizeof(emf_prim) == sizeof(MatPrim), "emf_prim and MatPrim layouts differ");
+    if (!out || nprims < 0 || (nprims > 0 && !prims) || nverts < 0 || supersample < 1 ||
+        supersample > 64 || !(dx > 0) || !(dy > 0) || !(dz > 0) || k2 <= k1 || j2 <= j1 ||
+        i2 <= i1)
+        EMF_FAIL(EMF_ERR_ARG, "bad arguments");
+    for (int q = 0; q < nprims; ++q) {
+        const emf_prim &p = prims[q];
+        if (p.kind < 0 || p.kind > 2) EMF_FAIL(EMF_ERR_ARG, "primitive %d: unknown kind", q);
+        if (p.kind == 2 && (p.v0 < 0 || p.nv < 0 || (long long)p.v0 + p.nv > nverts || !vx || !vy))
+            EMF_FAIL(EMF_ERR_ARG, "primitive %d: vertices out of range", q);
+    }
+    const int nk = k2 - k1, nj = j2 - j1, ni = i2 - i1;
+    const long long ncell = (long long)ni * nj * nk;
+    // z segments as the host sampler forms them: consecutive distinct primitive
+    // z bounds, skipped when no cell of the box overlaps them
+    std::vector<double> b;
+    for (int q = 0; q < nprims; ++q) {
+        if (!(prims[q].z2 > prims[q].z1)) continue;
+        b.push_back(prims[q].z1);
+        b.push_back(prims[q].z2);
+    }
+    std::sort(b.begin(), b.end());
+    b.erase(std::unique(b.begin(), b.end()), b.end());
+    std::vector<double> zab;
+    std::vector<int> off(1, 0), sp;
+    for (size_t s = 0; s + 1 < b.size(); ++s) {
+        const double za = b[s], zb = b[s + 1];
+        bool used = false;
+        for (int i = i1; i < i2 && !used; ++i) {
+            const double zc = ((double)i + sz) * dz;
+            const double zlo = zc - dz / 2, zhi = zc + dz / 2;
+            used = std::max(std::min(zhi, zb) - std::max(zlo, za), 0.0) / dz > 0;
+        }
+        if (!used) continue;
+        zab.push_back(za);
+        zab.push_back(zb);
+        for (int q = 0; q < nprims; ++q)
+            if (prims[q].z2 > prims[q].z1 && prims[q].z1 <= za && prims[q].z2 >= zb) sp.push_back(q);
+        off.push_back((int)sp.size());
+    }
+    const int nseg = (int)zab.size() / 2;
+    HIPCHK(hipSetDevice(device));
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    MatPrim *dp = nullptr;
+    double *dvx = nullptr, *dvy = nullptr, *dz_ab = nullptr;
+    int *doff = nullptr, *dsp = nullptr;
+    double2 *v2 = nullptr, *dout = nullptr;
+    int rc = EMF_OK;
+    auto ok = [&](int r) { if (rc == EMF_OK && r != EMF_OK) rc = r; return rc == EMF_OK; };
+    if (ok(dalloc(&dout, (size_t)ncell)) && nseg > 0 && ok(dalloc(&dp, (size_t)nprims)) &&
+        ok(dalloc(&doff, off.size())) && ok(dalloc(&dsp, std::max<size_t>(1, sp.size()))) &&
+        ok(dalloc(&dz_ab, zab.size())) && ok(dalloc(&v2, (size_t)nseg * nj * nk)) &&
+        (nverts == 0 || (ok(dalloc(&dvx, (size_t)nverts)) && ok(dalloc(&dvy, (size_t)nverts))))) {
+        if (hipMemcpyAsync(dp, prims, nprims * sizeof(MatPrim), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(doff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
+            (!sp.empty() && hipMemcpyAsync(dsp, sp.data(), sp.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess) ||
+            hipMemcpyAsync(dz_ab, zab.data(), zab.size() * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
+            (nverts > 0 && (hipMemcpyAsync(dvx, vx, nverts * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
+                            hipMemcpyAsync(dvy, vy, nverts * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)))
+            rc = EMF_ERR_HIP;
+        if (rc == EMF_OK) {
+            const long long nv2 = (long long)nseg * nj * nk;
+            hipLaunchKernelGGL(k_mat_inplane, dim3(grid1d(nv2, 256)), dim3(256), 0, st, dp, dvx, dvy,
+                               doff, dsp, nseg, k1, nk, j1, nj, sx, sy, dx, dy, supersample, bg_re,
+                               bg_im, v2);
+            if (hipGetLastError() != hipSuccess) rc = EMF_ERR_HIP;
+        }
+    }
+    if (rc == EMF_OK) {
+        hipLaunchKernelGGL(k_mat_combine, dim3(grid1d(ncell, 256)), dim3(256), 0, st, v2, dz_ab, nseg,
+                           i1, ni, nj, nk, sz, dz, bg_re, bg_im, dout);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(out, dout, (size_t)ncell * sizeof(double2), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = EMF_ERR_HIP;
+    }
+    (void)hipStreamSynchronize(st);
+    dfree(dp); dfree(dvx); dfree(dvy); dfree(dz_ab); dfree(doff); dfree(dsp); dfree(v2); dfree(dout);
+    (void)hipStreamDestroy(st);
+    if (rc != EMF_OK) EMF_FAIL(rc, "material sampling failed");
     return EMF_OK;
 }
 

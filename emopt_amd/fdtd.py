@@ -26,6 +26,7 @@ import numpy as np
 from . import _lib
 from ._lib import SolveInfo, c128, ptr
 from .defs import FIELD_INDEX, FieldComponent
+from .grid import StructuredMaterial3D
 from .misc import DomainCoordinates
 from .simulation import MaxwellSolver
 
@@ -276,9 +277,18 @@ class FDTD(MaxwellSolver):
         for q, (sx, sy, sz) in enumerate(self.YEE_SHIFTS):
             m = self._eps if q < 3 else self._mu
             a = np.empty(Nx * Ny * Nz, dtype=np.complex128)
-            m.get_values(0, Nx, 0, Ny, 0, Nz, sx=sx, sy=sy, sz=sz, arr=a)
+            self._sample(m, 0, Nx, 0, Ny, 0, Nz, sx, sy, sz, arr=a)
             arrs.append(c128(a))
         self.set_material_arrays(arrs[:3], arrs[3:])
+
+    def _sample(self, m, k1, k2, j1, j2, i1, i2, sx, sy, sz, arr=None):
+        """Material values of a box at one Yee offset: structured materials
+        are sampled on the GPU (the same doubles as their host sampler),
+        anything else by its own get_values."""
+        if isinstance(m, StructuredMaterial3D):
+            return m.get_values_device(k1, k2, j1, j2, i1, i2, sx=sx, sy=sy, sz=sz,
+                                       device=self._device, arr=arr)
+        return m.get_values(k1, k2, j1, j2, i1, i2, sx=sx, sy=sy, sz=sz, arr=arr)
 
     def set_material_arrays(self, eps_xyz, mu_xyz):
         """Upload already-sampled material arrays ([Nz,Ny,Nx] complex each)."""
@@ -311,7 +321,7 @@ class FDTD(MaxwellSolver):
             return
         for c in range(3):
             sx, sy, sz = self.YEE_SHIFTS[c]
-            tmp = c128(self._eps.get_values(k1, k2, j1, j2, i1, i2, sx=sx, sy=sy, sz=sz))
+            tmp = c128(self._sample(self._eps, k1, k2, j1, j2, i1, i2, sx, sy, sz))
             self._lib.emf_update_eps_box(self._h, c, ptr(tmp), k1, k2, j1, j2, i1, i2)
 
     # --------------------------------------------------------------- sources

@@ -11,6 +11,10 @@ provides:
 * primitives :class:`Rectangle`, :class:`Polygon`, :class:`Ellipse`
 * :class:`FunctionMaterial3D` for analytic / array-backed distributions.
 
+StructuredMaterial3D also samples on the GPU (``get_values_device``, the
+emf_sample_structured kernels), with the arithmetic below operation for
+operation; FDTD.build/update use that path for it.
+
 Cell averaging: a Yee cell value is the sum over z-slabs of the stack of
 (slab fraction of the cell's z extent) x (in-plane average), background 1.0
 outside the stack, exactly as src/Grid.cpp:736-824 combines layers.  In-plane,
@@ -232,6 +236,56 @@ class StructuredMaterial3D(Material3D):
             remaining -= cov
         val += remaining * self.background
         return val
+
+    # emf_prim of include/emfdtd.h
+    PRIM_DTYPE = np.dtype([("kind", "<i4"), ("clip", "<i4"), ("v0", "<i4"), ("nv", "<i4"),
+                           ("re", "<f8"), ("im", "<f8"), ("z1", "<f8"), ("z2", "<f8"),
+                           ("bbox", "<f8", (4,)), ("q", "<f8", (10,))])
+
+    def device_primitives(self):
+        """The primitives as emf_prim records in priority order (layer, then
+        insertion), with the polygon vertex arrays."""
+        order = sorted(self._prims, key=lambda p: (p[0].layer, p[3]))
+        recs = np.zeros(len(order), dtype=self.PRIM_DTYPE)
+        vx, vy = [], []
+        nv = 0
+        for r, (prim, z1, z2, _) in zip(recs, order):
+            r["re"], r["im"] = prim.material_value.real, prim.material_value.imag
+            r["z1"], r["z2"] = z1, z2
+            r["bbox"] = prim.bbox()
+            if isinstance(prim, Rectangle):
+                r["kind"] = 0
+            elif isinstance(prim, Ellipse):
+                r["kind"] = 1
+                r["q"][:6] = (prim.x0, prim.y0, prim.a, prim.b, np.cos(prim.angle), np.sin(prim.angle))
+                if prim.clip is not None:
+                    r["clip"] = 1
+                    r["q"][6:] = prim.clip
+            elif isinstance(prim, Polygon):
+                r["kind"] = 2
+                r["v0"], r["nv"] = nv, len(prim.xs)
+                vx.append(prim.xs)
+                vy.append(prim.ys)
+                nv += len(prim.xs)
+            else:
+                raise TypeError("no device sampler for %s" % type(prim).__name__)
+        vx = np.ascontiguousarray(np.concatenate(vx) if vx else np.zeros(1), dtype=np.double)
+        vy = np.ascontiguousarray(np.concatenate(vy) if vy else np.zeros(1), dtype=np.double)
+        return recs, vx, vy, nv
+
+    def get_values_device(self, k1, k2, j1, j2, i1, i2, sx=0.0, sy=0.0, sz=0.0, device=0,
+                          arr=None, reshape=True):
+        """get_values computed on GPU `device` (emf_sample_structured): the
+        same arithmetic in the same order, so the same doubles."""
+        from . import _lib
+        lib = _lib.load()
+        recs, vx, vy, nv = self.device_primitives()
+        out = np.empty((i2 - i1, j2 - j1, k2 - k1), dtype=np.complex128)
+        lib.emf_sample_structured(int(device), recs.ctypes.data, len(recs), vx.ctypes.data,
+                                  vy.ctypes.data, nv, self._dx, self._dy, self._dz,
+                                  int(self.supersample), self.background.real, self.background.imag,
+                                  k1, k2, j1, j2, i1, i2, sx, sy, sz, out.ctypes.data)
+        return self._out(out, arr, reshape)
 
     def get_values(self, k1, k2, j1, j2, i1, i2, sx=0.0, sy=0.0, sz=0.0, arr=None, reshape=True):
         dx, dy, dz = self._dx, self._dy, self._dz

@@ -278,6 +278,33 @@ EMF_API int emf_copy_bandwidth(int device, int64_t bytes, int reps, double *gbps
  * source factors are the same), and does not depend on the other members.
  * Members need real eps and uniform mu.  While a handle is a running member
  * it must not be used through any other call. */
+/* ----------------------------------------------------- material sampling
+ * Cell-averaged values of z-stacked primitives at one Yee offset, on the GPU
+ * (replaces the CPU sampling of StructuredMaterial3D, src/Grid.cpp:736-874 /
+ * emopt/grid.py:874-961, that EMopt's FDTD.build/update call,
+ * emopt/fdtd.py:842-992).  The arithmetic is that of the host sampler in
+ * emopt_amd/grid.py, operation for operation: rectangles by exact overlap,
+ * ellipses and polygons (even-odd rule) by ss x ss sample points per cell.
+ * prims are given in priority order (layer, then insertion); a z segment
+ * between consecutive primitive z bounds takes the primitives spanning it.
+ * out: complex128 [i2-i1][j2-j1][k2-k1] in host memory. */
+typedef struct {
+    int kind;          /* 0 rectangle, 1 ellipse, 2 polygon */
+    int clip;          /* ellipse: clipped to the box q[6..9] */
+    int v0, nv;        /* polygon: vertices [v0, v0 + nv) of vx, vy */
+    double re, im;     /* material value */
+    double z1, z2;     /* extent along z */
+    double bbox[4];    /* xmin, xmax, ymin, ymax (rectangle: the rectangle) */
+    double q[10];      /* ellipse: x0, y0, a, b, cos(angle), sin(angle), clip xmin, xmax,
+                          ymin, ymax */
+} emf_prim;
+
+EMF_API int emf_sample_structured(int device, const emf_prim *prims, int nprims,
+                                  const double *vx, const double *vy, int nverts, double dx,
+                                  double dy, double dz, int supersample, double bg_re,
+                                  double bg_im, int k1, int k2, int j1, int j2, int i1, int i2,
+                                  double sx, double sy, double sz, double *out);
+
 typedef struct emf_batch emf_batch;
 
 EMF_API int emf_batch_new(int device, emf_batch **out);
