@@ -172,7 +172,17 @@ __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const 
     // H(k-1) comes from the previous lane; lane 0 loads it
     double hz_k0 = 0, hy_k0 = 0;
     if (threadIdx.x == 0 && k > 0) { hz_k0 = ldo(A.Ho, o - 8u + cs2); hy_k0 = ldo(A.Ho, o - 8u + cs); }
-    if (CM == CM_INDEX) coef_gather(A, ci, bx, by, bz);
+    if (CM == CM_INDEX) {
+        // a wave whose cells share one table entry (bulk material) reads it
+        // with scalar loads; the others gather per lane
+        const unsigned c0 = __builtin_amdgcn_readfirstlane(ci);
+        if (__all((int)(ci == c0))) {
+            const unsigned t = c0 * 32u;
+            bx = ldc(A.bT, t); by = ldc(A.bT, t + 8u); bz = ldc(A.bT, t + 16u);
+        } else {
+            coef_gather(A, ci, bx, by, bz);
+        }
+    }
     double hz_k = __shfl_up(hz, 1), hy_k = __shfl_up(hy, 1);
     if (threadIdx.x == 0) { hz_k = hz_k0; hy_k = hy_k0; }
     // ---- boundary ghosts at the min faces
