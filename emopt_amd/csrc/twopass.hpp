@@ -9,10 +9,13 @@
 // go out before a plane's other loads and the table gather after them.
 //
 // A 64 x TY work-group owns a tile of (x, y) columns and sweeps zc planes
-// (zc = 1 is one cell per thread).  Every load a plane needs -- fields,
+// (zc = 1 is one cell per thread); each wave is one row, so the row index is
+// wave-uniform and row-dependent values (y-slab tests, CPML table entries)
+// live in scalar registers.  Every load a plane needs -- fields,
 // coefficients, psi and CPML table entries -- is issued before any store, so
-// a thread waits for memory once per plane; in-plane neighbours come through
-// L1/L2.
+// a thread waits for memory once per plane.  The x neighbour comes from the
+// adjacent lane (the wave's edge lane loads it); row and plane neighbours
+// come through L1/L2.
 #pragma once
 #include "fused.hpp"
 
