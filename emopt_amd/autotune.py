@@ -88,7 +88,7 @@ def tune_solver(sim, candidates=None, steps=40, warmup=8, cache_path=None, verbo
 
 
 # (steps per graph, z planes per work-group, work-groups per CU)
-BATCH_CANDIDATES = [(64, 2, 8), (64, 2, 4), (64, 4, 8), (32, 2, 8), (128, 2, 8), (64, 1, 8)]
+BATCH_CANDIDATES = [(64, 2, 16), (64, 2, 24), (64, 2, 8), (64, 4, 16), (32, 2, 16), (128, 2, 16), (64, 1, 16)]
 
 
 def tune_batch(sims, candidates=None, graphs=3, device=0, verbose=False):

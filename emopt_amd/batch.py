@@ -32,7 +32,7 @@ GROUP_BYTES = 260 * 10**6
 
 
 class Batch:
-    def __init__(self, device=0, steps_per_graph=64, zc=2, blocks_per_cu=8,
+    def __init__(self, device=0, steps_per_graph=64, zc=2, blocks_per_cu=16,
                  group_bytes=GROUP_BYTES, rotate_groups=False):
         self._lib = _lib.load()
         h = ctypes.c_void_p()

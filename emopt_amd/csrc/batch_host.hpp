@@ -30,7 +30,7 @@ struct emf_batch {
     hipStream_t stream = nullptr;
     int spg = 64;                       // steps per graph replay
     int zc = 2;                         // z planes per work-group
-    int bpc = 8;                        // persistent work-groups per CU
+    int bpc = 16;                       // persistent work-groups per CU
     long long group_bytes = 0;          // hot-state bytes per group (0: one group)
     int gnext = 0;                      // group the next replay steps
     int rotate = 0;                     // 1: groups take turns; 0: first group until done

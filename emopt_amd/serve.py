@@ -69,7 +69,7 @@ class Request:
 
 class Server:
     def __init__(self, device=0, hbm_budget=200e9, prep_workers=4, steps_per_graph=32, zc=2,
-                 blocks_per_cu=8, graphs_per_poll=4, on_done=None, batch=None, max_running=8,
+                 blocks_per_cu=16, graphs_per_poll=4, on_done=None, batch=None, max_running=8,
                  prep_ahead=None):
         self.batch = batch if batch is not None else Batch(device, steps_per_graph, zc,
                                                            blocks_per_cu)

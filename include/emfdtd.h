@@ -312,7 +312,9 @@ EMF_API int emf_batch_delete(emf_batch *b);
 
 /* steps_per_graph = steps in one graph replay (default 64); zc = z planes per
  * work-group for members admitted from now on (default 2); blocks_per_cu =
- * persistent work-groups per CU (default 8).  Only while nothing runs. */
+ * persistent work-groups per CU (default 16: more, smaller chunks of the
+ * tile range balance better than one resident round).  Only while nothing
+ * runs. */
 EMF_API int emf_batch_set_tuning(emf_batch *b, int steps_per_graph, int zc, int blocks_per_cu);
 
 /* Cache-aware stepping: members are grouped (in admission order) so that a
