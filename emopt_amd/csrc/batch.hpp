@@ -209,9 +209,9 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_h(const BTable *__restrict__
     }
 }
 
-// The E pass walks each chunk backwards, so it starts on the tiles the H
-// pass wrote last (still in the Infinity Cache), as the single-solve E pass
-// does with work-group order 3.
+// The E pass takes the chunks in reverse order and walks each backwards, so
+// it starts on the tiles the H pass wrote last (still in the Infinity Cache),
+// as the single-solve E pass does with work-group order 3.
 __global__ __launch_bounds__(PX * BTY) void k_batch_e(const BTable *__restrict__ T,
                                                       const BMember *__restrict__ M,
                                                       const BState *__restrict__ S)
@@ -219,7 +219,10 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_e(const BTable *__restrict__
     int t_lo, t_hi, e_lo, e_hi;
     batch_group(T, t_lo, t_hi, e_lo, e_hi);
     const int chunk = (t_hi - t_lo + (int)gridDim.x - 1) / (int)gridDim.x;
-    const int t0 = t_lo + (int)blockIdx.x * chunk, t1 = min(t0 + chunk, t_hi);
+    // chunks in reverse dispatch order: the first work-groups to run take the
+    // chunks whose H pass ran last
+    const int w = (int)gridDim.x - 1 - (int)blockIdx.x;
+    const int t0 = t_lo + w * chunk, t1 = min(t0 + chunk, t_hi);
     if (t0 >= t1) return;
     BCursor c;
     batch_seek(T, S, t1 - 1, c);
