@@ -1741,13 +1741,15 @@ int emf_sample_structured(int device, const emf_prim *prims, int nprims, const d
                           int j2, int i1, int i2, double sx, double sy, double sz, double *out)
 {
     static_assert(sizeof(emf_prim) == sizeof(MatPrim), "emf_prim and MatPrim layouts differ");
-    if (!out || nprims < 0 || (nprims > 0 && !prims) || nverts < 0 || supersample < 1 ||
+    if (!out || nprims < 0 || (nprims > 0 && !prims) || nverts < 0 || supersample < 0 ||
         supersample > 64 || !(dx > 0) || !(dy > 0) || !(dz > 0) || k2 <= k1 || j2 <= j1 ||
         i2 <= i1)
         EMF_FAIL(EMF_ERR_ARG, "bad arguments");
     for (int q = 0; q < nprims; ++q) {
         const emf_prim &p = prims[q];
         if (p.kind < 0 || p.kind > 2) EMF_FAIL(EMF_ERR_ARG, "primitive %d: unknown kind", q);
+        if (supersample == 0 && p.kind == 1)
+            EMF_FAIL(EMF_ERR_ARG, "primitive %d: the exact overlap rule takes ellipses as polygons", q);
         if (p.kind == 2 && (p.v0 < 0 || p.nv < 0 || (long long)p.v0 + p.nv > nverts || !vx || !vy))
             EMF_FAIL(EMF_ERR_ARG, "primitive %d: vertices out of range", q);
     }
@@ -1786,13 +1788,14 @@ int emf_sample_structured(int device, const emf_prim *prims, int nprims, const d
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     MatPrim *dp = nullptr;
     double *dvx = nullptr, *dvy = nullptr, *dz_ab = nullptr;
-    int *doff = nullptr, *dsp = nullptr;
+    int *doff = nullptr, *dsp = nullptr, *derr = nullptr;
     double2 *v2 = nullptr, *dout = nullptr;
     int rc = EMF_OK;
     auto ok = [&](int r) { if (rc == EMF_OK && r != EMF_OK) rc = r; return rc == EMF_OK; };
     if (ok(dalloc(&dout, (size_t)ncell)) && nseg > 0 && ok(dalloc(&dp, (size_t)nprims)) &&
         ok(dalloc(&doff, off.size())) && ok(dalloc(&dsp, std::max<size_t>(1, sp.size()))) &&
         ok(dalloc(&dz_ab, zab.size())) && ok(dalloc(&v2, (size_t)nseg * nj * nk)) &&
+        ok(dalloc(&derr, 1)) &&
         (nverts == 0 || (ok(dalloc(&dvx, (size_t)nverts)) && ok(dalloc(&dvy, (size_t)nverts))))) {
         if (hipMemcpyAsync(dp, prims, nprims * sizeof(MatPrim), hipMemcpyHostToDevice, st) != hipSuccess ||
             hipMemcpyAsync(doff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
@@ -1801,14 +1804,20 @@ int emf_sample_structured(int device, const emf_prim *prims, int nprims, const d
             (nverts > 0 && (hipMemcpyAsync(dvx, vx, nverts * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
                             hipMemcpyAsync(dvy, vy, nverts * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)))
             rc = EMF_ERR_HIP;
+        if (rc == EMF_OK && hipMemsetAsync(derr, 0, sizeof(int), st) != hipSuccess) rc = EMF_ERR_HIP;
         if (rc == EMF_OK) {
             const long long nv2 = (long long)nseg * nj * nk;
             hipLaunchKernelGGL(k_mat_inplane, dim3(grid1d(nv2, 256)), dim3(256), 0, st, dp, dvx, dvy,
                                doff, dsp, nseg, k1, nk, j1, nj, sx, sy, dx, dy, supersample, bg_re,
-                               bg_im, v2);
+                               bg_im, v2, derr);
             if (hipGetLastError() != hipSuccess) rc = EMF_ERR_HIP;
         }
     }
+    int cell_err = 0;
+    if (rc == EMF_OK && derr &&
+        (hipMemcpyAsync(&cell_err, derr, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+         hipStreamSynchronize(st) != hipSuccess))
+        rc = EMF_ERR_HIP;
     if (rc == EMF_OK) {
         hipLaunchKernelGGL(k_mat_combine, dim3(grid1d(ncell, 256)), dim3(256), 0, st, v2, dz_ab, nseg,
                            i1, ni, nj, nk, sz, dz, bg_re, bg_im, dout);
@@ -1819,8 +1828,12 @@ int emf_sample_structured(int device, const emf_prim *prims, int nprims, const d
     }
     (void)hipStreamSynchronize(st);
     dfree(dp); dfree(dvx); dfree(dvy); dfree(dz_ab); dfree(doff); dfree(dsp); dfree(v2); dfree(dout);
+    dfree(derr);
     (void)hipStreamDestroy(st);
     if (rc != EMF_OK) EMF_FAIL(rc, "material sampling failed");
+    if (cell_err)
+        EMF_FAIL(EMF_ERR_ARG, "a cell meets more than %d primitives or %d polygon edges "
+                 "(exact overlap rule)", MAT_MAXU, MAT_MAXE);
     return EMF_OK;
 }
 

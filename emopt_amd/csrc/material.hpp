@@ -2,18 +2,23 @@
 // z-stacked primitives at one Yee offset (StructuredMaterial3D, the C++
 // class of src/Grid.cpp:736-874 behind emopt/grid.py:874-961), computed with
 // exactly the arithmetic of the host sampler in emopt_amd/grid.py -- same
-// sample points, same operation order, no fused multiply-adds -- so the two
-// give identical doubles.
+// points, same operation order, no fused multiply-adds -- so the two give
+// identical doubles.
 //
 //   value(i, j, k) = sum over z segments s of fz(i, s) * v(s, j, k)
 //                    + (1 - sum fz) * background
-//   v(s, j, k)     = sum over the primitives active in segment s, in priority
-//                    order, of min(coverage, remaining) * value, the
-//                    remaining part of the cell taking the background
+//   v(s, j, k)     = the in-plane average of the primitives active in
+//                    segment s, by one of two rules:
 //
-// Rectangle coverage is the exact overlap of the cell with the rectangle;
-// ellipse and polygon coverage is the fraction of ss x ss sample points
-// inside (cells outside the primitive's bounding box: 0).
+// ss == 0, the reference's rule (src/Grid.cpp:460-525), spelled out above
+// mat_within: corner-gated primitives each taking the exact area of what is
+// left of the cell.
+//
+// ss > 0, supersampled: each primitive in priority order takes
+// min(coverage, remaining), the rest is background.  Rectangle coverage is the
+// exact overlap of the cell with the rectangle; ellipse and polygon coverage
+// is the fraction of ss x ss sample points inside (cells outside the
+// primitive's bounding box: 0).
 #pragma once
 #include "kernels.hpp"
 
@@ -79,12 +84,187 @@ __device__ inline double mat_coverage(const MatPrim &p, const double *__restrict
     return (double)count / (double)(ss * ss);
 }
 
-// v(s, j, k) for the used segments: v2[(s * nj + jj) * nk + kk]
+// ---------------------------------------------------------------------------
+// The reference's overlap rule (src/Grid.cpp:460-525 with GridCell::intersect,
+// src/Grid.cpp:74-108).  Per cell, primitives in priority order:
+//   * a primitive counts only when one of the four cell corners lies strictly
+//     inside it (Rectangle::contains_point, src/Grid.cpp:219-225; Polygon via
+//     boost::geometry::within, src/Grid.cpp:362-374: boundary points are out);
+//   * all four corners inside and nothing of the cell taken yet: the cell is
+//     the primitive's value;
+//   * otherwise the primitive takes the part of the cell not yet taken that
+//     it covers, exactly (boost::geometry::difference there);
+//   * the cell stops when nothing is left; what is left is background.
+// The taken part is the cell intersected with the union of the primitives
+// that counted so far, so each overlap is the growth of that union's area.
+// Rectangles are the polygon ring of src/Grid.cpp:204-213.  The reference
+// clips in __float128 (src/Grid.hpp:22); this is double precision.
+// ---------------------------------------------------------------------------
+constexpr int MAT_MAXU = 8;     // primitives counted in one cell
+constexpr int MAT_MAXE = 48;    // polygon edges through one cell
+constexpr int MAT_MAXB = 256;   // x breakpoints in one cell
+
+struct MatEdge {
+    double x1, y1, x2, y2;
+    int q;                      // which counted primitive
+};
+
+__device__ inline int mat_nvert(const MatPrim &p) { return p.kind == 0 ? 4 : p.nv; }
+
+__device__ inline void mat_vertex(const MatPrim &p, const double *__restrict__ vx,
+                                  const double *__restrict__ vy, int a, double &x, double &y)
+{
+    if (p.kind == 0) {
+        x = a < 2 ? p.bbox[0] : p.bbox[1];
+        y = (a == 0 || a == 3) ? p.bbox[2] : p.bbox[3];
+    } else {
+        x = vx[p.v0 + a];
+        y = vy[p.v0 + a];
+    }
+}
+
+// strictly inside (a point on the boundary is not)
+__device__ inline bool mat_within(const MatPrim &p, const double *__restrict__ vx,
+                                  const double *__restrict__ vy, double x, double y)
+{
+#pragma clang fp contract(off)
+    if (!(x > p.bbox[0] && x < p.bbox[1] && y > p.bbox[2] && y < p.bbox[3])) return false;
+    if (p.kind == 0) return true;
+    bool res = false;
+    for (int a = 0; a < p.nv; ++a) {
+        const int b = a + 1 < p.nv ? a + 1 : 0;
+        const double xa = vx[p.v0 + a], ya = vy[p.v0 + a];
+        const double xb = vx[p.v0 + b], yb = vy[p.v0 + b];
+        if ((xb - xa) * (y - ya) - (yb - ya) * (x - xa) == 0.0 && x >= fmin(xa, xb) &&
+            x <= fmax(xa, xb) && y >= fmin(ya, yb) && y <= fmax(ya, yb))
+            return false;
+        if ((ya > y) != (yb > y)) {
+            const double xint = xa + (y - ya) * (xb - xa) / (yb - ya);
+            if (x < xint) res = !res;
+        }
+    }
+    return res;
+}
+
+// Area of [xa, xb] x [ya, yb] inside the union of the primitives u[0..nu).
+// Between consecutive x breakpoints (ends of the edges through the cell,
+// their crossings of y = ya and y = yb, crossings of edges of different
+// primitives inside the cell) the covered length of a vertical line is linear
+// in x, so each slab contributes its width times the covered length at its
+// midpoint.  Only edges through the cell are kept; whether the midpoint line
+// enters the cell inside a primitive is the parity of that primitive's edges
+// crossing it at or below ya, counted over all its edges (it can only change
+// where a kept edge crosses y = ya, a breakpoint).  *err is set when a cell
+// exceeds the fixed limits above.
+__device__ inline double mat_union_area(const MatPrim *__restrict__ prims,
+                                        const double *__restrict__ vx,
+                                        const double *__restrict__ vy, const int *u, int nu,
+                                        double xa, double xb, double ya, double yb, bool &err)
+{
+#pragma clang fp contract(off)
+    MatEdge e[MAT_MAXE];
+    int ne = 0;
+    for (int s = 0; s < nu; ++s) {
+        const MatPrim &p = prims[u[s]];
+        const int nv = mat_nvert(p);
+        for (int a = 0; a < nv; ++a) {
+            double x1, y1, x2, y2;
+            mat_vertex(p, vx, vy, a, x1, y1);
+            mat_vertex(p, vx, vy, a + 1 < nv ? a + 1 : 0, x2, y2);
+            if (fmax(x1, x2) <= xa || fmin(x1, x2) >= xb || fmax(y1, y2) <= ya || fmin(y1, y2) >= yb)
+                continue;
+            if (ne == MAT_MAXE) { err = true; return 0.0; }
+            e[ne].x1 = x1; e[ne].y1 = y1; e[ne].x2 = x2; e[ne].y2 = y2; e[ne].q = s;
+            ++ne;
+        }
+    }
+    double bx[MAT_MAXB];
+    int nb = 0;
+    bx[nb++] = xa;
+    bx[nb++] = xb;
+    auto push = [&](double x) {
+        if (!(x > xa && x < xb)) return;
+        if (nb == MAT_MAXB) { err = true; return; }
+        bx[nb++] = x;
+    };
+    for (int i = 0; i < ne; ++i) {
+        const MatEdge &f = e[i];
+        push(f.x1);
+        push(f.x2);
+        if ((f.y1 > ya) != (f.y2 > ya)) push(f.x1 + (ya - f.y1) * (f.x2 - f.x1) / (f.y2 - f.y1));
+        if ((f.y1 > yb) != (f.y2 > yb)) push(f.x1 + (yb - f.y1) * (f.x2 - f.x1) / (f.y2 - f.y1));
+    }
+    for (int i = 0; i < ne; ++i)
+        for (int j = i + 1; j < ne; ++j) {
+            const MatEdge &f = e[i], &h = e[j];
+            if (f.q == h.q) continue;
+            const double d = (f.x2 - f.x1) * (h.y2 - h.y1) - (f.y2 - f.y1) * (h.x2 - h.x1);
+            if (d == 0.0) continue;
+            const double t = ((h.x1 - f.x1) * (h.y2 - h.y1) - (h.y1 - f.y1) * (h.x2 - h.x1)) / d;
+            const double r = ((h.x1 - f.x1) * (f.y2 - f.y1) - (h.y1 - f.y1) * (f.x2 - f.x1)) / d;
+            if (!(t > 0.0 && t < 1.0 && r > 0.0 && r < 1.0)) continue;
+            const double y = f.y1 + t * (f.y2 - f.y1);
+            if (y > ya && y < yb) push(f.x1 + t * (f.x2 - f.x1));
+        }
+    if (err) return 0.0;
+    for (int i = 1; i < nb; ++i) {      // insertion sort
+        const double v = bx[i];
+        int j = i - 1;
+        while (j >= 0 && bx[j] > v) { bx[j + 1] = bx[j]; --j; }
+        bx[j + 1] = v;
+    }
+    double area = 0.0;
+    for (int b = 0; b + 1 < nb; ++b) {
+        const double w = bx[b + 1] - bx[b];
+        if (!(w > 0.0)) continue;
+        const double xm = 0.5 * (bx[b] + bx[b + 1]);
+        unsigned in = 0;                // primitives the line is inside at y = ya
+        for (int s = 0; s < nu; ++s) {
+            const MatPrim &p = prims[u[s]];
+            const int nv = mat_nvert(p);
+            for (int a = 0; a < nv; ++a) {
+                double x1, y1, x2, y2;
+                mat_vertex(p, vx, vy, a, x1, y1);
+                mat_vertex(p, vx, vy, a + 1 < nv ? a + 1 : 0, x2, y2);
+                if (!(fmin(x1, x2) < xm && fmax(x1, x2) > xm)) continue;
+                if (y1 + (xm - x1) * (y2 - y1) / (x2 - x1) <= ya) in ^= 1u << s;
+            }
+        }
+        double ys[MAT_MAXE];
+        int qs[MAT_MAXE];
+        int ny = 0;
+        for (int i = 0; i < ne; ++i) {
+            const MatEdge &f = e[i];
+            if (!(fmin(f.x1, f.x2) < xm && fmax(f.x1, f.x2) > xm)) continue;
+            const double y = f.y1 + (xm - f.x1) * (f.y2 - f.y1) / (f.x2 - f.x1);
+            if (!(y > ya && y < yb)) continue;
+            int j = ny - 1;
+            while (j >= 0 && ys[j] > y) { ys[j + 1] = ys[j]; qs[j + 1] = qs[j]; --j; }
+            ys[j + 1] = y;
+            qs[j + 1] = f.q;
+            ++ny;
+        }
+        double len = 0.0, yp = ya;
+        for (int i = 0; i < ny; ++i) {
+            if (in) len += ys[i] - yp;
+            yp = ys[i];
+            in ^= 1u << qs[i];
+        }
+        if (in) len += yb - yp;
+        area += w * len;
+    }
+    return area;
+}
+
+// v(s, j, k) for the used segments: v2[(s * nj + jj) * nk + kk].  ss > 0:
+// supersampled coverage taking at most what is left of the cell; ss == 0: the
+// reference's rule above (rectangles and polygons only).
 __global__ void k_mat_inplane(const MatPrim *__restrict__ prims, const double *__restrict__ vx,
                               const double *__restrict__ vy, const int *__restrict__ seg_off,
                               const int *__restrict__ seg_prim, int nseg, int k1, int nk, int j1,
                               int nj, double sx, double sy, double dx, double dy, int ss,
-                              double bg_re, double bg_im, double2 *__restrict__ v2)
+                              double bg_re, double bg_im, double2 *__restrict__ v2,
+                              int *__restrict__ err)
 {
 #pragma clang fp contract(off)
     const long long total = (long long)nseg * nj * nk;
@@ -93,16 +273,55 @@ __global__ void k_mat_inplane(const MatPrim *__restrict__ prims, const double *_
         const int kk = (int)(g % nk), jj = (int)((g / nk) % nj), s = (int)(g / ((long long)nk * nj));
         const double xc = ((double)(k1 + kk) + sx) * dx;
         const double yc = ((double)(j1 + jj) + sy) * dy;
-        double rem = 1.0, vr = 0.0, vi = 0.0;
-        for (int q = seg_off[s]; q < seg_off[s + 1]; ++q) {
-            const MatPrim &p = prims[seg_prim[q]];
-            const double cov = fmin(mat_coverage(p, vx, vy, xc, yc, dx, dy, ss), rem);
-            vr += cov * p.re;
-            vi += cov * p.im;
-            rem -= cov;
+        double vr = 0.0, vi = 0.0;
+        if (ss > 0) {
+            double rem = 1.0;
+            for (int q = seg_off[s]; q < seg_off[s + 1]; ++q) {
+                const MatPrim &p = prims[seg_prim[q]];
+                const double cov = fmin(mat_coverage(p, vx, vy, xc, yc, dx, dy, ss), rem);
+                vr += cov * p.re;
+                vi += cov * p.im;
+                rem -= cov;
+            }
+            vr += rem * bg_re;
+            vi += rem * bg_im;
+        } else {
+            const double xa = xc - dx / 2, xb = xc + dx / 2, ya = yc - dy / 2, yb = yc + dy / 2;
+            const double A = (xb - xa) * (yb - ya);
+            double rem = A;
+            int u[MAT_MAXU];
+            int nu = 0;
+            bool full = false, bad = false;
+            for (int q = seg_off[s]; q < seg_off[s + 1] && !bad; ++q) {
+                const int pi = seg_prim[q];
+                const MatPrim &p = prims[pi];
+                const bool c1 = mat_within(p, vx, vy, xa, ya), c2 = mat_within(p, vx, vy, xb, ya);
+                const bool c3 = mat_within(p, vx, vy, xb, yb), c4 = mat_within(p, vx, vy, xa, yb);
+                if (c1 && c2 && c3 && c4 && rem == A) {
+                    vr = p.re;
+                    vi = p.im;
+                    full = true;
+                    break;
+                }
+                if (c1 || c2 || c3 || c4) {
+                    if (nu == MAT_MAXU) { bad = true; break; }
+                    u[nu++] = pi;
+                    double rn = A - mat_union_area(prims, vx, vy, u, nu, xa, xb, ya, yb, bad);
+                    if (rn <= A * 1e-13) rn = 0.0;   // fully taken (an empty difference there)
+                    const double ov = (rem - rn) / A;
+                    vr += ov * p.re;
+                    vi += ov * p.im;
+                    rem = rn;
+                }
+                if (rem == 0.0) break;
+            }
+            if (bad) *err = 1;
+            if (!full && rem > 0.0) {
+                const double r = rem / A;
+                vr += r * bg_re;
+                vi += r * bg_im;
+            }
         }
-        vr += rem * bg_re;
-        vi += rem * bg_im;
         v2[g] = make_double2(vr, vi);
     }
 }

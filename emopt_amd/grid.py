@@ -17,13 +17,22 @@ operation; FDTD.build/update use that path for it.
 
 Cell averaging: a Yee cell value is the sum over z-slabs of the stack of
 (slab fraction of the cell's z extent) x (in-plane average), background 1.0
-outside the stack, exactly as src/Grid.cpp:736-824 combines layers.  In-plane,
-primitives are visited in priority order (lower ``layer`` first); each takes
-its overlap with the part of the cell not yet covered, the remainder is
-background 1.0 (src/Grid.cpp:460-525).  Rectangle overlaps are exact; polygon
-and ellipse overlaps are supersampled (``supersample`` points per axis), since
-the reference's polygon clipping depends on boost.geometry, which this
-project does not use.
+outside the stack, exactly as src/Grid.cpp:736-824 combines layers.  The
+in-plane average follows one of two rules:
+
+* ``overlap="exact"`` (default), the reference's rule
+  (StructuredMaterial2D::get_value, src/Grid.cpp:460-525): primitives are
+  visited in priority order (lower ``layer`` first); one counts in a cell only
+  when a cell corner lies strictly inside it; if all four corners are inside
+  and nothing of the cell is taken yet the cell takes its value; otherwise it
+  takes the exact area of the part of the cell not yet taken that it covers
+  (GridCell::intersect, src/Grid.cpp:74-108, boost.geometry in __float128
+  there, a slab sweep in double precision here); the cell stops once nothing
+  is left, and the rest is background.  Ellipses (which the reference builds
+  as polygons) are inscribed polygons of ``ellipse_vertices`` vertices.
+* ``overlap="supersample"``: each primitive takes min(coverage, what is left);
+  rectangle coverage is exact, ellipse and polygon coverage is the fraction of
+  ``supersample`` x ``supersample`` points inside.
 """
 from __future__ import annotations
 
@@ -124,6 +133,11 @@ class Rectangle(MaterialPrimitive):
         fy = np.clip(np.minimum(yc + dy / 2, y1) - np.maximum(yc - dy / 2, y0), 0, None) / dy
         return fy[:, None] * fx[None, :]
 
+    def ring(self):
+        """The polygon ring of src/Grid.cpp:204-213."""
+        x0, x1, y0, y1 = self.bbox()
+        return np.array([x0, x0, x1, x1]), np.array([y0, y1, y1, y0])
+
 
 class _Sampled(MaterialPrimitive):
     def inside(self, x, y):
@@ -163,6 +177,18 @@ class Ellipse(_Sampled):
             y0, y1 = max(y0, self.clip[2]), min(y1, self.clip[3])
         return x0, x1, y0, y1
 
+    def polygon(self, n):
+        """Vertices of the inscribed n-gon (points at angles 2 pi i / n of the
+        ellipse's own axes), clipped to the clip box; empty arrays when
+        nothing is left."""
+        t = 2 * np.pi * np.arange(n) / n
+        u, v = self.a * np.cos(t), self.b * np.sin(t)
+        c, s = np.cos(self.angle), np.sin(self.angle)
+        xs, ys = self.x0 + u * c - v * s, self.y0 + u * s + v * c
+        if self.clip is not None:
+            xs, ys = _clip_to_box(xs, ys, self.clip)
+        return xs, ys
+
     def inside(self, x, y):
         c, s = np.cos(self.angle), np.sin(self.angle)
         u = (x - self.x0) * c + (y - self.y0) * s
@@ -188,6 +214,9 @@ class Polygon(_Sampled):
     def bbox(self):
         return self.xs.min(), self.xs.max(), self.ys.min(), self.ys.max()
 
+    def ring(self):
+        return self.xs, self.ys
+
     def inside(self, x, y):
         res = np.zeros(x.shape, dtype=bool)
         xs, ys = self.xs, self.ys
@@ -203,14 +232,206 @@ class Polygon(_Sampled):
         return res
 
 
+def _clip_to_box(xs, ys, box):
+    """Sutherland-Hodgman clip of a polygon to an axis-aligned box."""
+    pts = list(zip(xs, ys))
+    for axis, val, keep_above in ((0, box[0], True), (0, box[1], False),
+                                  (1, box[2], True), (1, box[3], False)):
+        if not pts:
+            break
+        out = []
+        for a in range(len(pts)):
+            p, q = pts[a - 1], pts[a]
+            pin = p[axis] >= val if keep_above else p[axis] <= val
+            qin = q[axis] >= val if keep_above else q[axis] <= val
+            if pin != qin:
+                t = (val - p[axis]) / (q[axis] - p[axis])
+                cut = [p[0] + t * (q[0] - p[0]), p[1] + t * (q[1] - p[1])]
+                cut[axis] = val
+                out.append(tuple(cut))
+            if qin:
+                out.append(q)
+        pts = out
+    if len(pts) < 3:
+        return np.zeros(0), np.zeros(0)
+    return np.array([p[0] for p in pts]), np.array([p[1] for p in pts])
+
+
+# ------------------------------------------- the reference's overlap rule
+# Limits of one cell, as in the device sampler (emf_sample_structured).
+_MAXU, _MAXE, _MAXB = 8, 48, 256
+
+
+class _Ring:
+    """A rectangle (kind 0) or polygon (kind 2) as the exact rule sees it."""
+
+    def __init__(self, kind, xs, ys, bbox, value):
+        self.kind, self.xs, self.ys = kind, np.asarray(xs, float), np.asarray(ys, float)
+        self.bbox = tuple(float(b) for b in bbox)
+        self.value = complex(value)
+
+    def within(self, x, y):
+        """Strictly inside, elementwise (boundary points are out: boost's
+        within, src/Grid.cpp:362-374; Rectangle::contains_point,
+        src/Grid.cpp:219-225)."""
+        b = self.bbox
+        res = (x > b[0]) & (x < b[1]) & (y > b[2]) & (y < b[3])
+        if self.kind == 0 or not np.any(res):
+            return res
+        odd = np.zeros(np.shape(x), dtype=bool)
+        edge = np.zeros(np.shape(x), dtype=bool)
+        xs, ys = self.xs, self.ys
+        n = len(xs)
+        for a in range(n):
+            c = a + 1 if a + 1 < n else 0
+            xa, ya, xb, yb = xs[a], ys[a], xs[c], ys[c]
+            edge |= (((xb - xa) * (y - ya) - (yb - ya) * (x - xa) == 0.0) &
+                     (x >= min(xa, xb)) & (x <= max(xa, xb)) & (y >= min(ya, yb)) & (y <= max(ya, yb)))
+            if (ya > yb) or (ya < yb):
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    xint = xa + (y - ya) * (xb - xa) / (yb - ya)
+                odd ^= ((ya > y) != (yb > y)) & (x < xint)
+        return res & odd & ~edge
+
+
+def _union_area(rings, xa, xb, ya, yb):
+    """Area of [xa, xb] x [ya, yb] inside the union of the rings: between
+    consecutive x breakpoints (ends of the edges through the cell, their
+    crossings of y = ya and y = yb, crossings of edges of different rings
+    inside the cell) the covered length of a vertical line is linear, so each
+    slab adds width x covered length at its midpoint; the line enters the
+    cell inside a ring when an odd number of the ring's edges cross it at or
+    below ya.  The operations are those of mat_union_area in
+    emopt_amd/csrc/material.hpp, in the same order."""
+    e = []
+    for q, r in enumerate(rings):
+        xs, ys = r.xs, r.ys
+        nxt = np.r_[np.arange(1, len(xs)), 0]
+        x2, y2 = xs[nxt], ys[nxt]
+        keep = ~((np.maximum(xs, x2) <= xa) | (np.minimum(xs, x2) >= xb) |
+                 (np.maximum(ys, y2) <= ya) | (np.minimum(ys, y2) >= yb))
+        for a in np.nonzero(keep)[0]:
+            if len(e) == _MAXE:
+                raise ValueError(_LIMIT_MSG)
+            e.append((float(xs[a]), float(ys[a]), float(x2[a]), float(y2[a]), q))
+    bx = [xa, xb]
+
+    def push(x):
+        if not (x > xa and x < xb):
+            return
+        if len(bx) == _MAXB:
+            raise ValueError(_LIMIT_MSG)
+        bx.append(x)
+
+    for x1, y1, x2, y2, _ in e:
+        push(x1)
+        push(x2)
+        if (y1 > ya) != (y2 > ya):
+            push(x1 + (ya - y1) * (x2 - x1) / (y2 - y1))
+        if (y1 > yb) != (y2 > yb):
+            push(x1 + (yb - y1) * (x2 - x1) / (y2 - y1))
+    for i in range(len(e)):
+        fx1, fy1, fx2, fy2, fq = e[i]
+        for j in range(i + 1, len(e)):
+            hx1, hy1, hx2, hy2, hq = e[j]
+            if fq == hq:
+                continue
+            d = (fx2 - fx1) * (hy2 - hy1) - (fy2 - fy1) * (hx2 - hx1)
+            if d == 0.0:
+                continue
+            t = ((hx1 - fx1) * (hy2 - hy1) - (hy1 - fy1) * (hx2 - hx1)) / d
+            r = ((hx1 - fx1) * (fy2 - fy1) - (hy1 - fy1) * (fx2 - fx1)) / d
+            if not (t > 0.0 and t < 1.0 and r > 0.0 and r < 1.0):
+                continue
+            y = fy1 + t * (fy2 - fy1)
+            if y > ya and y < yb:
+                push(fx1 + t * (fx2 - fx1))
+    bx.sort()
+    rx = []
+    for r in rings:
+        nxt = np.r_[np.arange(1, len(r.xs)), 0]
+        rx.append((r.xs, r.ys, r.xs[nxt], r.ys[nxt]))
+    area = 0.0
+    for b in range(len(bx) - 1):
+        w = bx[b + 1] - bx[b]
+        if not w > 0.0:
+            continue
+        xm = 0.5 * (bx[b] + bx[b + 1])
+        inside = 0
+        for q, (x1, y1, x2, y2) in enumerate(rx):
+            cross = (np.minimum(x1, x2) < xm) & (np.maximum(x1, x2) > xm)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                y = y1 + (xm - x1) * (y2 - y1) / (x2 - x1)
+            if np.count_nonzero(cross & (y <= ya)) % 2:
+                inside ^= 1 << q
+        ev = []
+        for x1, y1, x2, y2, q in e:
+            if not (min(x1, x2) < xm and max(x1, x2) > xm):
+                continue
+            y = y1 + (xm - x1) * (y2 - y1) / (x2 - x1)
+            if y > ya and y < yb:
+                ev.append((y, q))
+        ev.sort(key=lambda v: v[0])
+        length, yp = 0.0, ya
+        for y, q in ev:
+            if inside:
+                length += y - yp
+            yp = y
+            inside ^= 1 << q
+        if inside:
+            length += yb - yp
+        area += w * length
+    return area
+
+
+_LIMIT_MSG = ("a cell meets more than %d primitives or %d polygon edges "
+              "(exact overlap rule)" % (_MAXU, _MAXE))
+
+
+def _cell_exact(order, xa, xb, ya, yb, bg):
+    """One cell by the reference's rule, the operations of k_mat_inplane
+    (ss == 0) in the same order; order = the rings' corner flags
+    (c1, c2, c3, c4) in priority order."""
+    A = (xb - xa) * (yb - ya)
+    rem = A
+    vr = vi = 0.0
+    used = []
+    for r, (c1, c2, c3, c4) in order:
+        if c1 and c2 and c3 and c4 and rem == A:
+            return r.value.real, r.value.imag
+        if c1 or c2 or c3 or c4:
+            if len(used) == _MAXU:
+                raise ValueError(_LIMIT_MSG)
+            used.append(r)
+            rn = A - _union_area(used, xa, xb, ya, yb)
+            if rn <= A * 1e-13:
+                rn = 0.0
+            ov = (rem - rn) / A
+            vr += ov * r.value.real
+            vi += ov * r.value.imag
+            rem = rn
+        if rem == 0.0:
+            break
+    if rem > 0.0:
+        f = rem / A
+        vr += f * bg.real
+        vi += f * bg.imag
+    return vr, vi
+
+
 class StructuredMaterial3D(Material3D):
     """Primitives extruded over [z1, z2) ranges (emopt/grid.py:874-961)."""
 
-    def __init__(self, X, Y, Z, dx, dy, dz, supersample=8, background=1.0):
+    def __init__(self, X, Y, Z, dx, dy, dz, supersample=8, background=1.0, overlap="exact",
+                 ellipse_vertices=128):
+        if overlap not in ("exact", "supersample"):
+            raise ValueError("overlap must be 'exact' or 'supersample'")
         self._X, self._Y, self._Z = X, Y, Z
         self._dx, self._dy, self._dz = dx, dy, dz
         self.supersample = supersample
         self.background = complex(background)
+        self.overlap = overlap
+        self.ellipse_vertices = int(ellipse_vertices)
         self._prims = []    # (prim, z1, z2, insertion order)
 
     @property
@@ -226,7 +447,62 @@ class StructuredMaterial3D(Material3D):
         for p in prims:
             self.add_primitive(p, z1, z2)
 
+    def _ring(self, prim):
+        """The primitive as the exact rule sees it (None: nothing left)."""
+        if isinstance(prim, Rectangle):
+            return _Ring(0, *prim.ring(), prim.bbox(), prim.material_value)
+        if isinstance(prim, Ellipse):
+            xs, ys = prim.polygon(self.ellipse_vertices)
+        elif isinstance(prim, Polygon):
+            xs, ys = prim.ring()
+        else:
+            raise TypeError("no overlap rule for %s" % type(prim).__name__)
+        if len(xs) < 3:
+            return None
+        return _Ring(2, xs, ys, (xs.min(), xs.max(), ys.min(), ys.max()), prim.material_value)
+
+    def _inplane_exact(self, active, xc, yc):
+        dx, dy = self._dx, self._dy
+        rings = [r for r in (self._ring(p[0]) for p in sorted(active, key=lambda p: (p[0].layer, p[3])))
+                 if r is not None]
+        xa, xb = xc - dx / 2, xc + dx / 2
+        ya, yb = yc - dy / 2, yc + dy / 2
+        ny, nx = len(yc), len(xc)
+        bg = self.background
+        vr = np.full((ny, nx), 1.0 * bg.real)
+        vi = np.full((ny, nx), 1.0 * bg.imag)
+        flags = []
+        decided = np.zeros((ny, nx), dtype=bool)
+        partial = np.zeros((ny, nx), dtype=bool)
+        for r in rings:
+            c = [np.zeros((ny, nx), dtype=bool) for _ in range(4)]
+            kx = np.nonzero((xb > r.bbox[0]) & (xa < r.bbox[1]))[0]
+            ky = np.nonzero((yb > r.bbox[2]) & (ya < r.bbox[3]))[0]
+            if len(kx) and len(ky):
+                blk = np.ix_(ky, kx)
+                XA, YA = np.meshgrid(xa[kx], ya[ky])
+                XB, YB = np.meshgrid(xb[kx], yb[ky])
+                for q, (X, Y) in enumerate(((XA, YA), (XB, YA), (XB, YB), (XA, YB))):
+                    c[q][blk] = r.within(X, Y)
+            flags.append(c)
+            anyc = c[0] | c[1] | c[2] | c[3]
+            allc = c[0] & c[1] & c[2] & c[3]
+            first = anyc & ~decided
+            full = first & allc
+            vr[full] = r.value.real
+            vi[full] = r.value.imag
+            partial |= first & ~allc
+            decided |= first
+        for j, k in zip(*np.nonzero(partial)):
+            order = [(r, tuple(bool(f[q][j, k]) for q in range(4))) for r, f in zip(rings, flags)]
+            vr[j, k], vi[j, k] = _cell_exact(order, xa[k], xb[k], ya[j], yb[j], bg)
+        out = np.empty((ny, nx), dtype=np.complex128)
+        out.real, out.imag = vr, vi
+        return out
+
     def _inplane(self, active, xc, yc):
+        if self.overlap == "exact":
+            return self._inplane_exact(active, xc, yc)
         dx, dy = self._dx, self._dy
         remaining = np.ones((len(yc), len(xc)))
         val = np.zeros((len(yc), len(xc)), dtype=np.complex128)
@@ -244,14 +520,29 @@ class StructuredMaterial3D(Material3D):
 
     def device_primitives(self):
         """The primitives as emf_prim records in priority order (layer, then
-        insertion), with the polygon vertex arrays."""
+        insertion), with the polygon vertex arrays.  Under the exact rule an
+        ellipse goes as its inscribed polygon (and one clipped to nothing is
+        left out)."""
         order = sorted(self._prims, key=lambda p: (p[0].layer, p[3]))
+        if self.overlap == "exact":
+            order = [(p if not isinstance(p, Ellipse) else self._ring(p), z1, z2, n)
+                     for p, z1, z2, n in order]
+            order = [o for o in order if o[0] is not None]
         recs = np.zeros(len(order), dtype=self.PRIM_DTYPE)
         vx, vy = [], []
         nv = 0
         for r, (prim, z1, z2, _) in zip(recs, order):
-            r["re"], r["im"] = prim.material_value.real, prim.material_value.imag
             r["z1"], r["z2"] = z1, z2
+            if isinstance(prim, _Ring):       # an ellipse as a polygon
+                r["re"], r["im"] = prim.value.real, prim.value.imag
+                r["bbox"] = prim.bbox
+                r["kind"] = 2
+                r["v0"], r["nv"] = nv, len(prim.xs)
+                vx.append(prim.xs)
+                vy.append(prim.ys)
+                nv += len(prim.xs)
+                continue
+            r["re"], r["im"] = prim.material_value.real, prim.material_value.imag
             r["bbox"] = prim.bbox()
             if isinstance(prim, Rectangle):
                 r["kind"] = 0
@@ -283,7 +574,8 @@ class StructuredMaterial3D(Material3D):
         out = np.empty((i2 - i1, j2 - j1, k2 - k1), dtype=np.complex128)
         lib.emf_sample_structured(int(device), recs.ctypes.data, len(recs), vx.ctypes.data,
                                   vy.ctypes.data, nv, self._dx, self._dy, self._dz,
-                                  int(self.supersample), self.background.real, self.background.imag,
+                                  0 if self.overlap == "exact" else int(self.supersample),
+                                  self.background.real, self.background.imag,
                                   k1, k2, j1, j2, i1, i2, sx, sy, sz, out.ctypes.data)
         return self._out(out, arr, reshape)
 
@@ -295,12 +587,15 @@ class StructuredMaterial3D(Material3D):
         zlo, zhi = zc - dz / 2, zc + dz / 2
         out = np.zeros((i2 - i1, j2 - j1, k2 - k1), dtype=np.complex128)
         covered = np.zeros(i2 - i1)
-        bounds = sorted({b for _, z1, z2, _ in self._prims for b in (z1, z2)})
+        prims = self._prims
+        if self.overlap == "exact":    # as device_primitives: no ellipse clipped to nothing
+            prims = [p for p in prims if not isinstance(p[0], Ellipse) or self._ring(p[0]) is not None]
+        bounds = sorted({b for _, z1, z2, _ in prims for b in (z1, z2)})
         for za, zb in zip(bounds[:-1], bounds[1:]):
             fz = np.clip(np.minimum(zhi, zb) - np.maximum(zlo, za), 0, None) / dz
             if not np.any(fz > 0):
                 continue
-            active = [p for p in self._prims if p[1] <= za and p[2] >= zb]
+            active = [p for p in prims if p[1] <= za and p[2] >= zb]
             v2 = self._inplane(active, xc, yc)
             nz = np.nonzero(fz > 0)[0]
             out[nz] += fz[nz, None, None] * v2[None, :, :]

@@ -283,8 +283,19 @@ EMF_API int emf_copy_bandwidth(int device, int64_t bytes, int reps, double *gbps
  * (replaces the CPU sampling of StructuredMaterial3D, src/Grid.cpp:736-874 /
  * emopt/grid.py:874-961, that EMopt's FDTD.build/update call,
  * emopt/fdtd.py:842-992).  The arithmetic is that of the host sampler in
- * emopt_amd/grid.py, operation for operation: rectangles by exact overlap,
- * ellipses and polygons (even-odd rule) by ss x ss sample points per cell.
+ * emopt_amd/grid.py, operation for operation, by one of two in-plane rules:
+ *   supersample == 0: the reference's rule (StructuredMaterial2D::get_value,
+ *     src/Grid.cpp:460-525): a primitive counts in a cell only when a cell
+ *     corner lies strictly inside it; four corners inside with nothing of the
+ *     cell taken yet give the primitive's value; otherwise it takes the exact
+ *     area of the part of the cell not yet taken that it covers
+ *     (GridCell::intersect, src/Grid.cpp:74-108), in double precision where
+ *     the reference clips in __float128; the rest is background.  Rectangles
+ *     and polygons only (ellipses as polygons); a cell may meet at most 8
+ *     counted primitives and 48 edges of them, else EMF_ERR_ARG;
+ *   supersample = ss in 1..64: rectangles by exact overlap, ellipses and
+ *     polygons (even-odd rule) by ss x ss sample points per cell, each
+ *     primitive taking at most what is left of the cell.
  * prims are given in priority order (layer, then insertion); a z segment
  * between consecutive primitive z bounds takes the primitives spanning it.
  * out: complex128 [i2-i1][j2-j1][k2-k1] in host memory. */

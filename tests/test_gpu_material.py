@@ -1,12 +1,13 @@
 """Structured material sampling on the GPU (emf_sample_structured) against
 the host sampler of emopt_amd/grid.py, which restates the reference's
-StructuredMaterial3D (src/Grid.cpp:736-874, emopt/grid.py:874-961; its
-polygon overlaps are supersampled here, see grid.py).  The device kernels
-repeat the host arithmetic operation for operation, so the comparison is
-exact (np.array_equal) at all six Yee offsets, on whole grids and boxes:
-rectangles with edges on and off cell boundaries, rotated and clipped
-ellipses, convex and concave polygons, overlapping layers and z ranges,
-complex values and background."""
+StructuredMaterial3D (src/Grid.cpp:736-874, emopt/grid.py:874-961), under
+both in-plane rules: the reference's exact one (corner-gated primitives
+taking exact areas, src/Grid.cpp:460-525) and the supersampled one.  The
+device kernels repeat the host arithmetic operation for operation, so the
+comparison is exact (np.array_equal) at all six Yee offsets, on whole grids
+and boxes: rectangles with edges on and off cell boundaries, rotated and
+clipped ellipses, convex and concave polygons, overlapping layers and z
+ranges, complex values and background."""
 import time
 
 import numpy as np
@@ -21,11 +22,12 @@ SHIFTS = ((0.5, 0.0, -0.5), (0.0, 0.5, -0.5), (0.0, 0.0, 0.0),
           (0.0, 0.5, 0.0), (0.5, 0.0, 0.0), (0.5, 0.5, -0.5))
 
 
-def _random_structure(seed, N=(40, 34, 26), d=0.05):
+def _random_structure(seed, N=(40, 34, 26), d=0.05, overlap="supersample"):
     rng = np.random.default_rng(seed)
     X, Y, Z = (n * d for n in N)
     m = StructuredMaterial3D(X, Y, Z, d, d, d, supersample=int(rng.integers(3, 9)),
-                             background=1.0 + 0.05j * rng.random())
+                             background=1.0 + 0.05j * rng.random(), overlap=overlap,
+                             ellipse_vertices=int(rng.integers(12, 200)))
     for q in range(14):
         kind = q % 3
         val = complex(1 + 11 * rng.random(), 0.3 * rng.random() * (q % 2))
@@ -57,9 +59,10 @@ def _random_structure(seed, N=(40, 34, 26), d=0.05):
     return m, N
 
 
+@pytest.mark.parametrize("overlap", ["exact", "supersample"])
 @pytest.mark.parametrize("seed", [0, 1, 2, 3])
-def test_device_sampler_matches_host(seed):
-    m, (Nx, Ny, Nz) = _random_structure(seed)
+def test_device_sampler_matches_host(seed, overlap):
+    m, (Nx, Ny, Nz) = _random_structure(seed, overlap=overlap)
     boxes = [(0, Nx, 0, Ny, 0, Nz), (3, 29, 5, 21, 2, 17), (0, 1, 0, Ny, 7, 8)]
     for (k1, k2, j1, j2, i1, i2) in boxes:
         for (sx, sy, sz) in SHIFTS:
@@ -78,6 +81,28 @@ def test_device_sampler_outside_all_primitives():
     g = m.get_values_device(0, 10, 0, 10, 0, 10)
     assert np.array_equal(g, m.get_values(0, 10, 0, 10, 0, 10))
     assert np.all(g == 2.25)
+
+
+def test_device_sampler_exact_rule_limits():
+    """A cell where ten partial rectangles count: the device reports the
+    limit as an error, as the host sampler does, and an ellipse is refused by
+    the exact rule's entry point unless given as a polygon."""
+    from emopt_amd import _lib
+    m = StructuredMaterial3D(2.0, 2.0, 1.0, 0.1, 0.1, 0.1)
+    for q in range(10):
+        m.add_primitive(Rectangle(1.0, 1.0, 1.0 - 0.01 * q, 1.0 + 0.013 * q, material_value=2.0,
+                                  layer=q), 0.0, 1.0)
+    with pytest.raises(ValueError, match="more than 8 primitives"):
+        m.get_values(0, 20, 0, 20, 0, 1, sx=0.3, sy=0.3, sz=0.5)
+    with pytest.raises(_lib.EmfError, match="more than 8 primitives"):
+        m.get_values_device(0, 20, 0, 20, 0, 1, sx=0.3, sy=0.3, sz=0.5)
+    recs, vx, vy, nv = m.device_primitives()
+    recs["kind"][0] = 1
+    out = np.empty((1, 20, 20), dtype=np.complex128)
+    with pytest.raises(_lib.EmfError, match="ellipses as polygons"):
+        _lib.load().emf_sample_structured(0, recs.ctypes.data, len(recs), vx.ctypes.data,
+                                          vy.ctypes.data, nv, 0.1, 0.1, 0.1, 0, 1.0, 0.0,
+                                          0, 20, 0, 20, 0, 1, 0.0, 0.0, 0.0, out.ctypes.data)
 
 
 def test_device_sampler_large_design_region():
