@@ -292,25 +292,32 @@ def bench_materials(reps=3):
     """Material sampling of the 400x200x120 workload (40 ellipses in a
     240x120x10 design slab over cladding): the three eps components at their
     Yee offsets on the host sampler and on the device (emf_sample_structured,
-    including the copy back to host memory), best of `reps`; the results are
-    compared for equality."""
+    including the copy back to host memory), best of `reps`, under the
+    reference's exact overlap rule (the default) and the supersampled one;
+    the host and device results are compared for equality."""
     from emopt_amd.fdtd import FDTD
     from tests.cases import ellipse_design_eps
     Nx, Ny, Nz, d = 400, 200, 120, 0.03
-    eps = ellipse_design_eps(Nx, Ny, Nz, d, seed=2)
     shifts = FDTD.YEE_SHIFTS[:3]
-    eps.get_values_device(0, 8, 0, 8, 0, 8)    # warm-up
-    th, td, same = [], [], True
-    for _ in range(reps):
-        t = time.perf_counter()
-        h = [eps.get_values(0, Nx, 0, Ny, 0, Nz, sx=a, sy=b, sz=c) for a, b, c in shifts]
-        th.append(time.perf_counter() - t)
-        t = time.perf_counter()
-        g = [eps.get_values_device(0, Nx, 0, Ny, 0, Nz, sx=a, sy=b, sz=c) for a, b, c in shifts]
-        td.append(time.perf_counter() - t)
-        same = same and all(np.array_equal(x, y) for x, y in zip(g, h))
-    return {"workload": "eps_xyz_400x200x120_ellipses40", "host_s": min(th), "device_s": min(td),
-            "speedup": min(th) / min(td), "identical": bool(same)}
+    res = {}
+    for overlap in ("exact", "supersample"):
+        eps = ellipse_design_eps(Nx, Ny, Nz, d, seed=2, overlap=overlap)
+        eps.get_values_device(0, 8, 0, 8, 0, 8)    # warm-up
+        th, td, same = [], [], True
+        for _ in range(reps):
+            t = time.perf_counter()
+            h = [eps.get_values(0, Nx, 0, Ny, 0, Nz, sx=a, sy=b, sz=c) for a, b, c in shifts]
+            th.append(time.perf_counter() - t)
+            t = time.perf_counter()
+            g = [eps.get_values_device(0, Nx, 0, Ny, 0, Nz, sx=a, sy=b, sz=c) for a, b, c in shifts]
+            td.append(time.perf_counter() - t)
+            same = same and all(np.array_equal(x, y) for x, y in zip(g, h))
+        res[overlap] = {"host_s": min(th), "device_s": min(td), "speedup": min(th) / min(td),
+                        "identical": bool(same)}
+    out = {"workload": "eps_xyz_400x200x120_ellipses40", "rule": "exact"}
+    out.update(res["exact"])
+    out["supersample"] = res["supersample"]
+    return out
 
 
 def bench_diff(n_fd=4, rtol=1e-5):
@@ -545,8 +552,11 @@ def main():
 
     if not args.no_solve:
         res["materials"] = bench_materials()
-        log("[bench] materials: host %.3fs, device %.3fs, identical %s" % (
-            res["materials"]["host_s"], res["materials"]["device_s"], res["materials"]["identical"]))
+        mt = res["materials"]
+        log("[bench] materials, exact rule: host %.3fs, device %.3fs, identical %s; supersampled: "
+            "host %.3fs, device %.3fs, identical %s" % (
+                mt["host_s"], mt["device_s"], mt["identical"], mt["supersample"]["host_s"],
+                mt["supersample"]["device_s"], mt["supersample"]["identical"]))
     if args.batch > 0:
         res["batched"] = bench_batched(args.batch)
     if args.serve_seconds > 0:

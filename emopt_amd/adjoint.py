@@ -240,10 +240,15 @@ def rect_edge_deps(sim, eps, prim, attr, delta, bbox):
     k1, k2, j1, j2, i1, i2 = bbox
     comps, cells, vals = [], [], []
     v0 = getattr(prim, attr)
+    # the solver's own sampling (structured materials on its GPU, the same
+    # doubles as the host sampler), else the material's get_values
+    sample = getattr(sim, "_sample", None)
+    if sample is None:
+        sample = lambda m, *box: m.get_values(*box[:6], sx=box[6], sy=box[7], sz=box[8])
     samples = []  # eps at p + delta and p - delta, per component
     for sgn in (+1, -1):
         setattr(prim, attr, v0 + sgn * delta)
-        samples.append([eps.get_values(k1, k2, j1, j2, i1, i2, sx=sx, sy=sy, sz=sz)
+        samples.append([sample(eps, k1, k2, j1, j2, i1, i2, sx, sy, sz)
                         for (sx, sy, sz) in sim.YEE_SHIFTS[:3]])
     setattr(prim, attr, v0)
     for c in range(3):

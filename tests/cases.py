@@ -78,15 +78,16 @@ def strip_case(Nx, Ny, Nz, w=10, d=0.04, wl=1.55, core=(0.5, 0.22), rtol=1e-4,
 
 
 def ellipse_design_eps(Nx, Ny, Nz, d, design=(240, 120, 10), n_ell=40, seed=2,
-                       axes=(0.1, 0.6)):
+                       axes=(0.1, 0.6), overlap="supersample"):
     """Cladding (1.444^2) everywhere; a design slab of design[0] x design[1]
     x design[2] cells centred in the grid filled with n_ell seeded ellipses
     (semi-axes uniform in `axes` um, eps 3.48^2 inside), boundary cells
-    area-averaged."""
+    area-averaged (by default supersampled, the rule these workloads were
+    defined with; overlap="exact" gives the reference's rule)."""
     from emopt_amd.grid import StructuredMaterial3D, Rectangle, Ellipse
     X, Y, Z = span(Nx, d), span(Ny, d), span(Nz, d)
     rng = np.random.default_rng(seed)
-    eps = StructuredMaterial3D(X, Y, Z, d, d, d)
+    eps = StructuredMaterial3D(X, Y, Z, d, d, d, overlap=overlap)
     eps.add_primitive(Rectangle(X / 2, Y / 2, 2 * X, 2 * Y, material_value=EPS_OX, layer=10), -Z, 2 * Z)
     dxw, dyw, dzw = design[0] * d, design[1] * d, design[2] * d
     box = (X / 2 - dxw / 2, X / 2 + dxw / 2, Y / 2 - dyw / 2, Y / 2 + dyw / 2)
@@ -206,7 +207,7 @@ def batch_case(q, seed=3, dims=((64, 256), (64, 160), (48, 96)), w_range=(8, 12)
     wl = float(rng.uniform(1.50, 1.60))
     w = int(rng.integers(w_range[0], w_range[1] + 1))
     X, Y, Z = span(Nx, d), span(Ny, d), span(Nz, d)
-    eps = StructuredMaterial3D(X, Y, Z, d, d, d)
+    eps = StructuredMaterial3D(X, Y, Z, d, d, d, overlap="supersample")
     eps.add_primitive(Rectangle(X / 2, Y / 2, 2 * X, 2 * Y, material_value=EPS_OX, layer=10),
                       -Z, 2 * Z)
     t = float(rng.uniform(0.15, 0.35))
