@@ -133,8 +133,8 @@ __device__ __forceinline__ void phasor_column(const BMember &B, const BState &S,
         for (int c = 0; c < 6; ++c) {
             const double f = c < 3 ? ldo(B.A.Eo, o + (unsigned)c * B.A.cs)
                                    : ldo(B.A.Ho, o + (unsigned)(c - 3) * B.A.cs);
-            const double2 v = r[c * n + gc];
-            r[c * n + gc] = phasor3(v.x, v.y, f, c < 3 ? S.te : S.th);
+            const double2 v = ld2(r + c * n + gc);
+            st2(r + c * n + gc, phasor3(v.x, v.y, f, c < 3 ? S.te : S.th));
         }
     }
 }
@@ -200,10 +200,10 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_h(const BTable *__restrict__
         const BMember &B = M[c.m];
         int k, j, z0, z1;
         if (!batch_column(B, t - T->off[c.e], k, j, z0, z1)) continue;
-        const double *f = B.fac + c.step * NFAC;
+        const unsigned fo = (unsigned)(c.step * NFAC) * 8u;   // step factors, read-only here
         PassSrc ps;
         ps.n = B.nsrc; ps.comp0 = 3; ps.blk = B.blk;
-        ps.st = f[0]; ps.ct = f[1]; ps.env = f[2];
+        ps.st = ldc(B.fac, fo + 0u); ps.ct = ldc(B.fac, fo + 8u); ps.env = ldc(B.fac, fo + 16u);
         double *cap = c.mode == BM_CAP0 ? B.res : (c.mode == BM_CAP1 ? B.res + 1 : nullptr);
         h_column<false>(B.A, B.g, ps, k, j, z0, z1, cap, B.n);
     }
@@ -232,10 +232,10 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_e(const BTable *__restrict__
         const BMember &B = M[c.m];
         int k, j, z0, z1;
         if (!batch_column(B, t - T->off[c.e], k, j, z0, z1)) continue;
-        const double *f = B.fac + c.step * NFAC;
+        const unsigned fo = (unsigned)(c.step * NFAC) * 8u;   // step factors, read-only here
         PassSrc ps;
         ps.n = B.nsrc; ps.comp0 = 0; ps.blk = B.blk;
-        ps.st = f[3]; ps.ct = f[4]; ps.env = f[5];
+        ps.st = ldc(B.fac, fo + 24u); ps.ct = ldc(B.fac, fo + 32u); ps.env = ldc(B.fac, fo + 40u);
         if (B.A.bI && z1 - z0 <= 2) e_column<CM_INDEX, 2>(B.A, B.g, ps, k, j, z0, z1);
         else if (B.A.bI) e_column<CM_INDEX, 0>(B.A, B.g, ps, k, j, z0, z1);
         else e_column<CM_REAL, 0>(B.A, B.g, ps, k, j, z0, z1);

@@ -57,7 +57,7 @@ enum { CM_REAL = 0, CM_COMPLEX = 1, CM_INDEX = 2 };
 __device__ __forceinline__ unsigned coef_index(const FusedArgs &A, unsigned o)
 {
     const unsigned e = o >> 3, w2 = A.bIw == 2;
-    const unsigned v = *(const unsigned *)((const char *)A.bI + (w2 ? (e << 1) & ~3u : e << 2));
+    const unsigned v = *gptr((const unsigned *)((const char *)A.bI + (w2 ? (e << 1) & ~3u : e << 2)));
     return w2 ? (v >> ((e & 1u) << 4)) & 0xffffu : v;
 }
 
@@ -66,9 +66,9 @@ __device__ __forceinline__ unsigned coef_index(const FusedArgs &A, unsigned o)
 __device__ __forceinline__ void coef_gather(const FusedArgs &A, unsigned ci, double &bx,
                                             double &by, double &bz)
 {
-    const double2 xy = *reinterpret_cast<const double2 *>(A.bT + 4 * ci);
+    const double2 xy = ld2(reinterpret_cast<const double2 *>(A.bT + 4 * ci));
     bx = xy.x; by = xy.y;
-    bz = A.bT[4 * ci + 2];
+    bz = *gptr(A.bT + 4 * ci + 2);
 }
 
 // Byte offsets of the psi slab arrays of a set ([face][component], faces
@@ -113,10 +113,17 @@ __device__ __forceinline__ unsigned psof(const Layout &L, int axis, bool lo, int
 // Accesses by 32-bit byte offset from a uniform base pointer: the backend then
 // addresses with an SGPR base plus one VGPR offset shared by all arrays,
 // instead of a 64-bit VGPR address per array.  The fused path is used only
-// when every array is smaller than 4 GiB.
+// when every array is smaller than 4 GiB.  Every array lives in device global
+// memory, and the accesses say so: where a base pointer comes from memory (a
+// batch member's descriptor) rather than from the kernel arguments, the
+// backend could not infer it and would emit flat accesses, which count on the
+// scalar-memory counter too and make every wait for a scalar load also wait
+// for the field loads in flight.
+typedef __attribute__((address_space(1))) double gdouble;
+
 __device__ __forceinline__ double ldo(const double *base, unsigned off)
 {
-    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + off);
+    return *(const gdouble *)(reinterpret_cast<const char *>(base) + off);
 }
 
 // Load from data that no kernel writes while a step runs (the CPML tables),
@@ -136,18 +143,17 @@ __device__ __forceinline__ double ldc(const double *base, unsigned off)
 // its psi in the cache from step to step.
 __device__ __forceinline__ double ldn(const double *base, unsigned off)
 {
-    return __builtin_nontemporal_load(
-        reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + off));
+    return __builtin_nontemporal_load((const gdouble *)(reinterpret_cast<const char *>(base) + off));
 }
 
 __device__ __forceinline__ void stn(double *base, unsigned off, double v)
 {
-    __builtin_nontemporal_store(v, reinterpret_cast<double *>(reinterpret_cast<char *>(base) + off));
+    __builtin_nontemporal_store(v, (gdouble *)(reinterpret_cast<char *>(base) + off));
 }
 
 __device__ __forceinline__ void sto(double *base, unsigned off, double v)
 {
-    *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + off) = v;
+    *(gdouble *)(reinterpret_cast<char *>(base) + off) = v;
 }
 
 // psi access, streaming (NT) or plain
@@ -296,9 +302,9 @@ __device__ __forceinline__ void h_sources(const Geo &g, const PassSrc &sh, const
 {
     for (int q = 0; q < sh.n; ++q) {
         long long m;
-        const SrcBlk &sb = sh.blk[q];
+        const SrcBlk sb = src_blk(sh.blk, q);
         if (in_box(sb, i, j, k, m)) {
-            const double2 vx = sb.f[3][m], vy = sb.f[4][m], vz = sb.f[5][m];
+            const double2 vx = ld2(sb.f[3] + m), vy = ld2(sb.f[4] + m), vz = ld2(sb.f[5] + m);
             const double ux = MU_ARRAY ? ldo(A.muR, o) : A.mu_s;
             const double uy = MU_ARRAY ? ldo(A.muR, o + A.cs) : A.mu_s;
             const double uz = MU_ARRAY ? ldo(A.muR, o + 2 * A.cs) : A.mu_s;
@@ -335,7 +341,7 @@ __global__ __launch_bounds__(FX * TY) void k_step_fused(FusedArgs A, Geo g, Pass
         const int tk0 = bx * FXO - 1, tk1 = tk0 + FX;
         const int tz0 = z0 > 0 ? z0 - 1 : z0;
         for (int q = 0; q < sh.n; ++q) {
-            const SrcBlk &s = sh.blk[q];
+            const SrcBlk s = src_blk(sh.blk, q);
             if (s.k0 < tk1 && s.k0 + s.K > tk0 && s.j0 <= j && s.j0 + s.J > j - 1 && s.i0 < z1 &&
                 s.i0 + s.I > tz0)
                 row_src = true;
@@ -477,9 +483,9 @@ __global__ __launch_bounds__(FX * TY) void k_step_fused(FusedArgs A, Geo g, Pass
             if (row_src) {
                 for (int q = 0; q < se.n; ++q) {
                     long long m;
-                    const SrcBlk &sb = se.blk[q];
+                    const SrcBlk sb = src_blk(se.blk, q);
                     if (in_box(sb, i, j, k, m)) {
-                        const double2 vx = sb.f[0][m], vy = sb.f[1][m], vz = sb.f[2][m];
+                        const double2 vx = ld2(sb.f[0] + m), vy = ld2(sb.f[1] + m), vz = ld2(sb.f[2] + m);
                         nex = nex - src_val(se, vx) * bex;
                         ney = ney - src_val(se, vy) * bey;
                         nez = nez - src_val(se, vz) * bez;
@@ -547,13 +553,13 @@ __global__ __launch_bounds__(FX * TY) void k_step_lds(FusedArgs A, Geo g, PassSr
         const int tk0 = bx * FXO - 1, tk1 = tk0 + FX, tj0 = by * (TY - 1) - 1, tj1 = tj0 + TY;
         const int tz0 = z0 > 0 ? z0 - 1 : z0;
         for (int q = 0; q < sh.n; ++q) {
-            const SrcBlk &s = sh.blk[q];
+            const SrcBlk s = src_blk(sh.blk, q);
             if (s.k0 < tk1 && s.k0 + s.K > tk0 && s.j0 < tj1 && s.j0 + s.J > tj0 && s.i0 < z1 &&
                 s.i0 + s.I > tz0)
                 tsh = true;
         }
         for (int q = 0; q < se.n; ++q) {
-            const SrcBlk &s = se.blk[q];
+            const SrcBlk s = src_blk(se.blk, q);
             if (s.k0 < tk1 && s.k0 + s.K > tk0 && s.j0 < tj1 && s.j0 + s.J > tj0 && s.i0 < z1 &&
                 s.i0 + s.I > z0)
                 tse = true;
@@ -667,9 +673,9 @@ __global__ __launch_bounds__(FX * TY) void k_step_lds(FusedArgs A, Geo g, PassSr
             if (tse) {
                 for (int q = 0; q < se.n; ++q) {
                     long long m;
-                    const SrcBlk &sb = se.blk[q];
+                    const SrcBlk sb = src_blk(se.blk, q);
                     if (in_box(sb, i, j, k, m)) {
-                        const double2 vx = sb.f[0][m], vy = sb.f[1][m], vz = sb.f[2][m];
+                        const double2 vx = ld2(sb.f[0] + m), vy = ld2(sb.f[1] + m), vz = ld2(sb.f[2] + m);
                         nex = nex - src_val(se, vx) * bex;
                         ney = ney - src_val(se, vy) * bey;
                         nez = nez - src_val(se, vz) * bez;

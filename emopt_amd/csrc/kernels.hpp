@@ -55,6 +55,56 @@ struct SrcBlk {
     const double2 *f[6];
 };
 
+// Global-memory views of device pointers (every array here lives in device
+// global memory): accesses through them are global, not flat, also where the
+// pointer itself was loaded from memory rather than passed as an argument.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gptr(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gptr(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+// double2 through the global address space (a native two-element vector,
+// which binds where HIP_vector_type's copy constructor would not)
+typedef double gvec2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ double2 ld2(const double2 *p)
+{
+    const gvec2 v = *(const __attribute__((address_space(1))) gvec2 *)p;
+    return make_double2(v.x, v.y);
+}
+
+__device__ __forceinline__ void st2(double2 *p, double2 v)
+{
+    gvec2 w;
+    w.x = v.x;
+    w.y = v.y;
+    *(__attribute__((address_space(1))) gvec2 *)p = w;
+}
+
+// Source block q, read through the constant address space: descriptors are
+// written before a launch and only read while it runs, and all lanes read the
+// same one, so the backend loads it with scalar loads.
+__device__ __forceinline__ SrcBlk src_blk(const SrcBlk *blk, int q)
+{
+    typedef const __attribute__((address_space(4))) int cint;
+    typedef const double2 *dptr;
+    typedef const __attribute__((address_space(4))) dptr cdptr;
+    const SrcBlk *p = blk + q;
+    SrcBlk s;
+    s.i0 = *(cint *)&p->i0; s.j0 = *(cint *)&p->j0; s.k0 = *(cint *)&p->k0;
+    s.I = *(cint *)&p->I; s.J = *(cint *)&p->J; s.K = *(cint *)&p->K;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s.f[c] = *(cdptr *)&p->f[c];
+    return s;
+}
+
 // Sources of one pass: n descriptors in device memory evaluated at time t:
 // st = sin(t), ct = cos(t), env = ramp envelope (exactly 1 after the ramp)
 // of src_func_t (src/fdtd.cpp:1595-1602).  Blocks are applied in order, as

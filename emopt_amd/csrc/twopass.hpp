@@ -71,7 +71,7 @@ __device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const
     const bool jn = j + 1 < Ny, kn = k + 1 < Nx;
     bool col_src = false;   // does this column meet a source block?
     for (int q = 0; q < sh.n; ++q) {
-        const SrcBlk &s = sh.blk[q];
+        const SrcBlk s = src_blk(sh.blk, q);
         if (k >= s.k0 && k < s.k0 + s.K && j >= s.j0 && j < s.j0 + s.J && s.i0 < z1 && s.i0 + s.I > z0)
             col_src = true;
     }
@@ -95,9 +95,9 @@ __device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const
         if (!kn) d.ezk = d.eyk = 0;   // zero ghost past the max face
         if (cap) {
             const long long gc = (long long)(o >> 3);
-            cap[2 * gc] = d.ex; cap[2 * (capn + gc)] = d.ey; cap[2 * (2 * capn + gc)] = d.ez;
-            cap[2 * (3 * capn + gc)] = d.hx; cap[2 * (4 * capn + gc)] = d.hy;
-            cap[2 * (5 * capn + gc)] = d.hz;
+            *gptr(cap + 2 * gc) = d.ex; *gptr(cap + 2 * (capn + gc)) = d.ey;
+            *gptr(cap + 2 * (2 * capn + gc)) = d.ez; *gptr(cap + 2 * (3 * capn + gc)) = d.hx;
+            *gptr(cap + 2 * (4 * capn + gc)) = d.hy; *gptr(cap + 2 * (5 * capn + gc)) = d.hz;
         }
         double hx, hy, hz, np[6];
         h_compute(g, d, hx, hy, hz, np);
@@ -222,9 +222,9 @@ __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const 
     if (col_src) {
         for (int q = 0; q < se.n; ++q) {
             long long m;
-            const SrcBlk &sb = se.blk[q];
+            const SrcBlk sb = src_blk(se.blk, q);
             if (in_box(sb, i, j, k, m)) {
-                const double2 vx = sb.f[0][m], vy = sb.f[1][m], vz = sb.f[2][m];
+                const double2 vx = ld2(sb.f[0] + m), vy = ld2(sb.f[1] + m), vz = ld2(sb.f[2] + m);
                 nex = nex - src_val(se, vx) * bx;
                 ney = ney - src_val(se, vy) * by;
                 nez = nez - src_val(se, vz) * bz;
@@ -254,7 +254,7 @@ __device__ __forceinline__ void e_column(const FusedArgs &A, const Geo &g, const
     const bool bz0 = g.bc[2] == '0';
     bool col_src = false;
     for (int q = 0; q < se.n; ++q) {
-        const SrcBlk &s = se.blk[q];
+        const SrcBlk s = src_blk(se.blk, q);
         if (k >= s.k0 && k < s.k0 + s.K && j >= s.j0 && j < s.j0 + s.J && s.i0 < z1 && s.i0 + s.I > z0)
             col_src = true;
     }
