@@ -103,11 +103,12 @@ def ellipse_design_eps(Nx, Ny, Nz, d, design=(240, 120, 10), n_ell=40, seed=2,
 
 def large_case(Nx=400, Ny=200, Nz=120, d=0.03, w=12, wl=1.31, seed=2, rtol=1e-4):
     """Single large FP64 solve: 400x200x120 at dx=0.03 um, 12-cell CPML,
-    240x120x10 design slab of 40 seeded ellipses, lambda 1.31 um, planar Ey
-    source across the input face (x index w + 2, interior y-z)."""
+    240x120x10 design slab of 40 seeded ellipses (128-vertex polygons under
+    the reference's overlap rule), lambda 1.31 um, planar Ey source across
+    the input face (x index w + 2, interior y-z)."""
     from emopt_amd.grid import ConstantMaterial3D
     X, Y, Z = span(Nx, d), span(Ny, d), span(Nz, d)
-    eps = ellipse_design_eps(Nx, Ny, Nz, d, seed=seed)
+    eps = ellipse_design_eps(Nx, Ny, Nz, d, seed=seed, overlap="exact")
     mu = ConstantMaterial3D(1.0)
     mats = [sample(eps if q < 3 else mu, Nx, Ny, Nz, q) for q in range(6)]
     ks = w + 2

@@ -106,11 +106,11 @@ def test_device_sampler_exact_rule_limits():
 
 
 def test_device_sampler_large_design_region():
-    """The 400x200x120 workload's material (40 ellipses in a 240x120x10
-    design slab over cladding): identical values at one offset; prints the
-    host and device times."""
+    """The 400x200x120 workload's material (40 ellipses as 128-gons in a
+    240x120x10 design slab over cladding, the reference's overlap rule):
+    identical values at one offset; prints the host and device times."""
     Nx, Ny, Nz, d = 400, 200, 120, 0.03
-    eps = ellipse_design_eps(Nx, Ny, Nz, d, seed=2)
+    eps = ellipse_design_eps(Nx, Ny, Nz, d, seed=2, overlap="exact")
     t = time.perf_counter()
     h = eps.get_values(0, Nx, 0, Ny, 0, Nz, sx=0.5, sy=0.0, sz=-0.5)
     th = time.perf_counter() - t
@@ -129,7 +129,7 @@ def test_fdtd_build_uses_device_sampler():
     from emopt_amd.fdtd import FDTD
     from emopt_amd.grid import ConstantMaterial3D
     Nx, Ny, Nz, d = 40, 30, 24, 0.05
-    eps = ellipse_design_eps(Nx, Ny, Nz, d, design=(24, 16, 6), n_ell=6, seed=4)
+    eps = ellipse_design_eps(Nx, Ny, Nz, d, design=(24, 16, 6), n_ell=6, seed=4, overlap="exact")
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         sim = FDTD(span(Nx, d), span(Ny, d), span(Nz, d), d, d, d, 1.55)
