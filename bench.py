@@ -526,6 +526,8 @@ def main():
         "config": {"workload": "single_large_400x200x120_dx0.03_cpml12_ellipses40_ey_plane",
                    "grid": [case["Nx"], case["Ny"], case["Nz"]], "cells": cells,
                    "cpml_cells": 12, "wavelength_um": case["wl"], "Tn": sim.Tn,
+                   "eps": "40 seeded ellipses as 128-gons in a 240x120x10 slab, the "
+                          "reference's overlap rule",
                    "bytes_per_cell_update": B_ALG_STEP,
                    "step_kernels": "k_pass_h + k_pass_e (two launches per step)"},
         "hbm_frac_168B": value * B_ALG_STEP / (PEAK_HBM_GBS * 1e9),
