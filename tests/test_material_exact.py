@@ -333,3 +333,33 @@ def test_limits_raise():
     m = material(prims, d)
     with pytest.raises(ValueError, match="more than 8 primitives"):
         m.get_values(0, 20, 0, 20, 0, 1, sx=0.3, sy=0.3, sz=0.5)
+
+
+def test_overlap_argument_and_device_records():
+    """The exact rule hands the device kernel rectangles and polygons only:
+    an ellipse goes as its inscribed polygon (kind 2, ellipse_vertices
+    vertices) and one clipped to nothing is left out, on the host too."""
+    with pytest.raises(ValueError):
+        StructuredMaterial3D(1, 1, 1, 0.1, 0.1, 0.1, overlap="boost")
+    m = StructuredMaterial3D(2.0, 2.0, 1.0, 0.1, 0.1, 0.1, ellipse_vertices=40)
+    m.add_primitive(Rectangle(1.0, 1.0, 0.5, 0.4, material_value=3.0), 0.0, 1.0)
+    m.add_primitive(Ellipse(1.0, 1.0, 0.3, 0.2, angle=0.2, material_value=5.0, layer=0), 0.2, 0.6)
+    m.add_primitive(Ellipse(1.0, 1.0, 0.3, 0.2, clip=(5.0, 6.0, 0.0, 1.0), material_value=7.0),
+                    0.33, 0.77)
+    recs, vx, vy, nv = m.device_primitives()
+    assert list(recs["kind"]) == [2, 0]             # priority order: layer 0 first
+    assert recs["nv"][0] == 40 and nv == 40
+    xs, ys = Ellipse(1.0, 1.0, 0.3, 0.2, angle=0.2).polygon(40)
+    assert np.array_equal(vx[:40], xs) and np.array_equal(vy[:40], ys)
+    assert tuple(recs["bbox"][0]) == (xs.min(), xs.max(), ys.min(), ys.max())
+    # on the host the clipped-away ellipse is left out too: the same doubles as
+    # the material built without it
+    m2 = StructuredMaterial3D(2.0, 2.0, 1.0, 0.1, 0.1, 0.1, ellipse_vertices=40)
+    m2.add_primitive(Rectangle(1.0, 1.0, 0.5, 0.4, material_value=3.0), 0.0, 1.0)
+    m2.add_primitive(Ellipse(1.0, 1.0, 0.3, 0.2, angle=0.2, material_value=5.0, layer=0), 0.2, 0.6)
+    for sz in (0.0, -0.5, 0.3):
+        assert np.array_equal(m.get_values(0, 20, 0, 20, 0, 10, sz=sz),
+                              m2.get_values(0, 20, 0, 20, 0, 10, sz=sz))
+    sm = StructuredMaterial3D(2.0, 2.0, 1.0, 0.1, 0.1, 0.1, overlap="supersample")
+    sm.add_primitive(Ellipse(1.0, 1.0, 0.3, 0.2, material_value=5.0), 0.0, 1.0)
+    assert list(sm.device_primitives()[0]["kind"]) == [1]
