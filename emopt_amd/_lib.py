@@ -81,6 +81,7 @@ _SIGS = {
     "emf_solve": ([_P, c_int, c_double, c_double, c_int64, c_int64, POINTER(SolveInfo)], c_int),
     "emf_get_history": ([_P, _P, c_int], c_int),
     "emf_get_field": ([_P, c_int, c_int] + [c_int] * 6 + [_P], c_int),
+    "emf_get_field_interp": ([_P, c_int, c_int] + [c_int] * 12 + [_P], c_int),
     "emf_time_steps": ([_P, c_int64, c_int64, c_int64, c_int, _P], c_int),
     "emf_set_tuning": ([_P, c_int, c_int, c_int, c_int], c_int),
     "emf_set_streaming": ([_P, c_int], c_int),

@@ -1529,6 +1529,49 @@ int emf_get_field(emf_solver *s, int slot, int comp, int k1, int k2, int j1, int
     return EMF_OK;
 }
 
+int emf_get_field_interp(emf_solver *s, int slot, int comp, int k1, int k2, int j1, int j2,
+                         int i1, int i2, int ek1, int ek2, int ej1, int ej2, int ei1, int ei2,
+                         double *out)
+{
+    LOCK(s);
+    if (slot < 0 || slot > 1 || !s->res[slot][0]) EMF_FAIL(EMF_ERR_STATE, "result slot %d empty", slot);
+    if (comp < 0 || comp > 5 || !out) EMF_FAIL(EMF_ERR_ARG, "bad component or null output");
+    if (ek1 < 0 || ej1 < 0 || ei1 < 0 || ek2 > s->Nx || ej2 > s->Ny || ei2 > s->Nz ||
+        k1 < ek1 || j1 < ej1 || i1 < ei1 || k2 > ek2 || j2 > ej2 || i2 > ei2 || k1 > k2 ||
+        j1 > j2 || i1 > i2)
+        EMF_FAIL(EMF_ERR_ARG, "box out of range (the domain must lie in the grown box, and it in "
+                 "the grid)");
+    const long long cnt = (long long)(k2 - k1) * (j2 - j1) * (i2 - i1);
+    if (cnt == 0) return EMF_OK;
+    // stencils and mirrors of emopt/fdtd.py:1472-1547, components Ex..Hz
+    static const int nst[6] = {4, 4, 1, 2, 2, 8};
+    static const int off[6][8][3] = {
+        {{0, 0, 0}, {0, 0, -1}, {1, 0, 0}, {1, 0, -1}},
+        {{0, 0, 0}, {0, -1, 0}, {1, 0, 0}, {1, -1, 0}},
+        {{0, 0, 0}},
+        {{0, 0, 0}, {0, -1, 0}},
+        {{0, 0, 0}, {0, 0, -1}},
+        {{0, 0, 0}, {0, 0, -1}, {0, -1, 0}, {0, -1, -1}, {1, 0, 0}, {1, 0, -1}, {1, -1, 0}, {1, -1, -1}}};
+    static const int mirx[6] = {1, 0, 0, 0, 1, 1}, miry[6] = {0, 1, 0, 1, 0, 1};
+    IStencil st{};
+    st.n = nst[comp];
+    for (int q = 0; q < st.n; ++q)
+        for (int a = 0; a < 3; ++a) st.d[q][a] = off[comp][q][a];
+    const char bx = s->bc[0], by = s->bc[1];
+    st.mx = mirx[comp] && ek1 == 0 && (bx == 'E' || bx == 'H');
+    st.my = miry[comp] && ej1 == 0 && (by == 'E' || by == 'H');
+    st.sx = bx == 'E' ? -1.0 : 1.0;
+    st.sy = by == 'E' ? -1.0 : 1.0;
+    const IBox dom{k1, k2, j1, j2, i1, i2}, ext{ek1, ek2, ej1, ej2, ei1, ei2};
+    CHECK(ensure_scratch(s, cnt));
+    hipLaunchKernelGGL(k_field_interp, dim3(grid1d(cnt, 256)), dim3(256), 0, s->stream,
+                       s->res[slot][comp], s->Nx, s->Ny, dom, ext, st, s->scratch);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, s->scratch, cnt * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EMF_OK;
+}
+
 int emf_calc_ydAx(emf_solver *s, const double *eps0_x, const double *eps0_y,
                   const double *eps0_z, int k1, int k2, int j1, int j2, int i1, int i2, double *out)
 {

@@ -152,6 +152,56 @@ __device__ __forceinline__ bool in_box(const SrcBlk &s, int i, int j, int k, lon
     return false;
 }
 
+// --------------------------------------------------------- field interpolation
+// get_field_interp (emopt/fdtd.py:1401-1553) of one component on the box dom:
+// each cell is the mean of the raw field over the component's stencil (offsets
+// in order, own value first, then one division), where the raw field is that of
+// the box ext -- dom grown by one cell as the reference grows it -- and zero
+// outside it, except that ghosts one cell below ext's x / y start, when ext
+// starts at 0 and the face has an 'E'/'H' symmetry, mirror the first cell with
+// sign sx / sy (x applied before y, as the reference fills its padded copy).
+struct IBox {
+    int k1, k2, j1, j2, i1, i2;
+};
+
+struct IStencil {
+    int n;                 // offsets used (1..8)
+    int d[8][3];           // (di, dj, dk)
+    int mx, my;            // mirror at the x / y min face
+    double sx, sy;         // their signs
+};
+
+__global__ void k_field_interp(const double2 *__restrict__ F, int Nx, int Ny, IBox dom, IBox ext,
+                               IStencil st, double2 *__restrict__ out)
+{
+    const int nk = dom.k2 - dom.k1, nj = dom.j2 - dom.j1;
+    const long long total = (long long)nk * nj * (dom.i2 - dom.i1);
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < total;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int k = dom.k1 + (int)(g % nk), j = dom.j1 + (int)((g / nk) % nj);
+        const int i = dom.i1 + (int)(g / ((long long)nk * nj));
+        double re = 0.0, im = 0.0;
+        for (int q = 0; q < st.n; ++q) {
+            const int a = i + st.d[q][0];
+            int b = j + st.d[q][1], c = k + st.d[q][2];
+            const bool gy = st.my && b == ext.j1 - 1, gx = st.mx && c == ext.k1 - 1;
+            if (gy) ++b;
+            if (gx) ++c;
+            double vr = 0.0, vi = 0.0;
+            if (a >= ext.i1 && a < ext.i2 && b >= ext.j1 && b < ext.j2 && c >= ext.k1 && c < ext.k2) {
+                const double2 v = F[((long long)a * Ny + b) * Nx + c];
+                vr = v.x;
+                vi = v.y;
+            }
+            if (gx) { vr = st.sx * vr; vi = st.sx * vi; }
+            if (gy) { vr = st.sy * vr; vi = st.sy * vi; }
+            if (q == 0) { re = vr; im = vi; }
+            else { re += vr; im += vi; }
+        }
+        out[g] = make_double2(re / (double)st.n, im / (double)st.n);
+    }
+}
+
 // ------------------------------------------------------ convergence check
 //
 // One check of emopt/fdtd.py:1072-1144 for all nconv sample points, run by a

@@ -449,6 +449,18 @@ class FDTD(MaxwellSolver):
                                     i1, i2, ptr(out))
         return out
 
+    def _interp_device(self, component, dom, ext, slot):
+        """get_field_interp's averaging on the GPU (emf_get_field_interp):
+        dom, ext = (k1, k2, j1, j2, i1, i2) of the domain and of the grown box."""
+        if not self._have[slot]:
+            raise RuntimeError("no %s solution available" % ("forward", "adjoint")[slot])
+        k1, k2, j1, j2, i1, i2 = dom
+        out = np.zeros((i2 - i1, j2 - j1, k2 - k1), dtype=np.complex128)
+        if out.size:
+            self._lib.emf_get_field_interp(self._h, slot, FIELD_INDEX[component], *dom, *ext,
+                                           ptr(out))
+        return out
+
     def get_field(self, component, domain=None):
         """Raw (uninterpolated) forward field (emopt/fdtd.py:1322-1380)."""
         return self._get(component, domain, 0)

@@ -2,8 +2,10 @@
 the Ez grid and the Poynting source power.
 
 Semantics of emopt/fdtd.py:1401-1553 (``get_field_interp``) and
-emopt/fdtd.py:1555-1667 (``get_source_power``).  The solver copies back only
-the (one-cell expanded) domain that is asked for, not the whole array.
+emopt/fdtd.py:1555-1667 (``get_source_power``).  On a solver the averaging
+runs on its GPU (emf_get_field_interp) for domains inside the grid, and only
+the domain comes back; the numpy restatement below serves other domains and
+solver stand-ins, and is what the device result is tested against.
 """
 from __future__ import annotations
 
@@ -68,6 +70,13 @@ def interp_to_ez(sim, component, domain=None, squeeze=False, adjoint=False):
         if i2 < Nz - 1: i2 += 1
         dint = DomainCoordinates(k1 * dx, k2 * dx, j1 * dy, j2 * dy, i1 * dz, i2 * dz, dx, dy, dz)
     k1, k2, j1, j2, i1, i2 = dint.k1, dint.k2, dint.j1, dint.j2, dint.i1, dint.i2
+    dev = getattr(sim, "_interp_device", None)
+    if dev is not None and 0 <= k1 <= domain.k1 <= domain.k2 <= k2 <= Nx and \
+            0 <= j1 <= domain.j1 <= domain.j2 <= j2 <= Ny and 0 <= i1 <= domain.i1 <= domain.i2 <= i2 <= Nz:
+        # on the solver's GPU: the same stencil sums on the same grown box
+        f = dev(component, (domain.k1, domain.k2, domain.j1, domain.j2, domain.i1, domain.i2),
+                (k1, k2, j1, j2, i1, i2), 1 if adjoint else 0)
+        return np.squeeze(f) if squeeze else f
     fr = np.pad(get(component, dint), 1, "constant", constant_values=0)
     bc = sim.bc
     for ax in _MIRROR[component]:

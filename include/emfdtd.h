@@ -232,6 +232,19 @@ EMF_API int emf_get_history(emf_solver *s, double *out, int cap);
 EMF_API int emf_get_field(emf_solver *s, int slot, int comp, int k1, int k2, int j1, int j2,
                   int i1, int i2, double *out);
 
+/* Component `comp` of result slot `slot` interpolated onto the Ez grid on the
+ * box [k1,k2)x[j1,j2)x[i1,i2) (get_field_interp, emopt/fdtd.py:1401-1553):
+ * each cell is the mean of the raw field over the component's stencil
+ * (:1472-1547), the raw field being that of the grown box [ek1,ek2)x... --
+ * the domain grown by one cell where the reference grows it (:1443-1457) --
+ * and zero outside it; with an 'E'/'H' symmetry at the x / y min face and the
+ * grown box starting there, the ghost below mirrors the first cell.
+ * The domain must lie in the grown box and the grown box in the grid.  out:
+ * complex [i2-i1][j2-j1][k2-k1] in host memory. */
+EMF_API int emf_get_field_interp(emf_solver *s, int slot, int comp, int k1, int k2, int j1,
+                                 int j2, int i1, int i2, int ek1, int ek2, int ej1, int ej2,
+                                 int ei1, int ei2, double *out);
+
 /* ---- measurement (no counterpart in the reference) --------------------- */
 
 /* Runs `nsteps` time steps starting at step index n_start on the current
