@@ -14,7 +14,8 @@ that runs them synchronises its stream before it returns.
 
 Also reported:
   roofline      the dominant kernel's algorithmic bytes per launch divided by
-                its average launch time (HIP event pairs on the solver's own
+                its average launch time (each launch's own start and stop HIP
+                events, recorded by hipExtLaunchKernelGGL on the solver's
                 stream, in a second run of min(K, 200) steps after the timed
                 region), against 8 TB/s; `traffic` is the HBM bytes per launch
                 measured by rocprofv3 counters (FETCH_SIZE, WRITE_SIZE, scaled

@@ -7,6 +7,7 @@
 // convergence state and two complex result slots (forward, adjoint).  All
 // work is queued on the handle's own HIP stream (default priority).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <chrono>
@@ -136,6 +137,9 @@ struct emf_solver {
     std::mutex mtx;
     int dev = 0;
     hipStream_t stream = nullptr;
+    // when set, the next two-pass launch records its own start and stop time
+    // in these events (the dispatch's timestamps, no marker packets around it)
+    hipEvent_t tev0 = nullptr, tev1 = nullptr;
 
     // geometry / time
     int Nx = 0, Ny = 0, Nz = 0;
@@ -659,6 +663,16 @@ int pass_ty(const emf_solver *s)
     return s->tyg == 1 || s->tyg == 2 || s->tyg == 8 ? s->tyg : 4;
 }
 
+// a two-pass launch, through hipExtLaunchKernelGGL with the solver's timing
+// events when they are set
+#define PASS_LAUNCH(K, GR, BLK, ...)                                                         \
+    do {                                                                                     \
+        if (s->tev0)                                                                         \
+            hipExtLaunchKernelGGL(K, GR, BLK, 0, s->stream, s->tev0, s->tev1, 0, __VA_ARGS__); \
+        else                                                                                 \
+            hipLaunchKernelGGL(K, GR, BLK, 0, s->stream, __VA_ARGS__);                       \
+    } while (0)
+
 dim3 pass_grid(const emf_solver *s)
 {
     const int ty = pass_ty(s);
@@ -671,10 +685,10 @@ void launch_h_occ(const emf_solver *s, const FusedArgs &A, const Geo &g, const P
 {
     const dim3 gr = pass_grid(s);
     switch (pass_ty(s)) {
-    case 1: hipLaunchKernelGGL((k_pass_h<MU, 1, OCC, NT>), gr, dim3(PX, 1), 0, s->stream, A, g, ps, done); break;
-    case 2: hipLaunchKernelGGL((k_pass_h<MU, 2, OCC, NT>), gr, dim3(PX, 2), 0, s->stream, A, g, ps, done); break;
-    case 8: hipLaunchKernelGGL((k_pass_h<MU, 8, OCC, NT>), gr, dim3(PX, 8), 0, s->stream, A, g, ps, done); break;
-    default: hipLaunchKernelGGL((k_pass_h<MU, 4, OCC, NT>), gr, dim3(PX, 4), 0, s->stream, A, g, ps, done);
+    case 1: PASS_LAUNCH((k_pass_h<MU, 1, OCC, NT>), gr, dim3(PX, 1), A, g, ps, done); break;
+    case 2: PASS_LAUNCH((k_pass_h<MU, 2, OCC, NT>), gr, dim3(PX, 2), A, g, ps, done); break;
+    case 8: PASS_LAUNCH((k_pass_h<MU, 8, OCC, NT>), gr, dim3(PX, 8), A, g, ps, done); break;
+    default: PASS_LAUNCH((k_pass_h<MU, 4, OCC, NT>), gr, dim3(PX, 4), A, g, ps, done);
     }
 }
 
@@ -684,10 +698,10 @@ void launch_e_zu(const emf_solver *s, const FusedArgs &A, const Geo &g, const Pa
 {
     const dim3 gr = pass_grid(s);
     switch (pass_ty(s)) {
-    case 1: hipLaunchKernelGGL((k_pass_e<CM, 1, OCC, ZU, NT>), gr, dim3(PX, 1), 0, s->stream, A, g, ps, done); break;
-    case 2: hipLaunchKernelGGL((k_pass_e<CM, 2, OCC, ZU, NT>), gr, dim3(PX, 2), 0, s->stream, A, g, ps, done); break;
-    case 8: hipLaunchKernelGGL((k_pass_e<CM, 8, OCC, ZU, NT>), gr, dim3(PX, 8), 0, s->stream, A, g, ps, done); break;
-    default: hipLaunchKernelGGL((k_pass_e<CM, 4, OCC, ZU, NT>), gr, dim3(PX, 4), 0, s->stream, A, g, ps, done);
+    case 1: PASS_LAUNCH((k_pass_e<CM, 1, OCC, ZU, NT>), gr, dim3(PX, 1), A, g, ps, done); break;
+    case 2: PASS_LAUNCH((k_pass_e<CM, 2, OCC, ZU, NT>), gr, dim3(PX, 2), A, g, ps, done); break;
+    case 8: PASS_LAUNCH((k_pass_e<CM, 8, OCC, ZU, NT>), gr, dim3(PX, 8), A, g, ps, done); break;
+    default: PASS_LAUNCH((k_pass_e<CM, 4, OCC, ZU, NT>), gr, dim3(PX, 4), A, g, ps, done);
     }
 }
 
@@ -1701,9 +1715,12 @@ int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check
     CHECK(check_ready(s));
     if (check_every > 0) CHECK(build_samples(s));
     if (fused(s)) CHECK(ensure_alt(s));
-    // ev[0] start, ev[1] end; with per_kernel, ev[2+2q] / ev[3+2q] bracket the
-    // H pass (or the fused launch) and the E pass of step q.
-    std::vector<hipEvent_t> ev(per_kernel ? 2 * nsteps + 3 : 2);
+    // ev[0] start, ev[1] end.  With per_kernel, two passes: each launch of step q
+    // records its own start and stop (hipExtLaunchKernelGGL), H in ev[2+4q],
+    // ev[3+4q] and E in ev[4+4q], ev[5+4q]; fused kinds: ev[2+2q] / ev[3+2q]
+    // bracket the launch of step q.
+    const bool ext = per_kernel && !fused(s);
+    std::vector<hipEvent_t> ev(ext ? 4 * nsteps + 2 : (per_kernel ? 2 * nsteps + 3 : 2));
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipMemsetAsync(s->conv, 0, sizeof(ConvState), s->stream));
     HIPCHK(hipEventRecord(ev[0], s->stream));
@@ -1715,16 +1732,22 @@ int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check
     };
     for (int64_t q = 0; q < nsteps && rc == EMF_OK; ++q) {
         long long n = n_start + q;
-        mark(2 + 2 * q);
-        if (fused(s)) {
+        if (ext) {
+            s->tev0 = ev[2 + 4 * q];
+            s->tev1 = ev[3 + 4 * q];
+            rc = launch_h(s, (double)n * s->dt, nullptr);
+            s->tev0 = ev[4 + 4 * q];
+            s->tev1 = ev[5 + 4 * q];
+            if (rc == EMF_OK) rc = launch_e(s, ((double)n + 0.5) * s->dt, nullptr);
+            s->tev0 = s->tev1 = nullptr;
+        } else if (fused(s)) {
+            mark(2 + 2 * q);
             rc = launch_fused(s, n, from_set(s, n, n_start, start), nullptr);
             mark(3 + 2 * q);
         } else {
             rc = launch_h(s, (double)n * s->dt, nullptr);
-            mark(3 + 2 * q);
             if (rc == EMF_OK) rc = launch_e(s, ((double)n + 0.5) * s->dt, nullptr);
         }
-        mark(4 + 2 * q);
         if (rc == EMF_OK && check_every > 0 && (q + 1) % check_every == 0) {
             const int ps = from_set(s, n + 1, n_start, start);
             hipLaunchKernelGGL(k_conv_check, dim3(1), dim3(64), 0, s->stream, s->Eb[ps][0],
@@ -1741,9 +1764,12 @@ int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check
         float ms;
         (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
         tot = ms;
-        for (int64_t q = 0; per_kernel && q < nsteps; ++q) {
+        for (int64_t q = 0; ext && q < nsteps; ++q) {
+            (void)hipEventElapsedTime(&ms, ev[2 + 4 * q], ev[3 + 4 * q]); th += ms;
+            (void)hipEventElapsedTime(&ms, ev[4 + 4 * q], ev[5 + 4 * q]); te += ms;
+        }
+        for (int64_t q = 0; per_kernel && !ext && q < nsteps; ++q) {
             (void)hipEventElapsedTime(&ms, ev[2 + 2 * q], ev[3 + 2 * q]); th += ms;
-            (void)hipEventElapsedTime(&ms, ev[3 + 2 * q], ev[4 + 2 * q]); te += ms;
         }
     }
     for (auto &e : ev) (void)hipEventDestroy(e);

@@ -251,9 +251,12 @@ EMF_API int emf_get_field_interp(emf_solver *s, int slot, int comp, int k1, int 
  * state (sources, CPML and fields as they are), with a convergence check
  * every `check_every` steps (0 = none), exactly as emf_solve's loop does, and
  * times them with HIP events on the handle's stream.  out[0] = total device
- * ms of the region.  With per_kernel != 0 an event pair also brackets every
- * launch: out[1] = summed ms of the H-pass (or fused-step) launches, out[2] =
- * summed ms of the E-pass launches; otherwise both are 0. */
+ * ms of the region.  With per_kernel != 0 every launch is timed as well: each
+ * H- and E-pass launch records its own start and stop event through
+ * hipExtLaunchKernelGGL (the dispatch's own timestamps, no marker packets
+ * between the kernels), and an event pair brackets each fused-step launch.
+ * out[1] = summed ms of the H-pass (or fused-step) launches, out[2] = summed
+ * ms of the E-pass launches (0 for fused kinds); otherwise both are 0. */
 EMF_API int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64_t check_every,
                            int per_kernel, double *out);
 
