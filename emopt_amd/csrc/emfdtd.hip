@@ -1916,23 +1916,27 @@ int emf_copy_bandwidth(int device, int64_t bytes, int reps, double *gbps)
     CHECK(dalloc(&a, n));
     if (dalloc(&b, n) != EMF_OK) { dfree(a); return EMF_ERR_NOMEM; }
     hipStream_t st;
-    hipEvent_t e0, e1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = EMF_OK;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) { dfree(a); dfree(b); EMF_FAIL(EMF_ERR_HIP, "stream"); }
-    hipEventCreate(&e0); hipEventCreate(&e1);
-    hipMemsetAsync(a, 0, n * sizeof(double2), st);
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+        hipMemsetAsync(a, 0, n * sizeof(double2), st) != hipSuccess)
+        rc = EMF_ERR_HIP;
     std::vector<float> t;
     const unsigned grid = (unsigned)((n + 255) / 256);
     for (int r = 0; r < reps + 1 && rc == EMF_OK; ++r) {
-        hipEventRecord(e0, st);
+        if (hipEventRecord(e0, st) != hipSuccess) { rc = EMF_ERR_HIP; break; }
         hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, st, a, b, n);
-        hipEventRecord(e1, st);
-        if (hipEventSynchronize(e1) != hipSuccess) rc = EMF_ERR_HIP;
         float ms = 0;
-        hipEventElapsedTime(&ms, e0, e1);
-        if (r > 0) t.push_back(ms);   // first launch is a warm-up
+        if (hipGetLastError() != hipSuccess || hipEventRecord(e1, st) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = EMF_ERR_HIP;
+        else if (r > 0)
+            t.push_back(ms);   // first launch is a warm-up
     }
-    hipEventDestroy(e0); hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(st);
     dfree(a); dfree(b);
     if (rc != EMF_OK) EMF_FAIL(rc, "copy kernel failed");
