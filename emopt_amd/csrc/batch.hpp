@@ -296,10 +296,11 @@ __global__ void k_batch_tick(const BTable *__restrict__ T, const BMember *__rest
                     reinterpret_cast<const char *>(B.A.Eo) + 2 * (size_t)B.A.cs);
                 conv_check(Ex, Ey, Ez, B.samples, B.nsamples, (double)(nd - 1) * B.dt,
                            B.amp_rtol, B.phi_rtol, nd - 1, B.conv, B.hist, B.hist_cap);
-                st.nchecks = B.conv->nchecks;
-                st.a_change = B.conv->a_change;
-                st.phi_change = B.conv->phi_change;
-                if (B.conv->done) { fin = true; st.status = 0; }
+                const auto *cv = gptr(B.conv);
+                st.nchecks = cv->nchecks;
+                st.a_change = cv->a_change;
+                st.phi_change = cv->phi_change;
+                if (cv->done) { fin = true; st.status = 0; }
             }
             if (!fin && nd >= B.n_stop) { fin = true; st.status = 1; }
         }
@@ -308,8 +309,8 @@ __global__ void k_batch_tick(const BTable *__restrict__ T, const BMember *__rest
         if (nd == st.n0 + B.Tn) { st.phase = BP_RUN2; mode = BM_CAP1; }
     } else if (st.phase == BP_RUN2) {
         if (nd == st.n0 + 2 * B.Tn) {
-            const double *f0 = B.fac + st.n0 * NFAC, *f1 = B.fac + (st.n0 + B.Tn) * NFAC,
-                         *f2 = B.fac + (st.n0 + 2 * B.Tn) * NFAC;
+            const auto *f0 = gptr(B.fac + st.n0 * NFAC), *f1 = gptr(B.fac + (st.n0 + B.Tn) * NFAC),
+                       *f2 = gptr(B.fac + (st.n0 + 2 * B.Tn) * NFAC);
             st.te.s21 = f2[0] - f1[0]; st.te.s10 = f1[0] - f0[0];
             st.te.c10 = f1[1] - f0[1]; st.te.c21 = f2[1] - f1[1];
             st.th.s21 = f2[6] - f1[6]; st.th.s10 = f1[6] - f0[6];

@@ -239,37 +239,42 @@ __device__ inline int conv_check(const double *__restrict__ Ex, const double *__
                                  double amp_rtol, double phi_rtol, long long nstep, ConvState *st,
                                  double *hist, int hist_cap)
 {
-    st->t0 = st->t1; st->t1 = st->t2; st->t2 = tnow;
-    const double t0 = st->t0, t1 = st->t1, t2 = st->t2;
+    // all through the global address space (the pointers may come from memory)
+    typedef __attribute__((address_space(1))) ConvState gconv;
+    gconv *const S = (gconv *)st;
+    S->t0 = S->t1; S->t1 = S->t2; S->t2 = tnow;
+    const double t0 = S->t0, t1 = S->t1, t2 = S->t2;
     const double *E[3] = { Ex, Ey, Ez };
     double sa = 0, s0 = 0, sp = 0, sp0 = 0;
     for (int q = 0; q < nc; ++q) {
         double ph[3], am[3];
         for (int c = 0; c < 3; ++c) {
-            st->eh[0][c][q] = st->eh[1][c][q];
-            st->eh[1][c][q] = st->eh[2][c][q];
-            st->eh[2][c][q] = sample[q] >= 0 ? E[c][sample[q]] : 0.0;
-            const double f0 = st->eh[0][c][q], f1 = st->eh[1][c][q], f2 = st->eh[2][c][q];
+            S->eh[0][c][q] = S->eh[1][c][q];
+            S->eh[1][c][q] = S->eh[2][c][q];
+            const long long at = *gptr(sample + q);
+            S->eh[2][c][q] = at >= 0 ? *gptr(E[c] + at) : 0.0;
+            const double f0 = S->eh[0][c][q], f1 = S->eh[1][c][q], f2 = S->eh[2][c][q];
             ph[c] = phase3(t0, t1, t2, f0, f1, f2);
             am[c] = amp3(t0, t1, t2, f0, f1, f2, ph[c]);
         }
         for (int c = 0; c < 3; ++c)
             if (am[c] < 0) { am[c] *= -1; ph[c] += M_PI; }
-        const double phi0 = st->phi1[q], a0 = st->a1[q];
+        const double phi0 = S->phi1[q], a0 = S->a1[q];
         const double phi1 = ph[0] + ph[1] + ph[2];
         const double a1 = am[0] + am[1] + am[2];
-        st->phi1[q] = phi1; st->a1[q] = a1;
+        S->phi1[q] = phi1; S->a1[q] = a1;
         sa += (a1 - a0) * (a1 - a0); s0 += a0 * a0;
         sp += (phi1 - phi0) * (phi1 - phi0); sp0 += phi0 * phi0;
     }
     const double ac = sqrt(sa) / sqrt(s0), pc = sqrt(sp) / sqrt(sp0);
-    st->a_change = ac; st->phi_change = pc;
-    const int h = st->nchecks++;
+    S->a_change = ac; S->phi_change = pc;
+    const int h = S->nchecks++;
     if (hist && h < hist_cap) {
-        hist[3 * h] = (double)nstep; hist[3 * h + 1] = ac; hist[3 * h + 2] = pc;
+        *gptr(hist + 3 * h) = (double)nstep; *gptr(hist + 3 * h + 1) = ac;
+        *gptr(hist + 3 * h + 2) = pc;
     }
     const int conv = (ac <= amp_rtol) && (pc <= phi_rtol);
-    st->done = conv;
+    S->done = conv;
     return conv;
 }
 
