@@ -159,6 +159,29 @@ def cpu_all_cores(case, seconds):
                        % (steps, case["Nx"], case["Ny"], case["Nz"], el, threads, _cpu_model()))
 
 
+def error_vs_reference(case, sim, n0):
+    """Relative L2 error per recovered complex component (Ex..Hz) of the
+    workload's solve at a fixed step count n0, against the oracle (the C
+    restatement of src/fdtd.cpp, OpenMP build, bit-identical per cell to the
+    serial build that the compiled reference kernel pins) on the host."""
+    from tests.cases import all_fields, rel_l2, run_oracle
+    sim.fixed_n0 = n0
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim._register(sim._sources)
+        sim._solve(0)
+    sim.fixed_n0 = None
+    got = all_fields(sim, case)
+    ora = run_oracle(case, fixed_n0=n0, lib="omp")
+    errs = [rel_l2(a, b) for a, b in zip(got, ora["fields"])]
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    return {"n0": int(n0), "Tn": int(ora["Tn"]), "steps": int(n0 + 2 * ora["Tn"]),
+            "rel_l2": dict(zip(("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"), errs)),
+            "max_rel_l2": max(errs), "tolerance": 1e-6,
+            "oracle": "oracle/cfdtd.c (OpenMP, %d threads) under the restated Python loop, "
+                      "%.1f s" % (threads, ora["seconds"])}
+
+
 def read_traffic(path=TRAFFIC_FILE):
     """Per-launch HBM bytes recorded by the counter runs (see --profile-summary)."""
     out = {}
@@ -453,6 +476,9 @@ def main():
     ap.add_argument("--omp-seconds", type=float, default=6.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--error-periods", type=int, default=28,
+                    help="compare the workload's phasors with the oracle's at n0 = this many "
+                         "Tn (0: skip); 28 Tn is past the source ramp")
     ap.add_argument("--batch", type=int, default=32, help="solves in the batched section (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="arrival window of the serving section (0: skip)")
@@ -552,6 +578,13 @@ def main():
                         "seconds": el, "solves_per_hr": 3600.0 / el,
                         "cell_updates_per_s": cells * ls["steps"] / el}
         log("[bench] full solve: %d steps in %.2fs (%s)" % (ls["steps"], el, ls["status"]))
+
+    if args.error_periods > 0:
+        t = time.perf_counter()
+        res["error_vs_reference"] = error_vs_reference(case, sim, args.error_periods * sim.Tn)
+        ev = res["error_vs_reference"]
+        log("[bench] error vs oracle at n0 = %d: max rel L2 %.2e (%.1fs)"
+            % (ev["n0"], ev["max_rel_l2"], time.perf_counter() - t))
 
     if not args.no_solve:
         res["materials"] = bench_materials()

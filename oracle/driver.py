@@ -544,3 +544,30 @@ class OracleFDTD:
         """The six recovered complex components, each shaped [Nz,Ny,Nx]."""
         sets = self.adj if adjoint else self.fwd
         return [a.reshape(self.Nz, self.Ny, self.Nx).copy() for a in sets[0]]
+
+
+def holder(Nx, Ny, Nz, dx, dy, dz, fwd, adj=None, mats=None, bc="000", w_pml=(0.0,) * 6):
+    """An OracleFDTD with no library behind it, holding given complex phasors
+    (six [Nz,Ny,Nx] arrays, forward and optionally adjoint) and material
+    arrays, so that the post-processing restatements above
+    (get_field_interp, get_source_power, get_A_diag, calc_ydAx) can check
+    fields computed elsewhere."""
+    o = OracleFDTD.__new__(OracleFDTD)
+    o.lib = None
+    o.Nx, o.Ny, o.Nz = Nx, Ny, Nz
+    o.I, o.J, o.K = Nz, Ny, Nx
+    o._dx, o._dy, o._dz = dx, dy, dz
+    o._X, o._Y, o._Z = dx * (Nx - 1), dy * (Ny - 1), dz * (Nz - 1)
+    o._bc = bc
+    o._w_pml = list(w_pml)
+    n = Nx * Ny * Nz
+
+    def flat(arrs):
+        if arrs is None:
+            return [np.zeros(n, dtype=np.complex128) for _ in range(6)]
+        return [np.ascontiguousarray(a, dtype=np.complex128).ravel().copy() for a in arrs]
+    o.fwd = [flat(fwd), flat(None)]
+    o.adj = [flat(adj), flat(None)]
+    o.mats = flat(mats) if mats is not None else [np.ones(n, dtype=np.complex128)
+                                                  for _ in range(6)]
+    return o

@@ -178,19 +178,28 @@ def run_gpu(case, fixed_n0=None, device=0):
     return out
 
 
-def run_oracle(case, fixed_n0=None, lib="oracle", max_steps=None):
+def run_oracle(case, fixed_n0=None, lib="oracle", max_steps=None, adjoint=False):
+    """The oracle's solve of `case`.  lib: "oracle" (serial C restatement),
+    "omp" (the same built with OpenMP, bit-identical per cell) or "ref"
+    (EMopt's own src/fdtd.cpp compiled from its sources).  adjoint=True
+    solves with case["adj_sources"] through solve_adjoint."""
     import warnings
     from oracle import driver as D
-    path = D.ORACLE_LIB if lib == "oracle" else D.REF_LIB
+    path = {"oracle": D.ORACLE_LIB, "omp": D.ORACLE_OMP_LIB, "ref": D.REF_LIB}[lib]
     d = case["d"]
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         o = D.OracleFDTD(D.load(path), case["X"], case["Y"], case["Z"], d, d, d, case["wl"],
                          rtol=case["rtol"], complex_eps=case.get("ceps", False))
         _configure(o, case, D.Domain)
+        for arrs, (k1, k2, j1, j2, i1, i2) in case.get("adj_sources", []):
+            o.add_adjoint_sources(arrs, D.Domain.from_indices(k1, k2, j1, j2, i1, i2))
         o.set_material_arrays(case["mats"][:3], case["mats"][3:])
-        o.solve_forward(fixed_n0=fixed_n0, max_steps=max_steps)
-    return dict(n0=o.n0, Tn=o.Tn, fields=o.phasors(), history=np.array(o.history),
+        if adjoint:
+            o.solve_adjoint(fixed_n0=fixed_n0, max_steps=max_steps)
+        else:
+            o.solve_forward(fixed_n0=fixed_n0, max_steps=max_steps)
+    return dict(n0=o.n0, Tn=o.Tn, fields=o.phasors(adjoint), history=np.array(o.history),
                 seconds=o.solve_seconds, status=o.status, sim=o)
 
 
@@ -401,16 +410,18 @@ def grating_problem(Nx=360, Ny=60, Nz=120, d=0.03, ntr=32, w=10, wl=1.55, seed=4
 
 
 def serve_case(q, seed=5, lo=(64, 64, 64), hi=(320, 240, 120), n_ell=12, w=10, rtol=1e-4,
-               materials=True):
+               materials=True, s=None):
     """Request q of the serving workload: grid dims log-uniform between `lo`
     and `hi` (one shared log factor), d uniform in 0.03-0.05 um, lambda
     uniform in 1.26-1.62 um, a slab of random ellipses (eps 3.48^2 in
     1.444^2) across the middle, CPML `w` cells, planar Ey source across the
     input face as in the single large workload.  materials=False skips the
-    (host-heavy) material sampling."""
+    (host-heavy) material sampling; s in [0, 1] overrides the drawn log
+    factor (0 gives `lo`, 1 gives `hi`)."""
     from emopt_amd.grid import ConstantMaterial3D
     rng = np.random.default_rng([seed, q])
-    s = rng.uniform()
+    s_drawn = rng.uniform()
+    s = s_drawn if s is None else float(s)
     Nx, Ny, Nz = (int(round(a * (b / a) ** s)) for a, b in zip(lo, hi))
     d = float(rng.uniform(0.03, 0.05))
     wl = float(rng.uniform(1.26, 1.62))
