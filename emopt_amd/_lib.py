@@ -53,6 +53,7 @@ _SIGS = {
     "emf_set_complex_eps": ([_P, c_int], c_int),
     "emf_set_mat_arrays": ([_P] + [_P] * 6, c_int),
     "emf_update_eps_box": ([_P, c_int, _P] + [c_int] * 6, c_int),
+    "emf_update_mu_box": ([_P, c_int, _P] + [c_int] * 6, c_int),
     "emf_get_mat_arrays": ([_P] + [_P] * 6, c_int),
     "emf_set_compression": ([_P, c_int], c_int),
     "emf_set_bc": ([_P, c_char_p], c_int),
@@ -89,7 +90,7 @@ _SIGS = {
                                c_int, c_double, c_double, c_int, c_int, c_int, c_int, c_int, c_int,
                                c_double, c_double, c_double, _P], c_int),
     "emf_copy_bandwidth": ([c_int, c_int64, c_int, _P], c_int),
-    "emf_calc_ydAx": ([_P, _P, _P, _P] + [c_int] * 6 + [_P], c_int),
+    "emf_calc_ydAx": ([_P] + [_P] * 6 + [c_int] * 6 + [_P], c_int),
     "emf_grad_assemble": ([_P, c_int64, _P, _P, _P, _P, c_int, c_double, _P], c_int),
     "emf_pml_stretch": ([_P, c_int64, _P, _P, _P], c_int),
     "emf_batch_new": ([c_int, POINTER(_P)], c_int),

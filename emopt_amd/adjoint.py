@@ -27,6 +27,8 @@ fill fractions of moving rectangle edges).
 """
 from __future__ import annotations
 
+import warnings
+
 import numpy as np
 
 from .misc import DomainCoordinates
@@ -277,6 +279,7 @@ class AdjointMethod:
     def __init__(self, sim, step=1e-8):
         self.sim = sim
         self._step = step
+        self.prev_params = []          # params of the last forward solve (fom)
 
     # --- user hooks (emopt/adjoint_method.py:130-260)
     def update_system(self, params):
@@ -302,10 +305,19 @@ class AdjointMethod:
 
     # --- the solves
     def fom(self, params):
+        """Forward solve and FOM (emopt/adjoint_method.py:418-447).  The
+        params are recorded (as a copy: the reference keeps the caller's
+        array itself, so an in-place change of it went unnoticed) for the
+        forward-solve guard of gradient."""
         self.update_system(params)
+        self.prev_params = np.array(params, dtype=np.double, copy=True)
         self.sim.update()
         self.sim.solve_forward()
         return self.calc_fom(self.sim, params)
+
+    def _forward_current(self, params):
+        """True when the last forward solve was at exactly these params."""
+        return np.array_equal(self.prev_params, np.asarray(params, dtype=np.double))
 
     def solve_adjoint_for(self, params):
         src = self.calc_dFdx(self.sim, params)
@@ -329,7 +341,14 @@ class AdjointMethod:
             self.update_system(params)
             for box in ubs:
                 sim.update(box)
-            grad[i] = -2 * np.real(sim.calc_ydAx(Ai) / step)
+            # A - A0 vanishes outside the update boxes: sum over their hull
+            idx = [sim.index_box(box) for box in ubs]
+            hull = (min(b[0] for b in idx), max(b[1] for b in idx), min(b[2] for b in idx),
+                    max(b[3] for b in idx), min(b[4] for b in idx), max(b[5] for b in idx))
+            if hull[0] >= hull[1] or hull[2] >= hull[3] or hull[4] >= hull[5]:
+                grad[i] = 0.0
+            else:
+                grad[i] = -2 * np.real(sim.calc_ydAx(Ai, box=hull) / step)
             params[i] = p0
             self.update_system(params)
             for box in ubs:
@@ -337,10 +356,71 @@ class AdjointMethod:
         return grad
 
     def gradient(self, params):
-        """Forward solve, adjoint solve, reference gradient loop."""
-        self.fom(params)
+        """Forward solve (skipped when the last one was at these params),
+        adjoint solve, reference gradient loop (emopt/adjoint_method.py:
+        548-606)."""
+        self.update_system(params)
+        self.sim.update()
+        if not self._forward_current(params):
+            self.prev_params = np.array(params, dtype=np.double, copy=True)
+            self.sim.solve_forward()
         self.solve_adjoint_for(params)
         return self.calc_gradient(self.sim, params) + self.calc_grad_p(self.sim, params)
+
+    def check_gradient(self, params, indices=[], plot=True, verbose=True,
+                       return_gradients=False, fd_step=1e-10):
+        """Adjoint gradient against one-sided finite differences
+        (emopt/adjoint_method.py:608-721): F(p_i + fd_step) - F(p) over
+        fd_step for each index in `indices` (all when empty).  Returns the
+        total relative error ||g_fd - g_am|| / ||g_fd|| (and, with
+        return_gradients, also g_fd and the full adjoint gradient); warns
+        when it is over 1 %.  plot draws the three bar charts when
+        matplotlib is importable."""
+        params = np.array(params, dtype=np.double)
+        if len(indices) == 0:
+            indices = np.arange(0, len(params), 1)
+        indices = np.asarray(indices, dtype=int)
+        self.update_system(params)
+        self.sim.update()
+        grad_am = self.gradient(params)
+        grad_fd = np.zeros(len(indices))
+        fom0 = self.fom(params)
+        if verbose:
+            print("Checking gradient...")
+        for i, j in enumerate(indices):
+            if verbose:
+                print("\tDerivative %d of %d" % (i + 1, len(indices)))
+            p0 = params[j]
+            params[j] += fd_step
+            fom1 = self.fom(params)
+            grad_fd[i] = (fom1 - fom0) / fd_step
+            params[j] = p0
+        errors = np.abs(grad_fd - grad_am[indices]) / np.abs(grad_fd)
+        error_tot = np.linalg.norm(grad_fd - grad_am[indices]) / np.linalg.norm(grad_fd)
+        if error_tot < 0.01:
+            if verbose:
+                print("The total error in the gradient is %0.4E" % error_tot)
+        else:
+            warnings.warn("The total error in the gradient is %0.4E which is over 1%%" % error_tot)
+        if plot:
+            try:
+                import matplotlib.pyplot as plt
+            except ImportError:
+                warnings.warn("matplotlib is not available: no gradient plot")
+            else:
+                f = plt.figure()
+                axs = [f.add_subplot(311), f.add_subplot(312), f.add_subplot(313)]
+                xs = np.arange(len(indices))
+                for ax, v, title in zip(axs, (grad_fd, grad_am[indices], errors),
+                                        ("Finite Differences", "Adjoint Method",
+                                         "Error in Adjoint Method")):
+                    ax.bar(xs, v)
+                    ax.set_title(title)
+                axs[2].set_yscale("log")
+                plt.show()
+        if return_gradients:
+            return error_tot, grad_fd, grad_am
+        return error_tot
 
     def gradient_device(self, params, fom_done=False, exact=None):
         """Forward (unless fom_done) and adjoint solves, then the device
@@ -354,8 +434,8 @@ class AdjointMethod:
         discrete frequency 2 sin(dt/2)/dt of the leapfrog (the CW steady
         state of the Yee scheme solves Maxwell's equations at that
         frequency).  With exact=False the result is the reference formula."""
-        if not fom_done:
-            self.fom(params)
+        if not fom_done or not self._forward_current(params):
+            self.fom(params)      # another forward solve ran since: redo this one
         self.solve_adjoint_for(params)
         pidx, comp, cell, deps = self.calc_deps(self.sim, params)
         if self.exact_adjoint if exact is None else exact:

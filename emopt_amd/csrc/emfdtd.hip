@@ -1128,6 +1128,41 @@ int emf_update_eps_box(emf_solver *s, int c, const double *vals, int k1, int k2,
     return EMF_OK;
 }
 
+int emf_update_mu_box(emf_solver *s, int c, const double *vals, int k1, int k2, int j1, int j2,
+                      int i1, int i2)
+{
+    LOCK(s);
+    if (c < 0 || c > 2 || !vals) EMF_FAIL(EMF_ERR_ARG, "bad component or null array");
+    if (!s->mat_set) EMF_FAIL(EMF_ERR_STATE, "materials not set");
+    if (k1 < 0 || j1 < 0 || i1 < 0 || k2 > s->Nx || j2 > s->Ny || i2 > s->Nz || k1 >= k2 ||
+        j1 >= j2 || i1 >= i2)
+        EMF_FAIL(EMF_ERR_ARG, "box out of range");
+    long long cnt = (long long)(k2 - k1) * (j2 - j1) * (i2 - i1);
+    if (s->mu_uniform) {
+        bool same = true;
+        for (long long m = 0; m < cnt && same; ++m)
+            same = vals[2 * m] == s->mu_re && vals[2 * m + 1] == 0.0;
+        if (same) return EMF_OK;              // still the uniform value: nothing changes
+        // the grid's mu becomes per cell: every component starts at the uniform value
+        for (int q = 0; q < 3; ++q) {
+            if (!s->mu[q]) CHECK(dalloc(&s->mu[q], s->n));
+            hipLaunchKernelGGL(k_fill2, dim3(grid1d(s->n, 256)), dim3(256), 0, s->stream, s->mu[q],
+                               s->n, make_double2(s->mu_re, 0.0));
+        }
+        HIPCHK(hipGetLastError());
+        s->mu_uniform = false;
+    }
+    CHECK(ensure_scratch(s, cnt));
+    HIPCHK(hipMemcpyAsync(s->scratch, vals, cnt * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+    Geo g = s->geo();
+    hipLaunchKernelGGL(k_scatter_box, dim3(grid1d(cnt, 256)), dim3(256), 0, s->stream, s->mu[c],
+                       s->scratch, g, k1, k2 - k1, j1, j2 - j1, i1, i2 - i1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->coef_dirty = true;
+    return EMF_OK;
+}
+
 int emf_set_compression(emf_solver *s, int on)
 {
     LOCK(s);
@@ -1586,11 +1621,27 @@ int emf_get_field_interp(emf_solver *s, int slot, int comp, int k1, int k2, int 
     return EMF_OK;
 }
 
+YdaxArgs ydax_args(const emf_solver *s)
+{
+    YdaxArgs y{};
+    for (int c = 0; c < 6; ++c) { y.a[c] = s->res[1][c]; y.f[c] = s->res[0][c]; }
+    for (int c = 0; c < 3; ++c) {
+        y.eps[c] = s->eps[c];
+        y.mu[c] = s->mu_uniform ? nullptr : s->mu[c];
+    }
+    y.mu_re = s->mu_re;
+    return y;
+}
+
 int emf_calc_ydAx(emf_solver *s, const double *eps0_x, const double *eps0_y,
-                  const double *eps0_z, int k1, int k2, int j1, int j2, int i1, int i2, double *out)
+                  const double *eps0_z, const double *mu0_x, const double *mu0_y,
+                  const double *mu0_z, int k1, int k2, int j1, int j2, int i1, int i2, double *out)
 {
     LOCK(s);
     if (!out || !eps0_x || !eps0_y || !eps0_z) EMF_FAIL(EMF_ERR_ARG, "null argument");
+    const bool with_mu = mu0_x || mu0_y || mu0_z;
+    if (with_mu && !(mu0_x && mu0_y && mu0_z))
+        EMF_FAIL(EMF_ERR_ARG, "give all three mu0 arrays or none");
     if (!s->res[0][0] || !s->res[1][0]) EMF_FAIL(EMF_ERR_STATE, "forward and adjoint results needed");
     if (k1 < 0 || j1 < 0 || i1 < 0 || k2 > s->Nx || j2 > s->Ny || i2 > s->Nz || k1 > k2 ||
         j1 > j2 || i1 > i2)
@@ -1599,16 +1650,17 @@ int emf_calc_ydAx(emf_solver *s, const double *eps0_x, const double *eps0_y,
     out[0] = out[1] = 0.0;
     if (cnt == 0) return EMF_OK;
     const int nblk = 1024;
-    CHECK(ensure_scratch(s, 3 * cnt + nblk));
-    double2 *z = s->scratch, *part = s->scratch + 3 * cnt;
-    const double *e0[3] = { eps0_x, eps0_y, eps0_z };
-    for (int c = 0; c < 3; ++c)
+    const int nz = with_mu ? 6 : 3;
+    CHECK(ensure_scratch(s, nz * cnt + nblk));
+    double2 *z = s->scratch, *part = s->scratch + nz * cnt;
+    const double *e0[6] = { eps0_x, eps0_y, eps0_z, mu0_x, mu0_y, mu0_z };
+    for (int c = 0; c < nz; ++c)
         HIPCHK(hipMemcpyAsync(z + c * cnt, e0[c], cnt * sizeof(double2), hipMemcpyHostToDevice,
                               s->stream));
     Geo g = s->geo();
-    double2 **A = s->res[1], **F = s->res[0];
-    hipLaunchKernelGGL(k_ydAx, dim3(nblk), dim3(RED_THREADS), 0, s->stream, A[0], A[1], A[2], F[0],
-                       F[1], F[2], s->eps[0], s->eps[1], s->eps[2], z, z + cnt, z + 2 * cnt, g, k1,
+    hipLaunchKernelGGL(k_ydAx, dim3(nblk), dim3(RED_THREADS), 0, s->stream, ydax_args(s), z,
+                       z + cnt, z + 2 * cnt, with_mu ? z + 3 * cnt : nullptr,
+                       with_mu ? z + 4 * cnt : nullptr, with_mu ? z + 5 * cnt : nullptr, g, k1,
                        k2 - k1, j1, j2 - j1, i1, i2 - i1, part);
     HIPCHK(hipGetLastError());
     std::vector<double2> h(nblk);
@@ -1634,7 +1686,7 @@ int emf_grad_assemble(emf_solver *s, int64_t n, const int32_t *pidx, const int32
     for (int64_t e = 0; e < n; ++e) {
         if (pidx[e] < 0 || pidx[e] >= nparam) EMF_FAIL(EMF_ERR_ARG, "parameter index out of range");
         if (e > 0 && pidx[e] < pidx[e - 1]) EMF_FAIL(EMF_ERR_ARG, "entries must be grouped by parameter");
-        if (comp[e] < 0 || comp[e] > 2) EMF_FAIL(EMF_ERR_ARG, "component must be 0..2");
+        if (comp[e] < 0 || comp[e] > 5) EMF_FAIL(EMF_ERR_ARG, "component must be 0..5");
         if (cell[e] < 0 || cell[e] >= s->n) EMF_FAIL(EMF_ERR_ARG, "cell index out of range");
         seg[pidx[e] + 1]++;
     }
@@ -1657,9 +1709,8 @@ int emf_grad_assemble(emf_solver *s, int64_t n, const int32_t *pidx, const int32
         if (e == hipSuccess && n > 0) e = hipMemcpy(dcomp, comp, n * sizeof(int), hipMemcpyHostToDevice);
         if (e == hipSuccess && n > 0) e = hipMemcpy(ddeps, deps, n * sizeof(double2), hipMemcpyHostToDevice);
         if (e != hipSuccess) { set_err("hipMemcpy failed: %s", hipGetErrorString(e)); rc = EMF_ERR_HIP; break; }
-        double2 **A = s->res[1], **F = s->res[0];
-        hipLaunchKernelGGL(k_grad, dim3(nparam), dim3(RED_THREADS), 0, s->stream, A[0], A[1], A[2],
-                           F[0], F[1], F[2], dseg, dcomp, dcell, ddeps, step, dgrad);
+        hipLaunchKernelGGL(k_grad, dim3(nparam), dim3(RED_THREADS), 0, s->stream, ydax_args(s),
+                           dseg, dcomp, dcell, ddeps, step, dgrad);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(grad, dgrad, nparam * sizeof(double), hipMemcpyDeviceToHost, s->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
@@ -1693,7 +1744,7 @@ int emf_pml_stretch(emf_solver *s, int64_t n, const int32_t *comp, const int64_t
         return 1.0 / (1.0 / kap + c / (1.0 - b * ph));
     };
     for (int64_t e = 0; e < n; ++e) {
-        if (comp[e] < 0 || comp[e] > 2) EMF_FAIL(EMF_ERR_ARG, "component must be 0..2");
+        if (comp[e] < 0 || comp[e] > 5) EMF_FAIL(EMF_ERR_ARG, "component must be 0..5");
         if (cell[e] < 0 || cell[e] >= s->n) EMF_FAIL(EMF_ERR_ARG, "cell index out of range");
         const long long q = cell[e];
         const int k = (int)(q % s->Nx), j = (int)((q / s->Nx) % s->Ny), i = (int)(q / g.nxy);

@@ -393,6 +393,12 @@ __global__ void k_fill(double *a, long long n, double v)
     for (; g < n; g += (long long)gridDim.x * blockDim.x) a[g] = v;
 }
 
+__global__ void k_fill2(double2 *a, long long n, double2 v)
+{
+    long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; g < n; g += (long long)gridDim.x * blockDim.x) a[g] = v;
+}
+
 // coefficient arrays from raw complex eps (real: b = dt/eps.re;
 // complex: a = (er - ei)/(er + ei), b = 1/(er + ei), er = eps.re/dt, ei = eps.im/2)
 __global__ void k_eps_coeffs(const double2 *__restrict__ eps, double *__restrict__ b,
@@ -483,54 +489,88 @@ __device__ __forceinline__ double2 ydx_term(double2 adj, double2 d, double2 fwd)
 
 constexpr int RED_THREADS = 256;
 
+// Sum over a work-group of RED_THREADS threads: each 64-lane wave reduces its
+// values by cross-lane butterflies, then one thread adds the waves' sums in
+// wave order (a fixed order, so results are reproducible).
 __device__ __forceinline__ double2 block_sum2(double2 v)
 {
-    __shared__ double2 sh[RED_THREADS];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            sh[threadIdx.x].x += sh[threadIdx.x + w].x;
-            sh[threadIdx.x].y += sh[threadIdx.x + w].y;
-        }
-        __syncthreads();
+    constexpr int W = RED_THREADS / 64;
+    __shared__ double2 sh[W];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v.x += __shfl_xor(v.x, o, 64);
+        v.y += __shfl_xor(v.y, o, 64);
     }
-    return sh[0];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double2 t = sh[0];
+    for (int w = 1; w < W; ++w) { t.x += sh[w].x; t.y += sh[w].y; }
+    return t;
 }
 
-// partial sums of sum_c adj_c * 1j * (eps_c - eps0_c) * fwd_c over a box; one
-// partial per work-group (fixed grid => fixed summation order)
+// adj * (-1j) * d * fwd, the H terms of emopt/fdtd.py:1713-1715 (numpy's
+// -1j is (-0.0, -1.0))
+__device__ __forceinline__ double2 ydx_term_h(double2 adj, double2 d, double2 fwd)
+{
+    return cmul(cmul(cmul(adj, make_double2(-0.0, -1.0)), d), fwd);
+}
+
+// The fields and materials y^T dA x reads: result slots of the adjoint (a) and
+// forward (f) solves, Ex..Hz; eps per component; mu per component, or null
+// with the uniform value mu_re.
+struct YdaxArgs {
+    const double2 *a[6];
+    const double2 *f[6];
+    const double2 *eps[3];
+    const double2 *mu[3];
+    double mu_re;
+};
+
+// partial sums of sum_c adj_c * 1j * (eps_c - eps0_c) * fwd_c
+//               + sum_c adj_Hc * -1j * (mu_c - mu0_c) * fwd_Hc   (z_mu non-null)
+// over a box, the reference's left-to-right order per cell; one partial per
+// work-group (fixed grid => fixed summation order)
 __global__ __launch_bounds__(RED_THREADS) void k_ydAx(
-    const double2 *__restrict__ a0, const double2 *__restrict__ a1, const double2 *__restrict__ a2,
-    const double2 *__restrict__ f0, const double2 *__restrict__ f1, const double2 *__restrict__ f2,
-    const double2 *__restrict__ e0, const double2 *__restrict__ e1, const double2 *__restrict__ e2,
-    const double2 *__restrict__ z0, const double2 *__restrict__ z1, const double2 *__restrict__ z2,
-    Geo g, int k1, int nk, int j1, int nj, int i1, int ni, double2 *partial)
+    YdaxArgs y, const double2 *__restrict__ z0, const double2 *__restrict__ z1,
+    const double2 *__restrict__ z2, const double2 *__restrict__ zm0,
+    const double2 *__restrict__ zm1, const double2 *__restrict__ zm2, Geo g, int k1, int nk,
+    int j1, int nj, int i1, int ni, double2 *partial)
 {
     long long n = (long long)nk * nj * ni;
+    const double2 *z[3] = { z0, z1, z2 }, *zm[3] = { zm0, zm1, zm2 };
     double2 acc = make_double2(0.0, 0.0);
     for (long long m = blockIdx.x * (long long)blockDim.x + threadIdx.x; m < n;
          m += (long long)gridDim.x * blockDim.x) {
         int k = (int)(m % nk), j = (int)((m / nk) % nj), i = (int)(m / ((long long)nk * nj));
         long long c = (long long)(i + i1) * g.nxy + (long long)(j + j1) * g.Nx + (k + k1);
-        double2 d, t;
-        d = make_double2(e0[c].x - z0[m].x, e0[c].y - z0[m].y);
-        t = ydx_term(a0[c], d, f0[c]); acc.x += t.x; acc.y += t.y;
-        d = make_double2(e1[c].x - z1[m].x, e1[c].y - z1[m].y);
-        t = ydx_term(a1[c], d, f1[c]); acc.x += t.x; acc.y += t.y;
-        d = make_double2(e2[c].x - z2[m].x, e2[c].y - z2[m].y);
-        t = ydx_term(a2[c], d, f2[c]); acc.x += t.x; acc.y += t.y;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const double2 e = y.eps[q][c], e0 = z[q][m];
+            const double2 d = make_double2(e.x - e0.x, e.y - e0.y);
+            const double2 t = ydx_term(y.a[q][c], d, y.f[q][c]);
+            acc.x += t.x; acc.y += t.y;
+        }
+        if (zm0) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const double2 mu = y.mu[q] ? y.mu[q][c] : make_double2(y.mu_re, 0.0);
+                const double2 m0 = zm[q][m];
+                const double2 d = make_double2(mu.x - m0.x, mu.y - m0.y);
+                const double2 t = ydx_term_h(y.a[3 + q][c], d, y.f[3 + q][c]);
+                acc.x += t.x; acc.y += t.y;
+            }
+        }
     }
     double2 tot = block_sum2(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// grad[p] = -2 Re(sum over the entries of parameter p of adj*1j*deps*fwd)/step;
-// entries are grouped by parameter, seg[p]..seg[p+1] (one work-group each)
+// grad[p] = -2 Re(sum over the entries of parameter p of adj*1j*deps*fwd)/step
+// for E components (comp 0..2) and of adj*(-1j)*dmu*fwd for H components
+// (comp 3..5, a change of mu_x..mu_z); entries are grouped by parameter,
+// seg[p]..seg[p+1] (one work-group each)
 __global__ __launch_bounds__(RED_THREADS) void k_grad(
-    const double2 *__restrict__ a0, const double2 *__restrict__ a1, const double2 *__restrict__ a2,
-    const double2 *__restrict__ f0, const double2 *__restrict__ f1, const double2 *__restrict__ f2,
-    const long long *__restrict__ seg, const int *__restrict__ comp,
+    YdaxArgs y, const long long *__restrict__ seg, const int *__restrict__ comp,
     const long long *__restrict__ cell, const double2 *__restrict__ deps, double step,
     double *grad)
 {
@@ -539,9 +579,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_grad(
     for (long long e = seg[p] + threadIdx.x; e < seg[p + 1]; e += blockDim.x) {
         const int c = comp[e];
         const long long g = cell[e];
-        const double2 adj = c == 0 ? a0[g] : (c == 1 ? a1[g] : a2[g]);
-        const double2 fwd = c == 0 ? f0[g] : (c == 1 ? f1[g] : f2[g]);
-        const double2 t = ydx_term(adj, deps[e], fwd);
+        const double2 t = c < 3 ? ydx_term(y.a[c][g], deps[e], y.f[c][g])
+                                : ydx_term_h(y.a[c][g], deps[e], y.f[c][g]);
         acc.x += t.x; acc.y += t.y;
     }
     double2 tot = block_sum2(acc);

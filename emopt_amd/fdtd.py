@@ -302,27 +302,38 @@ class FDTD(MaxwellSolver):
         self._mats_ready = True
 
     def update(self, bbox=None):
-        """Re-sample eps inside a real-space bounding box [xmin, xmax, ymin,
-        ymax, zmin, zmax] (emopt/fdtd.py:882-992); None rebuilds everything.
-        mu is not re-sampled in a box update: it is constant in every
-        supported workload."""
+        """Re-sample eps and mu inside a real-space bounding box [xmin, xmax,
+        ymin, ymax, zmin, zmax] at their six Yee offsets (emopt/fdtd.py:
+        882-992); None rebuilds everything.  mu values equal to the grid's
+        uniform mu change nothing on the device."""
         if bbox is None:
             v = self.verbose
             self.verbose = 0
             self.build()
             self.verbose = v
             return
-        dom = DomainCoordinates(bbox[0], bbox[1], bbox[2], bbox[3], bbox[4], bbox[5],
-                                self._dx, self._dy, self._dz)
-        k1, k2 = max(0, dom.k1), min(self._Nx, dom.k2)
-        j1, j2 = max(0, dom.j1), min(self._Ny, dom.j2)
-        i1, i2 = max(0, dom.i1), min(self._Nz, dom.i2)
+        k1, k2, j1, j2, i1, i2 = self.index_box(bbox)
         if k1 >= k2 or j1 >= j2 or i1 >= i2:
             return
         for c in range(3):
             sx, sy, sz = self.YEE_SHIFTS[c]
             tmp = c128(self._sample(self._eps, k1, k2, j1, j2, i1, i2, sx, sy, sz))
             self._lib.emf_update_eps_box(self._h, c, ptr(tmp), k1, k2, j1, j2, i1, i2)
+        for c in range(3):
+            sx, sy, sz = self.YEE_SHIFTS[3 + c]
+            tmp = c128(self._sample(self._mu, k1, k2, j1, j2, i1, i2, sx, sy, sz))
+            self._lib.emf_update_mu_box(self._h, c, ptr(tmp), k1, k2, j1, j2, i1, i2)
+            if self._mu_uniform and not np.all(tmp == tmp.flat[0]):
+                self._mu_uniform = False
+
+    def index_box(self, bbox):
+        """(k1, k2, j1, j2, i1, i2) of the cells a real-space box [xmin,
+        xmax, ymin, ymax, zmin, zmax] covers (DomainCoordinates rounding,
+        emopt/misc.py:212-279), clipped to the grid."""
+        dom = DomainCoordinates(bbox[0], bbox[1], bbox[2], bbox[3], bbox[4], bbox[5],
+                                self._dx, self._dy, self._dz)
+        return (max(0, dom.k1), min(self._Nx, dom.k2), max(0, dom.j1), min(self._Ny, dom.j2),
+                max(0, dom.i1), min(self._Nz, dom.i2))
 
     # --------------------------------------------------------------- sources
     def _local_overlap(self, domain):
@@ -486,15 +497,18 @@ class FDTD(MaxwellSolver):
         return tuple(arrs)
 
     def calc_ydAx(self, Adiag0, box=None):
-        """y^T (A - A0) x on the device (emopt/fdtd.py:1694-1717); the mu
-        term vanishes because mu is never updated.  Adiag0 holds full-grid
-        arrays (as get_A_diag returns them); with box = (k1, k2, j1, j2, i1,
-        i2) only that index box is summed (the rest of A - A0 is zero)."""
+        """y^T (A - A0) x on the device (emopt/fdtd.py:1694-1717), the eps
+        and mu terms.  Adiag0 holds full-grid arrays (as get_A_diag returns
+        them: eps_x..eps_z, mu_x..mu_z; with only three, mu is taken as
+        unchanged); with box = (k1, k2, j1, j2, i1, i2) only that index box is
+        summed (the rest of A - A0 is zero)."""
         out = np.zeros(2)
         k1, k2, j1, j2, i1, i2 = box if box is not None else (0, self._Nx, 0, self._Ny, 0, self._Nz)
         shp = (self._Nz, self._Ny, self._Nx)
-        e0 = [c128(np.asarray(a).reshape(shp)[i1:i2, j1:j2, k1:k2]).ravel() for a in Adiag0[:3]]
-        self._lib.emf_calc_ydAx(self._h, *[ptr(a) for a in e0], k1, k2, j1, j2, i1, i2, ptr(out))
+        z = [c128(np.asarray(a).reshape(shp)[i1:i2, j1:j2, k1:k2]).ravel() for a in Adiag0[:6]]
+        if len(z) < 6:
+            z = z[:3] + [None] * 3
+        self._lib.emf_calc_ydAx(self._h, *[ptr(a) for a in z], k1, k2, j1, j2, i1, i2, ptr(out))
         return complex(out[0], out[1])
 
     def pml_stretch(self, comp, cell):

@@ -113,6 +113,13 @@ EMF_API int emf_set_mat_arrays(emf_solver *s, const double *eps_x, const double 
 EMF_API int emf_update_eps_box(emf_solver *s, int c, const double *vals, int k1, int k2,
                        int j1, int j2, int i1, int i2);
 
+/* Overwrites mu_c (c = 0,1,2) on a sub-box, as emf_update_eps_box does for
+ * eps: the mu half of FDTD.update(bbox) (emopt/fdtd.py:958-992).  Values
+ * equal to the grid's uniform mu (real, imaginary part 0) change nothing;
+ * any other value makes mu per cell (the H pass then reads dt/mu per cell). */
+EMF_API int emf_update_mu_box(emf_solver *s, int c, const double *vals, int k1, int k2,
+                      int j1, int j2, int i1, int i2);
+
 /* Coefficient compression for real eps (no counterpart in the reference):
  * on (default), emf_set_mat_arrays also builds a per-cell index into the
  * table of distinct (eps_x, eps_y, eps_z) triples, and the E pass reads that
@@ -381,20 +388,24 @@ EMF_API int emf_batch_count(emf_batch *b, int *running, int *finished);
 
 /* y^T dA x of emopt/fdtd.py:1694-1717 on the device:
  *   sum_c sum_cells E_adj_c * 1j * (eps_c - eps0_c) * E_c
- * over the box [k1,k2)x[j1,j2)x[i1,i2), where eps0 is given as three complex
- * arrays of the box's shape.  Result written to out[0] (re), out[1] (im).
- * The mu term of the reference is omitted: mu is not a design variable in any
- * supported workload, so mu - mu0 = 0. */
+ *                 + H_adj_c * -1j * (mu_c - mu0_c) * H_c
+ * over the box [k1,k2)x[j1,j2)x[i1,i2), where eps0 and mu0 are given as
+ * complex arrays of the box's shape (the three mu0 arrays may all be NULL:
+ * mu unchanged, no mu term).  Result written to out[0] (re), out[1] (im). */
 EMF_API int emf_calc_ydAx(emf_solver *s, const double *eps0_x, const double *eps0_y,
-                  const double *eps0_z, int k1, int k2, int j1, int j2, int i1, int i2,
+                  const double *eps0_z, const double *mu0_x, const double *mu0_y,
+                  const double *mu0_z, int k1, int k2, int j1, int j2, int i1, int i2,
                   double *out);
 
 /* Hand-written gradient assembly for many design parameters at once:
- *   grad[p] = -2 Re( sum_c sum_cells E_adj_c * 1j * deps[p,c,cell] * E_c ) / step
- * where deps holds, per parameter, sparse (cell, component, d_eps) triples
- * (d_eps complex).  This is the reference's per-parameter loop
+ *   grad[p] = -2 Re( sum_c sum_cells E_adj_c * 1j * deps[p,c,cell] * E_c
+ *                   + H_adj_c * -1j * dmu[p,c,cell] * H_c ) / step
+ * where deps holds, per parameter, sparse (cell, component, d_material)
+ * triples (complex); component 0..2 is eps_x..eps_z against Ex..Ez, 3..5 is
+ * mu_x..mu_z against Hx..Hz.  This is the reference's per-parameter loop
  * (emopt/adjoint_method.py:504-533) collapsed into one device pass.
- * entries: n triples; cell index g = (i*Ny + j)*Nx + k; param ids in pidx. */
+ * entries: n triples grouped by parameter; cell index g = (i*Ny + j)*Nx + k;
+ * param ids in pidx. */
 EMF_API int emf_grad_assemble(emf_solver *s, int64_t n, const int32_t *pidx,
                       const int32_t *comp, const int64_t *cell, const double *deps,
                       int nparam, double step, double *grad);

@@ -1,11 +1,16 @@
-"""get_field_interp on the GPU (emf_get_field_interp) against the numpy
-restatement of emopt/fdtd.py:1401-1553 in emopt_amd/fields.py, on the same
-solved fields: every component, forward and adjoint slots, domains inside
-the grid and touching each min and max face, under bc '000', 'EH0' and 'HE0'
-(the 'E'/'H' mirrors at the x and y min faces).  The kernel sums the stencil
-in the same order and divides once, so the values are identical
-(np.array_equal); the numpy restatement itself is pinned by the closed-form
-checks of tests/test_host.py."""
+"""Field post-processing on the GPU against the oracle.
+
+get_field_interp on the GPU (emf_get_field_interp) against the oracle's
+restatement of emopt/fdtd.py:1401-1553 (oracle/driver.py, holding the same
+solved phasors) and against the product's numpy path (emopt_amd/fields.py):
+every component, forward and adjoint slots, domains inside the grid and
+touching each min and max face, under bc '000', 'EH0' and 'HE0' (the 'E'/'H'
+mirrors at the x and y min faces).  The kernel sums the stencil in the same
+order and divides once, so the values are identical (np.array_equal).  The
+source power and emf_calc_ydAx are checked against the oracle's
+get_source_power and calc_ydAx on the same fields.  The oracle's
+restatements are pinned by closed forms and finite differences in
+tests/test_oracle_post.py."""
 import warnings
 
 import numpy as np
@@ -13,7 +18,8 @@ import pytest
 
 from emopt_amd import fields
 from emopt_amd.misc import DomainCoordinates
-from tests.cases import build_sim, strip_case
+from oracle import driver as D
+from tests.cases import all_fields, build_sim, strip_case
 
 pytestmark = pytest.mark.gpu
 
@@ -34,6 +40,15 @@ class _HostOnly:
 
     def get_field_interp(self, component, domain=None, squeeze=False, adjoint=False):
         return fields.interp_to_ez(self, component, domain, squeeze, adjoint)
+
+
+def _oracle_holder(sim, case, mats=None):
+    """The oracle's post-processing restatement holding the solver's phasors
+    (forward and, when solved, adjoint) downloaded on the full grid."""
+    fwd = all_fields(sim, case)
+    adj = all_fields(sim, case, adjoint=True) if sim._have[1] else None
+    return D.holder(sim.Nx, sim.Ny, sim.Nz, sim.dx, sim.dy, sim.dz, fwd, adj, mats=mats,
+                    bc=sim.bc, w_pml=sim.w_pml)
 
 
 def _domains(sim):
@@ -62,13 +77,19 @@ def test_device_interp_matches_numpy(bc):
                                                                     sim.dx, sim.dy, sim.dz))
         sim.solve_adjoint()
     host = _HostOnly(sim)
+    ora = _oracle_holder(sim, case)
     checked = 0
     for adjoint in (False, True):
         for dom in _domains(sim):
+            odom = None if dom is None else D.Domain.from_indices(dom.k1, dom.k2, dom.j1, dom.j2,
+                                                                 dom.i1, dom.i2)
             for c in COMPS:
                 g = fields.interp_to_ez(sim, c, dom, adjoint=adjoint)
                 h = fields.interp_to_ez(host, c, dom, adjoint=adjoint)
-                assert g.shape == h.shape, (bc, c, dom and (dom.k1, dom.k2, dom.j1, dom.j2))
+                o = ora.get_field_interp(c, odom, adjoint=adjoint)
+                assert g.shape == h.shape == o.shape, (bc, c, dom and (dom.k1, dom.k2, dom.j1,
+                                                                       dom.j2))
+                assert np.array_equal(g, o), (bc, adjoint, c, np.abs(g - o).max())
                 assert np.array_equal(g, h), (bc, adjoint, c, np.abs(g - h).max())
                 checked += g.size > 0 and np.any(g != 0)
     assert checked > 60
@@ -77,13 +98,49 @@ def test_device_interp_matches_numpy(bc):
     assert np.array_equal(sim.get_field_interp("Hz", d), fields.interp_to_ez(host, "Hz", d))
 
 
-def test_device_interp_source_power():
+@pytest.mark.parametrize("bc", ["000", "HE0"])
+def test_device_interp_source_power(bc):
     """get_source_power (24 interpolations on the box one cell inside the
-    CPML) is the same number on the device and numpy paths."""
-    sim = build_sim(strip_case(36, 30, 24, w=6))
+    CPML) is the same number on the device and numpy paths, and the oracle's
+    get_source_power on the same phasors agrees to rounding."""
+    case = strip_case(36, 30, 24, w=6, bc=bc)
+    sim = build_sim(case)
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         sim.solve_forward()
     p_dev = fields.source_power(sim)
     p_host = fields.source_power(_HostOnly(sim))
+    p_ora = _oracle_holder(sim, case).get_source_power()
+    print("source power: device %.15e oracle %.15e" % (p_dev, p_ora))
     assert p_dev == p_host and p_dev > 0
+    assert abs(p_dev - p_ora) <= 1e-12 * abs(p_ora)
+
+
+def test_device_ydAx_matches_oracle():
+    """emf_calc_ydAx = sum E_adj * 1j * (eps - eps0) * E over the grid (and a
+    box) against the oracle's calc_ydAx on the same phasors and materials,
+    for a change of eps in a box of cells."""
+    case = strip_case(36, 30, 24, w=6)
+    sim = build_sim(case)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim.solve_forward()
+        src, (k1, k2, j1, j2, i1, i2) = case["sources"][0]
+        sim.add_adjoint_sources(src, DomainCoordinates.from_indices(k1 + 6, k2 + 6, j1, j2, i1,
+                                                                    i2, sim.dx, sim.dy, sim.dz))
+        sim.solve_adjoint()
+    A0 = sim.get_A_diag()
+    rng = np.random.default_rng(5)
+    box = (14, 21, 10, 19, 9, 15)
+    new = [a.reshape(sim.Nz, sim.Ny, sim.Nx).copy() for a in A0]
+    for q in range(3):
+        new[q][box[4]:box[5], box[2]:box[3], box[0]:box[1]] += \
+            0.3 * rng.random((box[5] - box[4], box[3] - box[2], box[1] - box[0]))
+    sim.set_material_arrays(new[:3], new[3:])
+    got_full = sim.calc_ydAx(A0)
+    got_box = sim.calc_ydAx(A0, box=box)
+    ref = _oracle_holder(sim, case, mats=new).calc_ydAx(A0)
+    print("ydAx: device %r (box %r), oracle %r" % (got_full, got_box, ref))
+    assert abs(ref) > 0
+    assert abs(got_full - ref) <= 1e-12 * abs(ref)
+    assert abs(got_box - ref) <= 1e-12 * abs(ref)
