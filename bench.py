@@ -29,12 +29,17 @@ Also reported:
                 forward dipoles, half adjoint plane sources): one after the
                 other through the single-solve path, then all together in one
                 batch; solves/hour, latency p50/p99, result agreement
-  serving       Poisson arrivals of solve requests of mixed grid size served
-                by the continuous-batching server under an HBM budget, and
-                the same requests through the single-solve path as a FIFO
-                queue; solves/hour and latency p50/p99 of both
+  serving       Poisson arrivals (2, 8 and 32 per second) of solve requests of
+                mixed grid size served by the continuous-batching server under
+                an HBM budget, and the same requests through the single-solve
+                path as a FIFO queue; solves/hour and latency p50/p99 of both
+  differentiation  the grating workload (360x60x120, 64 edge parameters):
+                forward and backward (adjoint solve + gradient assembly)
+                times, the gradient against central differences on 4 edges
+  error_vs_reference  the 400x200x120 phasors at a fixed step count against
+                the oracle, relative L2 per component
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch N] [--serve-seconds S]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch N] [--serve-requests R]
     python bench.py --profile-summary --stats S.csv [--fetch F.csv --write W.csv]
 """
 from __future__ import annotations
@@ -258,36 +263,21 @@ def bench_batched(n, seed=3):
             "max_rel_l2_vs_plain_sample": err, "sample": sample}
 
 
-def bench_serving(rate, seconds, seed=5, budget=200e9, max_running=8):
-    """Poisson arrivals at `rate` per second for `seconds` of requests of the
-    serving workload (dims log-uniform 64^3..320x240x120, d 0.03-0.05 um,
-    lambda 1.26-1.62 um, random ellipses), served by the continuous-batching
-    server; then the same requests through the single-solve path as one FIFO
-    queue with the same arrival times, each request's service time being its
+def bench_serving(rates, n_requests, seed=5, budget=200e9, max_running=8):
+    """The serving workload (dims log-uniform 64^3..320x240x120, d 0.03-0.05
+    um, lambda 1.26-1.62 um, random ellipses) at each Poisson arrival rate:
+    the same n_requests requests arrive at that rate (the window is the last
+    arrival time, stated per rate) and are served by the continuous-batching
+    server under the HBM budget; then the single-solve path as a FIFO queue
+    with the same arrival times, each request's service time being its
     problem build (host material sampling and upload) plus its solve,
-    measured one by one -- the work a plain request loop does per request."""
+    measured one by one once -- the work a plain request loop does per
+    request."""
     from emopt_amd.serve import Server
     from tests.cases import build_sim, serve_case, serve_request
-    rng = np.random.default_rng(seed)
-    arr, t = [], 0.0
-    while True:
-        t += rng.exponential(1.0 / rate)
-        if t > seconds:
-            break
-        arr.append(t)
-    reqs = [serve_request(q, seed=seed) for q in range(len(arr))]
-    srv = Server(hbm_budget=budget, prep_workers=8, max_running=max_running)
-    with warnings.catch_warnings():
-        warnings.simplefilter("ignore")
-        st = srv.run_trace(list(zip(arr, reqs)))
-    srv.close()
-    log("[bench] serving: %d requests, p50 %.2fs p99 %.2fs" % (len(arr), st.get("p50_s", 0),
-                                                               st.get("p99_s", 0)))
-    # single-solve path: FIFO with the same arrivals (host build not timed)
-    service = []
-    steps = 0
-    build_s = 0.0
-    for q in range(len(arr)):
+    # single-solve service times, once for all rates
+    service, steps, build_s = [], 0, 0.0
+    for q in range(n_requests):
         t0 = time.perf_counter()
         c = serve_case(q, seed=seed)
         sim = build_sim(c)
@@ -297,19 +287,39 @@ def bench_serving(rate, seconds, seed=5, budget=200e9, max_running=8):
         build_s += t1 - t0
         steps += sim.last_solve["steps"] * c["Nx"] * c["Ny"] * c["Nz"]
         del sim
-    fin, lat = 0.0, []
-    for a, s in zip(arr, service):
-        fin = max(fin, a) + s
-        lat.append(fin - a)
-    span = fin
-    plain = dict(solves_per_hr=len(arr) * 3600 / span, cell_updates_per_s=steps / span,
-                 busy_seconds=float(sum(service)), build_seconds=build_s, **_pcts(lat))
-    return {"workload": "serve_poisson_%gps_%gs_dims64^3-320x240x120" % (rate, seconds),
-            "requests": len(arr), "budget_bytes": budget, "max_running": max_running,
-            "served": st, "plain_fifo": plain,
+    log("[bench] serving: single-solve service of %d requests %.1fs" % (n_requests, sum(service)))
+    out = {"requests": n_requests, "budget_bytes": budget, "max_running": max_running,
+           "plain_busy_seconds": float(sum(service)), "plain_build_seconds": build_s,
+           "rates": []}
+    for k, rate in enumerate(rates):
+        rng = np.random.default_rng([seed, k])
+        arr = list(np.cumsum(rng.exponential(1.0 / rate, size=n_requests)))
+        reqs = [serve_request(q, seed=seed) for q in range(n_requests)]
+        srv = Server(hbm_budget=budget, prep_workers=8, max_running=max_running)
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            st = srv.run_trace(list(zip(arr, reqs)))
+        srv.close()
+        fin, lat = 0.0, []
+        for a, sv in zip(arr, service):
+            fin = max(fin, a) + sv
+            lat.append(fin - a)
+        plain = dict(solves_per_hr=n_requests * 3600 / fin, cell_updates_per_s=steps / fin,
+                     **_pcts(lat))
+        log("[bench] serving %g/s over %.1fs: served p50 %.2fs p99 %.2fs %.0f solves/h; FIFO p50 "
+            "%.2fs p99 %.2fs %.0f solves/h" % (rate, arr[-1], st.get("p50_s", 0),
+                                               st.get("p99_s", 0), st.get("solves_per_hr", 0),
+                                               plain["p50_s"], plain["p99_s"],
+                                               plain["solves_per_hr"]))
+        out["rates"].append({
+            "workload": "serve_poisson_%gps_%dreq_window%.1fs_dims64^3-320x240x120"
+                        % (rate, n_requests, arr[-1]),
+            "rate_per_s": rate, "window_s": float(arr[-1]), "served": st, "plain_fifo": plain,
+            "waits_on_free_hbm": srv.waits_on_free,
             "p99_ratio_plain_over_served": plain["p99_s"] / st["p99_s"] if st.get("p99_s") else None,
             "solves_per_hr_ratio": (st["solves_per_hr"] / plain["solves_per_hr"]
-                                    if st.get("solves_per_hr") else None)}
+                                    if st.get("solves_per_hr") else None)})
+    return out
 
 
 def bench_materials(reps=3):
@@ -480,11 +490,13 @@ def main():
                     help="compare the workload's phasors with the oracle's at n0 = this many "
                          "Tn (0: skip); 28 Tn is past the source ramp")
     ap.add_argument("--batch", type=int, default=32, help="solves in the batched section (0: skip)")
-    ap.add_argument("--serve-seconds", type=float, default=20.0,
-                    help="arrival window of the serving section (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=2.0)
+    ap.add_argument("--serve-requests", type=int, default=48,
+                    help="requests per arrival rate in the serving section (0: skip)")
+    ap.add_argument("--serve-rates", default="2,8,32", help="Poisson arrival rates, per second")
     ap.add_argument("--serve-running", type=int, default=8, help="solves admitted to the batch at once")
-    ap.add_argument("--diff", action="store_true", help="also run the differentiation workload")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiation workload")
+    ap.add_argument("--only", choices=["batched", "serving", "differentiation"],
+                    help="run only this section (for profiling it) and print its JSON")
     ap.add_argument("--profile-summary", action="store_true")
     ap.add_argument("--stats")
     ap.add_argument("--fetch")
@@ -500,6 +512,16 @@ def main():
         print("bench.py: this build runs on exactly one GPU (single-GPU solver, no "
               "multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.only:
+        if args.only == "batched":
+            out = bench_batched(args.batch)
+        elif args.only == "serving":
+            out = bench_serving([float(r) for r in args.serve_rates.split(",")],
+                                args.serve_requests, max_running=args.serve_running)
+        else:
+            out = bench_diff()
+        print(json.dumps({args.only: out}), flush=True)
+        return
 
     from tests.cases import large_case
     from emopt_amd import _lib
@@ -595,10 +617,10 @@ def main():
                 mt["supersample"]["device_s"], mt["supersample"]["identical"]))
     if args.batch > 0:
         res["batched"] = bench_batched(args.batch)
-    if args.serve_seconds > 0:
-        res["serving"] = bench_serving(args.serve_rate, args.serve_seconds,
-                                       max_running=args.serve_running)
-    if args.diff:
+    if args.serve_requests > 0:
+        res["serving"] = bench_serving([float(r) for r in args.serve_rates.split(",")],
+                                       args.serve_requests, max_running=args.serve_running)
+    if not args.no_diff:
         res["differentiation"] = bench_diff()
 
     if not args.no_cpu:

@@ -83,13 +83,18 @@ _SIGS = {
     "emf_get_history": ([_P, _P, c_int], c_int),
     "emf_get_field": ([_P, c_int, c_int] + [c_int] * 6 + [_P], c_int),
     "emf_get_field_interp": ([_P, c_int, c_int] + [c_int] * 12 + [_P], c_int),
+    "emf_interp_flux": ([_P, c_int, c_int] + [c_int] * 12 + [_P], c_int),
+    "emf_set_plane_weights": ([_P, c_int, c_int64] + [_P] * 6, c_int),
+    "emf_interp_dot": ([_P, c_int, c_int] + [c_int] * 12 + [_P], c_int),
     "emf_time_steps": ([_P, c_int64, c_int64, c_int64, c_int, _P], c_int),
     "emf_set_tuning": ([_P, c_int, c_int, c_int, c_int], c_int),
     "emf_set_streaming": ([_P, c_int], c_int),
+    "emf_set_h_shape": ([_P, c_int, c_int], c_int),
     "emf_sample_structured": ([c_int, _P, c_int, _P, _P, c_int, c_double, c_double, c_double,
                                c_int, c_double, c_double, c_int, c_int, c_int, c_int, c_int, c_int,
                                c_double, c_double, c_double, _P], c_int),
     "emf_copy_bandwidth": ([c_int, c_int64, c_int, _P], c_int),
+    "emf_mem_info": ([c_int, POINTER(c_int64), POINTER(c_int64)], c_int),
     "emf_calc_ydAx": ([_P] + [_P] * 6 + [c_int] * 6 + [_P], c_int),
     "emf_grad_assemble": ([_P, c_int64, _P, _P, _P, _P, c_int, c_double, _P], c_int),
     "emf_pml_stretch": ([_P, c_int64, _P, _P, _P], c_int),

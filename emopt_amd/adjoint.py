@@ -80,6 +80,35 @@ class ModeMatch:
         self.mode_match_fwd = 0.5 * self.am * np.conj(self.am) * np.real(self.Pm)
         self.mode_match_back = 0.5 * self.bm * np.conj(self.bm) * np.real(self.Pm)
 
+    def _weights(self):
+        """Weight planes W (Ex..Hz) with am1 = 0.5 ds sum(W_E . E) / Pm and
+        am2 = 0.5 ds sum(W_H . H) / conj(Pm): compute()'s sums regrouped per
+        field component."""
+        c = np.conj
+        sx, sy, sz = self.sx, self.sy, self.sz
+        return (-sy * c(self.Hzm) + sz * c(self.Hym), sx * c(self.Hzm) - sz * c(self.Hxm),
+                -sx * c(self.Hym) + sy * c(self.Hxm), sy * c(self.Ezm) - sz * c(self.Eym),
+                -sx * c(self.Ezm) + sz * c(self.Exm), sx * c(self.Eym) - sy * c(self.Exm))
+
+    def compute_device(self, sim, domain, adjoint=False, weight_set=0):
+        """compute() on the solver's GPU: the overlap sums are reduced there
+        over the fields interpolated on `domain` (emf_interp_dot), with the
+        weight planes uploaded once per solver, so no field plane crosses to
+        the host.  Same results as compute() on get_field_interp's planes up
+        to the summation order."""
+        key = (id(sim), weight_set)
+        if getattr(self, "_dev_key", None) != key:
+            sim.set_plane_weights(weight_set, self._weights())
+            self._dev_key = key
+        se, sh = sim.interp_dot(weight_set, domain, adjoint=adjoint)
+        ds = self.ds1 * self.ds2
+        self.am1 = 0.5 * ds * se / self.Pm
+        self.am2 = 0.5 * ds * sh / np.conj(self.Pm)
+        self.am = self.am1 + self.am2
+        self.bm = self.am1 - self.am2
+        self.mode_match_fwd = 0.5 * self.am * np.conj(self.am) * np.real(self.Pm)
+        self.mode_match_back = 0.5 * self.bm * np.conj(self.bm) * np.real(self.Pm)
+
     def get_mode_match_forward(self, P_in):
         return np.real(self.mode_match_fwd) / P_in
 
@@ -272,7 +301,16 @@ class AdjointMethod:
     (emopt/adjoint_method.py:97-547).  Subclasses define update_system,
     calc_fom, calc_dFdx, calc_grad_p and get_update_boxes as in the
     reference; optionally calc_deps (sparse dEps/dp) for the device gradient
-    assembly used by :class:`FDTDAdjointFunction`."""
+    assembly used by :class:`FDTDAdjointFunction`.
+
+    exact_adjoint (default True) is a choice of this package, not of the
+    reference: gradient_device and fdtd_fom then scale dEps/dp by the CPML
+    stretch product and the leapfrog's discrete frequency factor, giving the
+    gradient of the time-stepped system (what finite differences of the FOM
+    measure).  The reference's formula, -2 Re(E_adj i dEps E), is what
+    gradient() and calc_gradient() return, and what gradient_device returns
+    with exact=False; the two differ wherever the design crosses a CPML slab
+    (by 16.5 % in norm on the 360x60x120 grating workload)."""
 
     exact_adjoint = True
 

@@ -35,6 +35,7 @@ class Batch:
     def __init__(self, device=0, steps_per_graph=64, zc=2, blocks_per_cu=16,
                  group_bytes=GROUP_BYTES, rotate_groups=False):
         self._lib = _lib.load()
+        self.device = device
         h = ctypes.c_void_p()
         self._lib.emf_batch_new(device, ctypes.byref(h))
         self._h = h.value
@@ -98,6 +99,12 @@ class Batch:
             self.finished[mid] = dict(sim.last_solve)
             done.append(mid)
         return done
+
+    def free_bytes(self):
+        """Free device memory of the batch's GPU (hipMemGetInfo)."""
+        f, t = ctypes.c_int64(), ctypes.c_int64()
+        self._lib.emf_mem_info(self.device, ctypes.byref(f), ctypes.byref(t))
+        return int(f.value)
 
     @property
     def running(self):

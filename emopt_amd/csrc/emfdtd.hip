@@ -196,6 +196,7 @@ struct emf_solver {
 
     int kind = 1;                  // 0, 2 = fused step kernels; 1, 3 = separate H / E passes
     int tyg = 4;                   // waves (rows) per work-group
+    int tyg_h = 0, zc_h = 0;       // the H pass's rows and planes per work-group (0: as tyg, zc)
 
     ConvState *conv = nullptr;
     long long *samples = nullptr;
@@ -207,6 +208,11 @@ struct emf_solver {
 
     double2 *scratch = nullptr;
     size_t scratch_n = 0;
+
+    // weight planes of emf_interp_dot: per id, six optional complex planes
+    static constexpr int kWeightSets = 8;
+    double2 *wplane[kWeightSets][6] = {};
+    long long wcount[kWeightSets] = {};
 
     void *in_batch = nullptr;      // the emf_batch this handle is a member of
 
@@ -679,12 +685,22 @@ dim3 pass_grid(const emf_solver *s)
     return dim3((s->Nx + PX - 1) / PX, (s->Ny + ty - 1) / ty, (s->Nz + s->zc - 1) / s->zc);
 }
 
+// the H pass's work-group shape: its own when set, else the E pass's
+int pass_ty_h(const emf_solver *s)
+{
+    const int t = s->tyg_h > 0 ? s->tyg_h : s->tyg;
+    return t == 1 || t == 2 || t == 8 ? t : 4;
+}
+
+int pass_zc_h(const emf_solver *s) { return s->zc_h > 0 ? s->zc_h : s->zc; }
+
 template <bool MU, int OCC, bool NT>
 void launch_h_occ(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
                   const int *done)
 {
-    const dim3 gr = pass_grid(s);
-    switch (pass_ty(s)) {
+    const int ty = pass_ty_h(s);
+    const dim3 gr((s->Nx + PX - 1) / PX, (s->Ny + ty - 1) / ty, (s->Nz + g.zc - 1) / g.zc);
+    switch (ty) {
     case 1: PASS_LAUNCH((k_pass_h<MU, 1, OCC, NT>), gr, dim3(PX, 1), A, g, ps, done); break;
     case 2: PASS_LAUNCH((k_pass_h<MU, 2, OCC, NT>), gr, dim3(PX, 2), A, g, ps, done); break;
     case 8: PASS_LAUNCH((k_pass_h<MU, 8, OCC, NT>), gr, dim3(PX, 8), A, g, ps, done); break;
@@ -744,6 +760,7 @@ int launch_h(emf_solver *s, double t, const int *done)
     PassSrc ps = pass_sources(s, 3, t);
     FusedArgs A = make_args(s, s->cur, s->cur);
     Geo g = s->geo();
+    g.zc = pass_zc_h(s);
     if (s->mu_uniform) launch_h_t<false>(s, A, g, ps, done);
     else launch_h_t<true>(s, A, g, ps, done);
     HIPCHK(hipGetLastError());
@@ -1007,6 +1024,8 @@ int emf_delete(emf_solver *s)
         dfree(s->samples);
         dfree(s->hist);
         dfree(s->scratch);
+        for (int w = 0; w < emf_solver::kWeightSets; ++w)
+            for (int q = 0; q < 6; ++q) dfree(s->wplane[w][q]);
         if (s->done_host) (void)hipHostFree(s->done_host);
         (void)hipStreamDestroy(s->stream);
     }
@@ -1578,21 +1597,11 @@ int emf_get_field(emf_solver *s, int slot, int comp, int k1, int k2, int j1, int
     return EMF_OK;
 }
 
-int emf_get_field_interp(emf_solver *s, int slot, int comp, int k1, int k2, int j1, int j2,
-                         int i1, int i2, int ek1, int ek2, int ej1, int ej2, int ei1, int ei2,
-                         double *out)
+// The averaging stencil of get_field_interp for component comp (Ex..Hz) on a
+// grown box starting at (ek1, ej1): offsets and the 'E'/'H' mirrors of
+// emopt/fdtd.py:1472-1547.
+static IStencil interp_stencil(const emf_solver *s, int comp, int ek1, int ej1)
 {
-    LOCK(s);
-    if (slot < 0 || slot > 1 || !s->res[slot][0]) EMF_FAIL(EMF_ERR_STATE, "result slot %d empty", slot);
-    if (comp < 0 || comp > 5 || !out) EMF_FAIL(EMF_ERR_ARG, "bad component or null output");
-    if (ek1 < 0 || ej1 < 0 || ei1 < 0 || ek2 > s->Nx || ej2 > s->Ny || ei2 > s->Nz ||
-        k1 < ek1 || j1 < ej1 || i1 < ei1 || k2 > ek2 || j2 > ej2 || i2 > ei2 || k1 > k2 ||
-        j1 > j2 || i1 > i2)
-        EMF_FAIL(EMF_ERR_ARG, "box out of range (the domain must lie in the grown box, and it in "
-                 "the grid)");
-    const long long cnt = (long long)(k2 - k1) * (j2 - j1) * (i2 - i1);
-    if (cnt == 0) return EMF_OK;
-    // stencils and mirrors of emopt/fdtd.py:1472-1547, components Ex..Hz
     static const int nst[6] = {4, 4, 1, 2, 2, 8};
     static const int off[6][8][3] = {
         {{0, 0, 0}, {0, 0, -1}, {1, 0, 0}, {1, 0, -1}},
@@ -1611,6 +1620,37 @@ int emf_get_field_interp(emf_solver *s, int slot, int comp, int k1, int k2, int 
     st.my = miry[comp] && ej1 == 0 && (by == 'E' || by == 'H');
     st.sx = bx == 'E' ? -1.0 : 1.0;
     st.sy = by == 'E' ? -1.0 : 1.0;
+    return st;
+}
+
+// checks shared by the interpolating entry points
+static int check_interp_boxes(const emf_solver *s, int slot, int k1, int k2, int j1, int j2, int i1,
+                              int i2, int ek1, int ek2, int ej1, int ej2, int ei1, int ei2)
+{
+    if (slot < 0 || slot > 1 || !s->res[slot][0]) EMF_FAIL(EMF_ERR_STATE, "result slot %d empty", slot);
+    if (ek1 < 0 || ej1 < 0 || ei1 < 0 || ek2 > s->Nx || ej2 > s->Ny || ei2 > s->Nz ||
+        k1 < ek1 || j1 < ej1 || i1 < ei1 || k2 > ek2 || j2 > ej2 || i2 > ei2 || k1 > k2 ||
+        j1 > j2 || i1 > i2)
+        EMF_FAIL(EMF_ERR_ARG, "box out of range (the domain must lie in the grown box, and it in "
+                 "the grid)");
+    return EMF_OK;
+}
+
+int emf_get_field_interp(emf_solver *s, int slot, int comp, int k1, int k2, int j1, int j2,
+                         int i1, int i2, int ek1, int ek2, int ej1, int ej2, int ei1, int ei2,
+                         double *out)
+{
+    LOCK(s);
+    if (slot < 0 || slot > 1 || !s->res[slot][0]) EMF_FAIL(EMF_ERR_STATE, "result slot %d empty", slot);
+    if (comp < 0 || comp > 5 || !out) EMF_FAIL(EMF_ERR_ARG, "bad component or null output");
+    if (ek1 < 0 || ej1 < 0 || ei1 < 0 || ek2 > s->Nx || ej2 > s->Ny || ei2 > s->Nz ||
+        k1 < ek1 || j1 < ej1 || i1 < ei1 || k2 > ek2 || j2 > ej2 || i2 > ei2 || k1 > k2 ||
+        j1 > j2 || i1 > i2)
+        EMF_FAIL(EMF_ERR_ARG, "box out of range (the domain must lie in the grown box, and it in "
+                 "the grid)");
+    const long long cnt = (long long)(k2 - k1) * (j2 - j1) * (i2 - i1);
+    if (cnt == 0) return EMF_OK;
+    IStencil st = interp_stencil(s, comp, ek1, ej1);
     const IBox dom{k1, k2, j1, j2, i1, i2}, ext{ek1, ek2, ej1, ej2, ei1, ei2};
     CHECK(ensure_scratch(s, cnt));
     hipLaunchKernelGGL(k_field_interp, dim3(grid1d(cnt, 256)), dim3(256), 0, s->stream,
@@ -1621,7 +1661,94 @@ int emf_get_field_interp(emf_solver *s, int slot, int comp, int k1, int k2, int 
     return EMF_OK;
 }
 
-YdaxArgs ydax_args(const emf_solver *s)
+int emf_interp_flux(emf_solver *s, int slot, int axis, int k1, int k2, int j1, int j2, int i1,
+                    int i2, int ek1, int ek2, int ej1, int ej2, int ei1, int ei2, double *out)
+{
+    LOCK(s);
+    if (axis < 0 || axis > 2 || !out) EMF_FAIL(EMF_ERR_ARG, "axis must be 0..2, out non-null");
+    CHECK(check_interp_boxes(s, slot, k1, k2, j1, j2, i1, i2, ek1, ek2, ej1, ej2, ei1, ei2));
+    *out = 0.0;
+    const long long cnt = (long long)(k2 - k1) * (j2 - j1) * (i2 - i1);
+    if (cnt == 0) return EMF_OK;
+    // the component pairs of emopt/fdtd.py:1611-1665: x faces Ey Hz* - Ez Hy*,
+    // y faces Ex Hz* - Ez Hx*, z faces Ex Hy* - Ey Hx*
+    static const int pairs[3][4] = {{1, 5, 2, 4}, {0, 5, 2, 3}, {0, 4, 1, 3}};
+    FluxArgs fa{};
+    for (int q = 0; q < 4; ++q) {
+        fa.F[q] = s->res[slot][pairs[axis][q]];
+        fa.st[q] = interp_stencil(s, pairs[axis][q], ek1, ej1);
+    }
+    const int nblk = (int)std::min<long long>(256, (cnt + RED_THREADS - 1) / RED_THREADS);
+    CHECK(ensure_scratch(s, nblk));
+    const IBox dom{k1, k2, j1, j2, i1, i2}, ext{ek1, ek2, ej1, ej2, ei1, ei2};
+    hipLaunchKernelGGL(k_interp_flux, dim3(nblk), dim3(RED_THREADS), 0, s->stream, fa, s->Nx,
+                       s->Ny, dom, ext, s->scratch);
+    HIPCHK(hipGetLastError());
+    double2 part[256];
+    HIPCHK(hipMemcpyAsync(part, s->scratch, nblk * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    double acc = 0.0;
+    for (int b = 0; b < nblk; ++b) acc += part[b].x;
+    *out = acc;
+    return EMF_OK;
+}
+
+int emf_set_plane_weights(emf_solver *s, int id, int64_t count, const double *w0, const double *w1,
+                          const double *w2, const double *w3, const double *w4, const double *w5)
+{
+    LOCK(s);
+    if (id < 0 || id >= emf_solver::kWeightSets) EMF_FAIL(EMF_ERR_ARG, "weight set id must be 0..%d",
+                                                        emf_solver::kWeightSets - 1);
+    if (count < 0) EMF_FAIL(EMF_ERR_ARG, "count must be >= 0");
+    const double *w[6] = { w0, w1, w2, w3, w4, w5 };
+    for (int q = 0; q < 6; ++q) {
+        if (s->wcount[id] != count || !w[q]) { dfree(s->wplane[id][q]); s->wplane[id][q] = nullptr; }
+        if (!w[q] || count == 0) continue;
+        if (!s->wplane[id][q]) CHECK(dalloc(&s->wplane[id][q], (size_t)count));
+        HIPCHK(hipMemcpyAsync(s->wplane[id][q], w[q], count * sizeof(double2), hipMemcpyHostToDevice,
+                              s->stream));
+    }
+    s->wcount[id] = count;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EMF_OK;
+}
+
+int emf_interp_dot(emf_solver *s, int slot, int id, int k1, int k2, int j1, int j2, int i1, int i2,
+                   int ek1, int ek2, int ej1, int ej2, int ei1, int ei2, double *out)
+{
+    LOCK(s);
+    if (id < 0 || id >= emf_solver::kWeightSets || !out) EMF_FAIL(EMF_ERR_ARG, "bad weight set or null out");
+    CHECK(check_interp_boxes(s, slot, k1, k2, j1, j2, i1, i2, ek1, ek2, ej1, ej2, ei1, ei2));
+    const long long cnt = (long long)(k2 - k1) * (j2 - j1) * (i2 - i1);
+    if (cnt != s->wcount[id])
+        EMF_FAIL(EMF_ERR_ARG, "domain has %lld cells, weight set %d has %lld", cnt, id,
+                 (long long)s->wcount[id]);
+    for (int q = 0; q < 4; ++q) out[q] = 0.0;
+    if (cnt == 0) return EMF_OK;
+    DotArgs da{};
+    for (int q = 0; q < 6; ++q) {
+        da.F[q] = s->res[slot][q];
+        da.st[q] = interp_stencil(s, q, ek1, ej1);
+        da.W[q] = s->wplane[id][q];
+    }
+    const int nblk = (int)std::min<long long>(256, (cnt + RED_THREADS - 1) / RED_THREADS);
+    CHECK(ensure_scratch(s, 2 * nblk));
+    const IBox dom{k1, k2, j1, j2, i1, i2}, ext{ek1, ek2, ej1, ej2, ei1, ei2};
+    hipLaunchKernelGGL(k_interp_dot, dim3(nblk), dim3(RED_THREADS), 0, s->stream, da, s->Nx, s->Ny,
+                       dom, ext, s->scratch);
+    HIPCHK(hipGetLastError());
+    double2 part[512];
+    HIPCHK(hipMemcpyAsync(part, s->scratch, 2 * nblk * sizeof(double2), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (int b = 0; b < nblk; ++b) {
+        out[0] += part[2 * b].x; out[1] += part[2 * b].y;
+        out[2] += part[2 * b + 1].x; out[3] += part[2 * b + 1].y;
+    }
+    return EMF_OK;
+}
+
+static YdaxArgs ydax_args(const emf_solver *s)
 {
     YdaxArgs y{};
     for (int c = 0; c < 6; ++c) { y.a[c] = s->res[1][c]; y.f[c] = s->res[0][c]; }
@@ -1855,6 +1982,17 @@ int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc, int order)
     return EMF_OK;
 }
 
+int emf_set_h_shape(emf_solver *s, int tyg, int zc)
+{
+    LOCK(s);
+    if (!(tyg == 0 || tyg == 1 || tyg == 2 || tyg == 4 || tyg == 8))
+        EMF_FAIL(EMF_ERR_ARG, "tyg must be 0 (as the E pass), 1, 2, 4 or 8");
+    if (zc < 0) EMF_FAIL(EMF_ERR_ARG, "zc must be >= 0 (0: as the E pass)");
+    s->tyg_h = tyg;
+    s->zc_h = zc;
+    return EMF_OK;
+}
+
 int emf_sample_structured(int device, const emf_prim *prims, int nprims, const double *vx,
                           const double *vy, int nverts, double dx, double dy, double dz,
                           int supersample, double bg_re, double bg_im, int k1, int k2, int j1,
@@ -1954,6 +2092,17 @@ int emf_sample_structured(int device, const emf_prim *prims, int nprims, const d
     if (cell_err)
         EMF_FAIL(EMF_ERR_ARG, "a cell meets more than %d primitives or %d polygon edges "
                  "(exact overlap rule)", MAT_MAXU, MAT_MAXE);
+    return EMF_OK;
+}
+
+int emf_mem_info(int device, int64_t *free_bytes, int64_t *total_bytes)
+{
+    if (!free_bytes || !total_bytes) EMF_FAIL(EMF_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(device));
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
     return EMF_OK;
 }
 

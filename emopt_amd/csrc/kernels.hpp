@@ -171,6 +171,32 @@ struct IStencil {
     double sx, sy;         // their signs
 };
 
+// the interpolated value of one cell (k, j, i) of the domain
+__device__ __forceinline__ double2 interp_at(const double2 *__restrict__ F, int Nx, int Ny,
+                                             const IBox &ext, const IStencil &st, int k, int j,
+                                             int i)
+{
+    double re = 0.0, im = 0.0;
+    for (int q = 0; q < st.n; ++q) {
+        const int a = i + st.d[q][0];
+        int b = j + st.d[q][1], c = k + st.d[q][2];
+        const bool gy = st.my && b == ext.j1 - 1, gx = st.mx && c == ext.k1 - 1;
+        if (gy) ++b;
+        if (gx) ++c;
+        double vr = 0.0, vi = 0.0;
+        if (a >= ext.i1 && a < ext.i2 && b >= ext.j1 && b < ext.j2 && c >= ext.k1 && c < ext.k2) {
+            const double2 v = F[((long long)a * Ny + b) * Nx + c];
+            vr = v.x;
+            vi = v.y;
+        }
+        if (gx) { vr = st.sx * vr; vi = st.sx * vi; }
+        if (gy) { vr = st.sy * vr; vi = st.sy * vi; }
+        if (q == 0) { re = vr; im = vi; }
+        else { re += vr; im += vi; }
+    }
+    return make_double2(re / (double)st.n, im / (double)st.n);
+}
+
 __global__ void k_field_interp(const double2 *__restrict__ F, int Nx, int Ny, IBox dom, IBox ext,
                                IStencil st, double2 *__restrict__ out)
 {
@@ -180,25 +206,7 @@ __global__ void k_field_interp(const double2 *__restrict__ F, int Nx, int Ny, IB
          g += (long long)gridDim.x * blockDim.x) {
         const int k = dom.k1 + (int)(g % nk), j = dom.j1 + (int)((g / nk) % nj);
         const int i = dom.i1 + (int)(g / ((long long)nk * nj));
-        double re = 0.0, im = 0.0;
-        for (int q = 0; q < st.n; ++q) {
-            const int a = i + st.d[q][0];
-            int b = j + st.d[q][1], c = k + st.d[q][2];
-            const bool gy = st.my && b == ext.j1 - 1, gx = st.mx && c == ext.k1 - 1;
-            if (gy) ++b;
-            if (gx) ++c;
-            double vr = 0.0, vi = 0.0;
-            if (a >= ext.i1 && a < ext.i2 && b >= ext.j1 && b < ext.j2 && c >= ext.k1 && c < ext.k2) {
-                const double2 v = F[((long long)a * Ny + b) * Nx + c];
-                vr = v.x;
-                vi = v.y;
-            }
-            if (gx) { vr = st.sx * vr; vi = st.sx * vi; }
-            if (gy) { vr = st.sy * vr; vi = st.sy * vi; }
-            if (q == 0) { re = vr; im = vi; }
-            else { re += vr; im += vi; }
-        }
-        out[g] = make_double2(re / (double)st.n, im / (double)st.n);
+        out[g] = interp_at(F, Nx, Ny, ext, st, k, j, i);
     }
 }
 
@@ -585,6 +593,80 @@ __global__ __launch_bounds__(RED_THREADS) void k_grad(
     }
     double2 tot = block_sum2(acc);
     if (threadIdx.x == 0) grad[p] = -2 * (tot.x / step);
+}
+
+// ------------------------------------------- reductions over interpolated fields
+
+// Four interpolated components (a, b, c, d) of one result slot on a domain.
+struct FluxArgs {
+    const double2 *F[4];
+    IStencil st[4];
+};
+
+// partial sums of Re(a conj(b) - c conj(d)) over the domain (one per
+// work-group): the Poynting flux integrand of get_source_power
+// (emopt/fdtd.py:1611-1665) with each field interpolated onto the Ez grid
+__global__ __launch_bounds__(RED_THREADS) void k_interp_flux(FluxArgs fa, int Nx, int Ny,
+                                                             IBox dom, IBox ext,
+                                                             double2 *__restrict__ partial)
+{
+    const int nk = dom.k2 - dom.k1, nj = dom.j2 - dom.j1;
+    const long long total = (long long)nk * nj * (dom.i2 - dom.i1);
+    double acc = 0.0;
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < total;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int k = dom.k1 + (int)(g % nk), j = dom.j1 + (int)((g / nk) % nj);
+        const int i = dom.i1 + (int)(g / ((long long)nk * nj));
+        const double2 a = interp_at(fa.F[0], Nx, Ny, ext, fa.st[0], k, j, i);
+        const double2 b = interp_at(fa.F[1], Nx, Ny, ext, fa.st[1], k, j, i);
+        const double2 c = interp_at(fa.F[2], Nx, Ny, ext, fa.st[2], k, j, i);
+        const double2 d = interp_at(fa.F[3], Nx, Ny, ext, fa.st[3], k, j, i);
+        // Re(a conj(b)) - Re(c conj(d))
+        acc += (a.x * b.x + a.y * b.y) - (c.x * d.x + c.y * d.y);
+    }
+    const double2 tot = block_sum2(make_double2(acc, 0.0));
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// The six interpolated components of one result slot, and six weight planes
+// of the domain's shape (a null plane weighs 0).
+struct DotArgs {
+    const double2 *F[6];
+    IStencil st[6];
+    const double2 *W[6];
+};
+
+// partial sums over the domain of sum_q W_q * field_q, separately for the E
+// components (q = 0..2, partial[2 b]) and the H components (q = 3..5,
+// partial[2 b + 1]): the overlap integrals of ModeMatch
+// (emopt/fomutils.py:1047-1076) with the fields interpolated onto the Ez grid
+__global__ __launch_bounds__(RED_THREADS) void k_interp_dot(DotArgs da, int Nx, int Ny, IBox dom,
+                                                            IBox ext,
+                                                            double2 *__restrict__ partial)
+{
+    const int nk = dom.k2 - dom.k1, nj = dom.j2 - dom.j1;
+    const long long total = (long long)nk * nj * (dom.i2 - dom.i1);
+    double2 ae = make_double2(0.0, 0.0), ah = make_double2(0.0, 0.0);
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < total;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int k = dom.k1 + (int)(g % nk), j = dom.j1 + (int)((g / nk) % nj);
+        const int i = dom.i1 + (int)(g / ((long long)nk * nj));
+        for (int q = 0; q < 6; ++q) {
+            if (!da.W[q]) continue;
+            const double2 w = da.W[q][g];
+            const double2 f = interp_at(da.F[q], Nx, Ny, ext, da.st[q], k, j, i);
+            const double2 t = cmul(w, f);
+            if (q < 3) { ae.x += t.x; ae.y += t.y; }
+            else { ah.x += t.x; ah.y += t.y; }
+        }
+    }
+    const double2 te = block_sum2(ae);
+    __syncthreads();                       // block_sum2's LDS is reused
+    const double2 th = block_sum2(ah);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = te;
+        partial[2 * blockIdx.x + 1] = th;
+    }
 }
 
 }  // namespace emf

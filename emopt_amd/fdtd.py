@@ -361,12 +361,15 @@ class FDTD(MaxwellSolver):
         (self._adj_sources if adjoint else self._sources).append(sa)
 
     def set_sources(self, src, domain, mindex=0):
-        """Add a source given as (Jx, Jy, Jz, Mx, My, Mz) arrays on `domain`
-        (emopt/fdtd.py:749-788).  Mode-solver objects are not supported: the
-        eigen-mode solvers are outside this package."""
+        """Add a source given as (Jx, Jy, Jz, Mx, My, Mz) arrays, or as a
+        solved :class:`emopt_amd.modes.ModeFullVector` whose mode `mindex`
+        is launched, on `domain` (emopt/fdtd.py:749-788)."""
+        from .modes import ModeFullVector
+        if isinstance(src, ModeFullVector):
+            src = src.get_source(mindex, self._dx, self._dy, self._dz)
         if not isinstance(src, (tuple, list)) or len(src) != 6:
-            raise NotImplementedError("sources must be six current-density arrays "
-                                      "(Jx, Jy, Jz, Mx, My, Mz)")
+            raise ValueError("sources must be six current-density arrays (Jx, Jy, Jz, Mx, My, "
+                             "Mz) or a ModeFullVector")
         self._set_sources(src, domain, adjoint=False)
 
     def set_adjoint_sources(self, src):
@@ -472,6 +475,42 @@ class FDTD(MaxwellSolver):
                                            ptr(out))
         return out
 
+    def _flux_device(self, axis, dom, ext, slot):
+        """One face of get_source_power reduced on the GPU
+        (emf_interp_flux): sum of Re(a conj(b) - c conj(d)) of the fields
+        interpolated on dom (grown box ext)."""
+        if not self._have[slot]:
+            raise RuntimeError("no %s solution available" % ("forward", "adjoint")[slot])
+        out = np.zeros(1)
+        self._lib.emf_interp_flux(self._h, slot, int(axis), *dom, *ext, ptr(out))
+        return float(out[0])
+
+    def set_plane_weights(self, wid, weights):
+        """Upload six weight planes (Ex..Hz, None: zero) of one domain's shape
+        as set `wid` (0..7) for interp_dot."""
+        arrs = [None if w is None else c128(w).ravel() for w in weights]
+        count = next((a.size for a in arrs if a is not None), 0)
+        if any(a is not None and a.size != count for a in arrs):
+            raise ValueError("weight planes must have one shape")
+        self._lib.emf_set_plane_weights(self._h, int(wid), int(count), *[ptr(a) for a in arrs])
+        self._weights_shape = getattr(self, "_weights_shape", {})
+        self._weights_shape[wid] = count
+
+    def interp_dot(self, wid, domain, adjoint=False):
+        """(sum_E, sum_H) = sums over `domain` of weight set `wid` times the
+        fields interpolated onto the Ez grid (E components, H components),
+        reduced on the GPU (emf_interp_dot)."""
+        from .fields import grown_boxes
+        slot = 1 if adjoint else 0
+        if not self._have[slot]:
+            raise RuntimeError("no %s solution available" % ("forward", "adjoint")[slot])
+        _, _, dom, ext, ok = grown_boxes(self, domain)
+        if not ok:
+            raise ValueError("interp_dot needs a domain inside the grid")
+        out = np.zeros(4)
+        self._lib.emf_interp_dot(self._h, slot, int(wid), *dom, *ext, ptr(out))
+        return complex(out[0], out[1]), complex(out[2], out[3])
+
     def get_field(self, component, domain=None):
         """Raw (uninterpolated) forward field (emopt/fdtd.py:1322-1380)."""
         return self._get(component, domain, 0)
@@ -525,7 +564,8 @@ class FDTD(MaxwellSolver):
         """grad[p] = -2 Re(sum E_adj * 1j * deps * E) / step over sparse
         (parameter, component, cell, dEps) entries, on the device
         (k_grad): the reference's per-parameter loop
-        (emopt/adjoint_method.py:504-533) in one pass."""
+        (emopt/adjoint_method.py:504-533) in one pass.  Components 3..5 are
+        mu_x..mu_z entries (term -1j * dmu * H_adj * H)."""
         pidx = np.asarray(pidx, dtype=np.int32)
         order = np.argsort(pidx, kind="stable")
         pidx = np.ascontiguousarray(pidx[order])

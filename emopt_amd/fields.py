@@ -4,8 +4,9 @@ the Ez grid and the Poynting source power.
 Semantics of emopt/fdtd.py:1401-1553 (``get_field_interp``) and
 emopt/fdtd.py:1555-1667 (``get_source_power``).  On a solver the averaging
 runs on its GPU (emf_get_field_interp) for domains inside the grid, and only
-the domain comes back; the numpy restatement below serves other domains and
-solver stand-ins, and is what the device result is tested against.
+the domain comes back; the source power is reduced there too
+(emf_interp_flux), so only six scalars come back.  The numpy restatement
+below serves other domains and solver stand-ins.
 """
 from __future__ import annotations
 
@@ -50,11 +51,14 @@ def _avg(fr, comp):
     return out / float(len(_STENCIL[comp]))
 
 
-def interp_to_ez(sim, component, domain=None, squeeze=False, adjoint=False):
-    get = sim.get_adjoint_field if adjoint else sim.get_field
-    if component == "Ez":
-        f = get(component, domain)
-        return np.squeeze(f) if squeeze else f
+def grown_boxes(sim, domain):
+    """The domain get_field_interp averages on and the box it reads
+    (emopt/fdtd.py:1427-1457): (domain, dint, dom, ext, on_device) with dint
+    the grown box as a DomainCoordinates, dom and ext as (k1, k2, j1, j2, i1,
+    i2) -- ext is the domain grown by one cell at each min face above 0 and
+    at each max face below N - 1, rounded through DomainCoordinates as the
+    reference rounds it -- and on_device True when both lie in the grid (the
+    device kernels' precondition)."""
     Nx, Ny, Nz = sim.Nx, sim.Ny, sim.Nz
     dx, dy, dz = sim.dx, sim.dy, sim.dz
     if domain is None:
@@ -69,13 +73,25 @@ def interp_to_ez(sim, component, domain=None, squeeze=False, adjoint=False):
         if i1 > 0: i1 -= 1
         if i2 < Nz - 1: i2 += 1
         dint = DomainCoordinates(k1 * dx, k2 * dx, j1 * dy, j2 * dy, i1 * dz, i2 * dz, dx, dy, dz)
-    k1, k2, j1, j2, i1, i2 = dint.k1, dint.k2, dint.j1, dint.j2, dint.i1, dint.i2
+    ext = (dint.k1, dint.k2, dint.j1, dint.j2, dint.i1, dint.i2)
+    dom = (domain.k1, domain.k2, domain.j1, domain.j2, domain.i1, domain.i2)
+    k1, k2, j1, j2, i1, i2 = ext
+    ok = (0 <= k1 <= dom[0] <= dom[1] <= k2 <= Nx and 0 <= j1 <= dom[2] <= dom[3] <= j2 <= Ny
+          and 0 <= i1 <= dom[4] <= dom[5] <= i2 <= Nz)
+    return domain, dint, dom, ext, ok
+
+
+def interp_to_ez(sim, component, domain=None, squeeze=False, adjoint=False):
+    get = sim.get_adjoint_field if adjoint else sim.get_field
+    if component == "Ez":
+        f = get(component, domain)
+        return np.squeeze(f) if squeeze else f
+    domain, dint, dom, ext, ok = grown_boxes(sim, domain)
+    k1, k2, j1, j2, i1, i2 = ext
     dev = getattr(sim, "_interp_device", None)
-    if dev is not None and 0 <= k1 <= domain.k1 <= domain.k2 <= k2 <= Nx and \
-            0 <= j1 <= domain.j1 <= domain.j2 <= j2 <= Ny and 0 <= i1 <= domain.i1 <= domain.i2 <= i2 <= Nz:
+    if dev is not None and ok:
         # on the solver's GPU: the same stencil sums on the same grown box
-        f = dev(component, (domain.k1, domain.k2, domain.j1, domain.j2, domain.i1, domain.i2),
-                (k1, k2, j1, j2, i1, i2), 1 if adjoint else 0)
+        f = dev(component, dom, ext, 1 if adjoint else 0)
         return np.squeeze(f) if squeeze else f
     fr = np.pad(get(component, dint), 1, "constant", constant_values=0)
     bc = sim.bc
@@ -95,7 +111,19 @@ def interp_to_ez(sim, component, domain=None, squeeze=False, adjoint=False):
     return np.squeeze(f) if squeeze else f
 
 
+# faces of the source-power box: (axis, min face?, sign, the reference's
+# component pairs a conj(b) - c conj(d)) in the order of emopt/fdtd.py:1611-1665
+_FACES = ((0, True, -1.0, ("Ey", "Hz", "Ez", "Hy")), (0, False, 1.0, ("Ey", "Hz", "Ez", "Hy")),
+          (1, True, 1.0, ("Ex", "Hz", "Ez", "Hx")), (1, False, -1.0, ("Ex", "Hz", "Ez", "Hx")),
+          (2, True, -1.0, ("Ex", "Hy", "Ey", "Hx")), (2, False, 1.0, ("Ex", "Hy", "Ey", "Hx")))
+
+
 def source_power(sim):
+    """Poynting flux out of the box one cell inside the CPML
+    (emopt/fdtd.py:1555-1667), min faces with an 'E'/'H' symmetry skipped.
+    On a solver each face's sum is reduced on its GPU (emf_interp_flux) and
+    only a scalar comes back; otherwise the interpolated planes are summed
+    here."""
     dx, dy, dz = sim.dx, sim.dy, sim.dz
     w = sim.w_pml
     X, Y, Z = sim.X, sim.Y, sim.Z
@@ -106,32 +134,27 @@ def source_power(sim):
     zmin = w[4] + dz if w[4] > 0 else 0.0
     zmax = Z - w[5] - dz if w[5] > 0 else Z - dz
     D = lambda *a: DomainCoordinates(*a, dx, dy, dz)
-    g = sim.get_field_interp
+    planes = {(0, True): D(xmin, xmin, ymin, ymax, zmin, zmax),
+              (0, False): D(xmax, xmax, ymin, ymax, zmin, zmax),
+              (1, True): D(xmin, xmax, ymin, ymin, zmin, zmax),
+              (1, False): D(xmin, xmax, ymax, ymax, zmin, zmax),
+              (2, True): D(xmin, xmax, ymin, ymax, zmin, zmin),
+              (2, False): D(xmin, xmax, ymin, ymax, zmax, zmax)}
+    area = (dy * dz, dx * dz, dx * dy)
     bc = sim.bc
+    flux_dev = getattr(sim, "_flux_device", None)
+    g = sim.get_field_interp
     re = np.real
     P = 0.0
-
-    x1 = D(xmin, xmin, ymin, ymax, zmin, zmax)
-    Ey, Ez, Hy, Hz = g("Ey", x1), g("Ez", x1), g("Hy", x1), g("Hz", x1)
-    if bc[0] not in "EH":
-        P += -0.5 * dy * dz * np.sum(re(Ey * np.conj(Hz) - Ez * np.conj(Hy)))
-    x2 = D(xmax, xmax, ymin, ymax, zmin, zmax)
-    Ey, Ez, Hy, Hz = g("Ey", x2), g("Ez", x2), g("Hy", x2), g("Hz", x2)
-    P += 0.5 * dy * dz * np.sum(re(Ey * np.conj(Hz) - Ez * np.conj(Hy)))
-
-    y1 = D(xmin, xmax, ymin, ymin, zmin, zmax)
-    Ex, Ez, Hx, Hz = g("Ex", y1), g("Ez", y1), g("Hx", y1), g("Hz", y1)
-    if bc[1] not in "EH":
-        P += 0.5 * dx * dz * np.sum(re(Ex * np.conj(Hz) - Ez * np.conj(Hx)))
-    y2 = D(xmin, xmax, ymax, ymax, zmin, zmax)
-    Ex, Ez, Hx, Hz = g("Ex", y2), g("Ez", y2), g("Hx", y2), g("Hz", y2)
-    P += -0.5 * dx * dz * np.sum(re(Ex * np.conj(Hz) - Ez * np.conj(Hx)))
-
-    z1 = D(xmin, xmax, ymin, ymax, zmin, zmin)
-    Ex, Ey, Hx, Hy = g("Ex", z1), g("Ey", z1), g("Hx", z1), g("Hy", z1)
-    if bc[2] not in "EH":
-        P += -0.5 * dx * dy * np.sum(re(Ex * np.conj(Hy) - Ey * np.conj(Hx)))
-    z2 = D(xmin, xmax, ymin, ymax, zmax, zmax)
-    Ex, Ey, Hx, Hy = g("Ex", z2), g("Ey", z2), g("Hx", z2), g("Hy", z2)
-    P += 0.5 * dx * dy * np.sum(re(Ex * np.conj(Hy) - Ey * np.conj(Hx)))
+    for axis, is_min, sign, (a, b, c, d) in _FACES:
+        if is_min and bc[axis] in "EH":
+            continue
+        dom = planes[(axis, is_min)]
+        if flux_dev is not None:
+            _, _, db, ext, ok = grown_boxes(sim, dom)
+            if ok:
+                P += sign * 0.5 * area[axis] * flux_dev(axis, db, ext, 0)
+                continue
+        A, B, C, Dd = g(a, dom), g(b, dom), g(c, dom), g(d, dom)
+        P += sign * 0.5 * area[axis] * np.sum(re(A * np.conj(B) - C * np.conj(Dd)))
     return P

@@ -33,15 +33,23 @@ __all__ = ["Request", "Server", "state_bytes"]
 BATCH_MAX = 128
 
 
-def state_bytes(Nx, Ny, Nz, w, slots=1):
+def state_bytes(Nx, Ny, Nz, w, slots=1, wavelength=None, d=None):
     """Device bytes of one solver: E and H (48 B/cell), update coefficients
     (24 B) and their compressed index (up to 4 B), the eps copy kept for box
-    updates (48 B), result slots (96 B each) and the CPML psi slabs (E and H
-    sets, 2 components each)."""
+    updates (48 B), result slots (96 B each), the CPML psi slabs (E and H
+    sets, 2 components each), the material sampler's staging during the
+    build (16 B/cell), and, given the wavelength and cell size, the batch's
+    per-step factor table (64 B per step up to the step limit
+    500 wavelength/d + the two capture periods) and convergence history."""
     n = Nx * Ny * Nz
     slab = 2 * (Ny * Nz * w[0] + Ny * Nz * w[1] + Nx * Nz * w[2] + Nx * Nz * w[3]
                 + Nx * Ny * w[4] + Nx * Ny * w[5])
-    return n * (48 + 24 + 4 + 48 + 96 * slots) + 2 * slab * 8
+    total = n * (48 + 24 + 4 + 48 + 96 * slots) + 2 * slab * 8
+    if wavelength is not None and d is not None:
+        nlambda = wavelength / d
+        steps = 500 * nlambda + 4 * np.sqrt(3) * nlambda + 1
+        total += int(steps) * 64 + int(steps / max(1.0, 0.75 * np.sqrt(3) * nlambda) + 4) * 24
+    return int(total) + (n * 16 if wavelength is not None else 0)
 
 
 class Request:
@@ -70,7 +78,7 @@ class Request:
 class Server:
     def __init__(self, device=0, hbm_budget=200e9, prep_workers=4, steps_per_graph=32, zc=2,
                  blocks_per_cu=16, graphs_per_poll=4, on_done=None, batch=None, max_running=8,
-                 prep_ahead=None):
+                 prep_ahead=None, free_hbm=None, hbm_reserve=2 << 30):
         self.batch = batch if batch is not None else Batch(device, steps_per_graph, zc,
                                                            blocks_per_cu)
         self.budget = float(hbm_budget)
@@ -84,6 +92,13 @@ class Server:
         self.on_done = on_done
         self.max_running = min(int(max_running), BATCH_MAX)
         self.prep_ahead = prep_workers if prep_ahead is None else int(prep_ahead)
+        # free device memory: the batch's GPU by default (a stand-in batch
+        # without the query has none to report)
+        if free_hbm is None:
+            free_hbm = getattr(self.batch, "free_bytes", None)
+        self.free_hbm = free_hbm
+        self.hbm_reserve = float(hbm_reserve)
+        self.waits_on_free = 0         # admissions deferred by free memory
 
     # ------------------------------------------------------------ requests
     def submit(self, req, now=None):
@@ -95,11 +110,26 @@ class Server:
             return
         self.queue.append(req)
 
+    def _fits_free(self, head):
+        """The device has room for `head` now: free memory, less the reserve
+        and less what admitted requests still building will allocate."""
+        if self.free_hbm is None:
+            return True
+        pending = sum(r.nbytes for r, f in self.preparing if not f.done())
+        if head.nbytes + pending <= self.free_hbm() - self.hbm_reserve:
+            return True
+        if not self.running and not self.preparing:
+            return True            # nothing will free memory: try, and fail alone if it is short
+        self.waits_on_free += 1
+        return False
+
     def _admit(self):
         while self.queue:
             head = self.queue[0]
             busy = len(self.running) + len(self.preparing)
             if busy >= self.max_running + self.prep_ahead or self.used + head.nbytes > self.budget:
+                break
+            if not self._fits_free(head):
                 break
             self.queue.popleft()
             self.used += head.nbytes

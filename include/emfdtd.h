@@ -252,6 +252,33 @@ EMF_API int emf_get_field_interp(emf_solver *s, int slot, int comp, int k1, int 
                                  int j2, int i1, int i2, int ek1, int ek2, int ej1, int ej2,
                                  int ei1, int ei2, double *out);
 
+/* Poynting flux integrand of one face of get_source_power
+ * (emopt/fdtd.py:1611-1665) reduced on the device: out = sum over the domain
+ * of Re(a conj(b) - c conj(d)) with the fields interpolated as in
+ * emf_get_field_interp (same domain and grown box), where (a, b, c, d) is
+ * (Ey, Hz, Ez, Hy) for axis 0, (Ex, Hz, Ez, Hx) for axis 1 and
+ * (Ex, Hy, Ey, Hx) for axis 2.  The caller applies the face's sign and
+ * 0.5 * area.  Replaces the host sums over four downloaded planes. */
+EMF_API int emf_interp_flux(emf_solver *s, int slot, int axis, int k1, int k2, int j1, int j2,
+                    int i1, int i2, int ek1, int ek2, int ej1, int ej2, int ei1, int ei2,
+                    double *out);
+
+/* Stores up to six complex weight planes (NULL: weight 0) of `count` cells
+ * on the device as set `id` (0..7) of this solver, for emf_interp_dot.  No
+ * counterpart in the reference. */
+EMF_API int emf_set_plane_weights(emf_solver *s, int id, int64_t count, const double *w_ex,
+                          const double *w_ey, const double *w_ez, const double *w_hx,
+                          const double *w_hy, const double *w_hz);
+
+/* sum over the domain of sum_q W_q * field_q, the fields interpolated as in
+ * emf_get_field_interp and W the weight set `id` (its cell count must be the
+ * domain's): out[0..1] = the E components' sum (re, im), out[2..3] = the H
+ * components'.  The overlap integrals of ModeMatch.compute
+ * (emopt/fomutils.py:1047-1076) reduced on the device, with weights formed
+ * once from the reference mode. */
+EMF_API int emf_interp_dot(emf_solver *s, int slot, int id, int k1, int k2, int j1, int j2, int i1,
+                   int i2, int ek1, int ek2, int ej1, int ej2, int ei1, int ei2, double *out);
+
 /* ---- measurement (no counterpart in the reference) --------------------- */
 
 /* Runs `nsteps` time steps starting at step index n_start on the current
@@ -277,11 +304,22 @@ EMF_API int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64
  * identical; only speed differs. */
 EMF_API int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc, int order);
 
+/* Work-group shape of the H pass of the two-pass kernels (kinds 1 and 3):
+ * rows (waves) per work-group tyg in {1, 2, 4, 8} and z planes per work-group
+ * zc >= 1; 0 keeps the E pass's value (emf_set_tuning).  A launch shape only:
+ * results are the same.  No counterpart in the reference. */
+EMF_API int emf_set_h_shape(emf_solver *s, int tyg, int zc);
+
 /* Streaming (non-temporal) access to the CPML psi arrays in the two passes:
  * -1 (default) on when a step's data -- fields, coefficients and psi --
  * exceeds the 256 MB Infinity Cache, 0 off, 1 on.  A cache policy only:
  * results are the same either way.  No counterpart in the reference. */
 EMF_API int emf_set_streaming(emf_solver *s, int mode);
+
+/* Free and total device memory of `device` in bytes (hipMemGetInfo).  No
+ * counterpart in the reference; the solve server admits requests against
+ * it. */
+EMF_API int emf_mem_info(int device, int64_t *free_bytes, int64_t *total_bytes);
 
 /* Device copy bandwidth (GB/s, read + write bytes) of a plain vector copy of
  * `bytes` bytes, median of `reps` launches; a roofline reference point. */

@@ -406,10 +406,13 @@ class OracleFDTD:
     _NAMES = ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"]
 
     def get_field(self, comp, domain=None, adjoint=False):
+        """Raw phasor of one component on a domain (emopt/fdtd.py:1322-1359);
+        no domain means DomainCoordinates(0, X, 0, Y, 0, Z), which drops the
+        last plane along an axis where (N - 1) * d / d rounds below N - 1."""
         q = self._NAMES.index(comp)
         f = (self.adj if adjoint else self.fwd)[0][q].reshape(self.Nz, self.Ny, self.Nx)
         if domain is None:
-            return f.copy()
+            domain = Domain(0, self._X, 0, self._Y, 0, self._Z, self._dx, self._dy, self._dz)
         return f[domain.i, domain.j, domain.k].copy()
 
     def get_adjoint_field(self, comp, domain=None):

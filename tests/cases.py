@@ -275,6 +275,20 @@ def build_sim(case, device=0):
     return sim
 
 
+def case_from_sim(sim, rtol):
+    """The oracle's inputs for a configured solver: the material arrays it
+    holds on the device, its sources, CPML, bc and ramp."""
+    shp = (sim.Nz, sim.Ny, sim.Nx)
+    A = sim.get_A_diag()
+    srcs = [([a.reshape(s.I, s.J, s.K) for a in s.arrays],
+             (s.k0, s.k0 + s.K, s.j0, s.j0 + s.J, s.i0, s.i0 + s.I)) for s in sim._sources]
+    d = sim.dx
+    return dict(Nx=sim.Nx, Ny=sim.Ny, Nz=sim.Nz, d=d, wl=sim.wavelength,
+                X=span(sim.Nx, d), Y=span(sim.Ny, d), Z=span(sim.Nz, d), w=list(sim._w_cells),
+                rtol=rtol, ramp=sim.src_ramp_time, bc="".join(sim.bc),
+                mats=[a.reshape(shp) for a in A], sources=srcs)
+
+
 def all_fields(sim, case, adjoint=False):
     from emopt_amd.misc import DomainCoordinates
     d = case["d"]
@@ -315,6 +329,7 @@ def grating_problem(Nx=360, Ny=60, Nz=120, d=0.03, ntr=32, w=10, wl=1.55, seed=4
     Returns (sim, am, params)."""
     import warnings
     from emopt_amd.adjoint import AdjointMethodPNF3D, ModeMatch, rect_edge_deps
+    from emopt_amd.shape import boundary_deps, rect_edge_velocity
     from emopt_amd.fdtd import FDTD
     from emopt_amd.grid import StructuredMaterial3D, Rectangle, ConstantMaterial3D
     from emopt_amd.misc import DomainCoordinates, gaussian_mode
@@ -360,7 +375,6 @@ def grating_problem(Nx=360, Ny=60, Nz=120, d=0.03, ntr=32, w=10, wl=1.55, seed=4
     sim.set_sources(arrs, DomainCoordinates.from_indices(ks, ks + 1, 0, Ny, i0, i1, d, d, d))
     # FOM plane and target beam
     mon = DomainCoordinates(wd + d, X - wd - d, wd + d, Y - wd - d, z_mon, z_mon, d, d, d)
-    sim.field_domains = [mon]
     xs = mon.x - 0.5 * (x_start + x_end)
     ys = mon.y - Y / 2
     XX, YY = np.meshgrid(xs, ys)
@@ -382,29 +396,35 @@ def grating_problem(Nx=360, Ny=60, Nz=120, d=0.03, ntr=32, w=10, wl=1.55, seed=4
             return [mon]
 
         def calc_f(self, sim, p):
-            Ex, Ey, Ez, Hx, Hy, Hz = sim.saved_fields[0]
-            mm.compute(Ex=Ex, Ey=Ey, Hx=Hx, Hy=Hy)
+            mm.compute_device(sim, mon)          # overlap sums reduced on the GPU
             return mm.get_mode_match_forward(1.0)
 
         def calc_dfdx(self, sim, p):
-            Ex, Ey, Ez, Hx, Hy, Hz = sim.saved_fields[0]
-            mm.compute(Ex=Ex, Ey=Ey, Hx=Hx, Hy=Hy)
+            mm.compute_device(sim, mon)
             zero = np.zeros(mon.shape, dtype=np.complex128)
             return [(mm.get_dFdEx(), mm.get_dFdEy(), zero, mm.get_dFdHx(), mm.get_dFdHy(), zero)]
 
         def calc_deps(self, sim, p):
+            """d eps / d edge for every edge: analytic from the overlap rule
+            (deps="analytic", the default), or central differences of the
+            device sampling (deps="difference")."""
             out = [[], [], [], []]
             for i in range(len(p)):
                 r = trenches[i // 2]
-                k = int(p[i] / d)
-                comp, cell, dv = rect_edge_deps(sim, eps, r, "xl" if i % 2 == 0 else "xr",
-                                                1e-6 * d, (max(k - 2, 0), min(k + 3, sim.Nx), 0,
-                                                           sim.Ny, e_i0, e_i1))
+                edge = "xl" if i % 2 == 0 else "xr"
+                if self.deps == "analytic":
+                    comp, cell, dv = boundary_deps(sim, eps, r, rect_edge_velocity(edge))
+                else:
+                    k = int(p[i] / d)
+                    comp, cell, dv = rect_edge_deps(sim, eps, r, edge, 1e-6 * d,
+                                                    (max(k - 2, 0), min(k + 3, sim.Nx), 0,
+                                                     sim.Ny, e_i0, e_i1))
                 out[0].append(np.full(len(comp), i, np.int32))
                 out[1].append(comp); out[2].append(cell); out[3].append(dv)
             return tuple(np.concatenate(a) for a in out)
 
     am = GratingAM(sim, step=1e-8)
+    am.deps = "analytic"
     am.update_system(params)
     return sim, am, params
 
@@ -451,7 +471,8 @@ def serve_request(q, **kw):
 
     def build():
         return build_sim(serve_case(q, **kw))
-    req = Request(q, build, state_bytes(c["Nx"], c["Ny"], c["Nz"], c["w"]), adjoint=False,
+    req = Request(q, build, state_bytes(c["Nx"], c["Ny"], c["Nz"], c["w"], wavelength=c["wl"],
+                                        d=c["d"]), adjoint=False,
                   cells=c["Nx"] * c["Ny"] * c["Nz"],
                   meta=dict(dims=(c["Nx"], c["Ny"], c["Nz"]), wl=c["wl"], d=c["d"]))
     return req

@@ -156,3 +156,53 @@ def test_point_detector_gradient_inside_cpml():
     scale = max(abs(f) for _, f in res)
     for exact, fd in res:
         assert abs(exact - fd) <= 1e-4 * max(abs(fd), scale), (exact, fd)
+
+
+def test_backward_after_another_forward_uses_its_own_params(problem, grads):
+    """fdtd_fom at p, then at a second point, then backward of the first:
+    the backward must not reuse the second point's forward solve (the
+    reference's prev_params guard, emopt/adjoint_method.py:569-574)."""
+    import torch
+    from emopt_amd.adjoint import fdtd_fom
+    sim, am, p = problem
+    _, g_dev, _, _ = grads
+    p1 = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+    q = p.copy()
+    q[3] += 0.02
+    p2 = torch.tensor(q, dtype=torch.float64, requires_grad=True)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        F1 = fdtd_fom(p1, am)
+        F2 = fdtd_fom(p2, am)
+        F1.backward()
+        F2.backward()
+        g2 = am.gradient_device(q)
+        am.fom(p)
+    print("F1 %.6e F2 %.6e" % (float(F1.detach()), float(F2.detach())))
+    assert float(F1.detach()) != float(F2.detach())
+    np.testing.assert_allclose(p1.grad.numpy(), g_dev, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(p2.grad.numpy(), g2, rtol=1e-12, atol=0)
+
+
+def test_analytic_deps_match_device_differences(problem):
+    """The grating's d eps / d edge from the overlap rule's geometry
+    (emopt_amd/shape.py) against central differences of the solver's device
+    sampling, entry by entry over all parameters, components and cells."""
+    sim, am, p = problem
+
+    def dense(deps):
+        pidx, comp, cell, dv = deps
+        out = {}
+        for a, b, c, v in zip(pidx, comp, cell, dv):
+            out[(int(a), int(b), int(c))] = out.get((int(a), int(b), int(c)), 0) + v
+        return out
+    am.deps = "difference"
+    cd = dense(am.calc_deps(sim, p))
+    am.deps = "analytic"
+    an = dense(am.calc_deps(sim, p))
+    scale = max(abs(v) for v in cd.values())
+    keys = set(cd) | set(an)
+    err = max(abs(an.get(k, 0) - cd.get(k, 0)) for k in keys)
+    print("%d entries (analytic %d, differenced %d), max |deps| %.3e, max error %.3e"
+          % (len(keys), len(an), len(cd), scale, err))
+    assert err <= 1e-7 * scale

@@ -32,8 +32,8 @@ import warnings
 import numpy as np
 import pytest
 
-from tests.cases import (all_fields, batch_case, build_sim, large_case, rel_l2, run_gpu,
-                         run_oracle, serve_case, serve_request, span, strip_case)
+from tests.cases import (all_fields, batch_case, build_sim, case_from_sim, large_case, rel_l2,
+                         run_gpu, run_oracle, serve_case, serve_request, strip_case)
 
 pytestmark = pytest.mark.gpu
 
@@ -125,20 +125,6 @@ def test_batch_32_solves_full_size():
         assert max(errs) <= TOL
 
 
-def _case_from_sim(sim, rtol):
-    """The oracle's inputs for a configured solver: the material arrays it
-    holds on the device, its sources, CPML, bc and ramp."""
-    shp = (sim.Nz, sim.Ny, sim.Nx)
-    A = sim.get_A_diag()
-    srcs = [([a.reshape(s.I, s.J, s.K) for a in s.arrays],
-             (s.k0, s.k0 + s.K, s.j0, s.j0 + s.J, s.i0, s.i0 + s.I)) for s in sim._sources]
-    d = sim.dx
-    return dict(Nx=sim.Nx, Ny=sim.Ny, Nz=sim.Nz, d=d, wl=sim.wavelength,
-                X=span(sim.Nx, d), Y=span(sim.Ny, d), Z=span(sim.Nz, d), w=list(sim._w_cells),
-                rtol=rtol, ramp=sim.src_ramp_time, bc="".join(sim.bc),
-                mats=[a.reshape(shp) for a in A], sources=srcs)
-
-
 def test_grating_360x60x120_gradient_and_fields():
     import torch
     from emopt_amd.adjoint import fdtd_fom
@@ -151,7 +137,7 @@ def test_grating_360x60x120_gradient_and_fields():
     assert (sim.Nx, sim.Ny, sim.Nz) == (360, 60, 120) and len(p) == 64
     n0 = sim.last_solve["n0"]
     # forward fields against the oracle at a fixed step count
-    case = _case_from_sim(sim, rtol)
+    case = case_from_sim(sim, rtol)
     nf = 4 * sim.Tn
     sim.fixed_n0 = nf
     with warnings.catch_warnings():

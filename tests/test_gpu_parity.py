@@ -248,3 +248,33 @@ def test_psi_streaming_is_exact():
         b = _run_steps(case, 41, kind=1, tyg=tyg, zc=zc, order=3, stream=0)
         for x, y in zip(a, b):
             assert np.array_equal(x, y), (tyg, zc)
+
+
+def test_h_pass_shape_is_launch_shape_only():
+    """The H pass's own work-group shape (emf_set_h_shape) changes the
+    launch, not the arithmetic: the fields after 25 steps are identical to
+    the default shape's, for several rows x planes per work-group."""
+    case = random_case(70, 23, 19, w=(3, 4, 2, 3, 4, 3), seed=12, nsrc=2)
+    ref = _step_gpu(case, 25)
+    import warnings
+    from emopt_amd.fdtd import FDTD
+    from emopt_amd.misc import DomainCoordinates
+    from tests.cases import _configure
+    for tyg, zc in ((1, 1), (2, 3), (8, 4), (4, 7), (1, 19)):
+        d = case["d"]
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            sim = FDTD(case["X"], case["Y"], case["Z"], d, d, d, case["wl"])
+            _configure(sim, case, DomainCoordinates)
+        sim.set_material_arrays(case["mats"][:3], case["mats"][3:])
+        sim._register(sim._sources)
+        L, h = sim._lib, sim._h
+        L.emf_set_h_shape(h, tyg, zc)
+        L.emf_reset_pml(h)
+        L.emf_zero_fields(h)
+        for n in range(25):
+            L.emf_update_H(h, n, n * sim.dt)
+            L.emf_update_E(h, n, (n + 0.5) * sim.dt)
+        got = sim.time_fields()
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), (tyg, zc)

@@ -67,3 +67,24 @@ def test_budget_limits_concurrency():
     budget = 2.2 * max(sizes)
     out, st = _serve(range(4), [0.0] * 4, budget=budget)
     assert st["completed"] == 4
+
+
+def test_state_estimate_covers_device_allocation():
+    """state_bytes (the admission estimate) against the device memory a
+    request of the largest serving size (320x240x120) actually takes once
+    built and admitted to a batch: the estimate covers it, within 1.5x."""
+    from emopt_amd.batch import Batch
+    from emopt_amd.serve import state_bytes
+    b = Batch()
+    c = serve_case(0, s=1.0)
+    assert (c["Nx"], c["Ny"], c["Nz"]) == (320, 240, 120)
+    free0 = b.free_bytes()
+    sim = build_sim(c)
+    mid = b.add(sim, postprocess=False)
+    used = free0 - b.free_bytes()
+    est = state_bytes(c["Nx"], c["Ny"], c["Nz"], c["w"], wavelength=c["wl"], d=c["d"])
+    print("320x240x120 request: device memory taken %.3f GB, estimate %.3f GB" % (used / 1e9,
+                                                                                 est / 1e9))
+    b.remove(mid)
+    b.close()
+    assert 0 < used <= est <= 1.5 * used

@@ -154,6 +154,24 @@ def test_source_power_closed_form(kind, axis, bc):
     assert power() == pytest.approx(expect, rel=1e-12, abs=1e-14)
 
 
+@pytest.mark.parametrize("kind", ["oracle", "host"])
+def test_default_domain_drops_a_plane_where_the_span_rounds_down(kind):
+    """get_field without a domain takes DomainCoordinates(0, X, 0, Y, 0, Z)
+    (emopt/fdtd.py:1325-1327) with X = (N - 1) d: int(X / d) + 1 is N - 1
+    where (N - 1) d / d rounds below N - 1 (29 * 0.04 / 0.04 = 28.999...),
+    so that axis comes back one plane short; elsewhere it is whole."""
+    d, N = 0.04, (36, 30, 24)
+    assert int((N[1] - 1) * d / d) == N[1] - 2          # y rounds down
+    assert int((N[0] - 1) * d / d) == N[0] - 1          # x does not
+    lin = {"Ez": LIN}
+    if kind == "oracle":
+        o = D.holder(*N, d, d, d, _yee_fields(N, d, lin))
+        got = o.get_field("Ez")
+    else:
+        got = FakeSim(N, d, lin).get_field("Ez")
+    assert got.shape == (N[2], N[1] - 1, N[0])
+
+
 def test_get_A_diag_returns_copies():
     rng = np.random.default_rng(0)
     mats = [rng.random((4, 5, 6)) + 1j * rng.random((4, 5, 6)) for _ in range(6)]
