@@ -97,6 +97,7 @@ _SIGS = {
     "emf_mem_info": ([c_int, POINTER(c_int64), POINTER(c_int64)], c_int),
     "emf_calc_ydAx": ([_P] + [_P] * 6 + [c_int] * 6 + [_P], c_int),
     "emf_grad_assemble": ([_P, c_int64, _P, _P, _P, _P, c_int, c_double, _P], c_int),
+    "emf_grad_assemble_noalloc": ([_P, c_int64, _P, _P, _P, _P, c_int, c_double, _P], c_int),
     "emf_pml_stretch": ([_P, c_int64, _P, _P, _P], c_int),
     "emf_batch_new": ([c_int, POINTER(_P)], c_int),
     "emf_batch_delete": ([_P], c_int),

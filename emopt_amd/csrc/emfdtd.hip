@@ -947,6 +947,55 @@ int build_samples(emf_solver *s)
 
 }  // namespace
 
+// ---- allocation-free gradient assembly ----------------------------------
+// emf_grad_assemble_noalloc stages its entries, their per-parameter segments
+// and the gradient in this file-scope device array instead of allocating:
+// every GPU the library runs on holds its own copy until the process ends.
+// In host code the name is a stand-in, not a device address, so the address
+// is looked up once per device (hipGetSymbolAddress, with that device
+// current) and only that pointer is used afterwards.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu[kMaxDevices];     // one call at a time per device
+unsigned char *g_noalloc_ptr[kMaxDevices];
+
+// Device address of `scratch` on the current device `dev` (caller holds
+// g_noalloc_mu[dev]).
+int noalloc_scratch(int dev, unsigned char **p)
+{
+    if (!g_noalloc_ptr[dev]) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) EMF_FAIL(EMF_ERR_HIP, "no device address for the scratch array");
+        g_noalloc_ptr[dev] = static_cast<unsigned char *>(ptr);
+    }
+    *p = g_noalloc_ptr[dev];
+    return EMF_OK;
+}
+
+// seg[p] = first entry of parameter p in the sorted pidx (seg[nparam] = n)
+__global__ void k_segments(const int *__restrict__ pidx, long long n, int nparam,
+                           long long *__restrict__ seg)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > nparam) return;
+    long long lo = 0, hi = n;
+    while (lo < hi) {
+        const long long mid = (lo + hi) / 2;
+        if (pidx[mid] < p) lo = mid + 1;
+        else hi = mid;
+    }
+    seg[p] = lo;
+}
+
+size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
+
+}  // namespace
+
 // =========================================================================
 // C ABI
 // =========================================================================
@@ -1845,6 +1894,61 @@ int emf_grad_assemble(emf_solver *s, int64_t n, const int32_t *pidx, const int32
     } while (0);
     dfree(dseg); dfree(dcell); dfree(dcomp); dfree(ddeps); dfree(dgrad);
     return rc;
+}
+
+int emf_grad_assemble_noalloc(emf_solver *s, int64_t n, const int32_t *pidx, const int32_t *comp,
+                              const int64_t *cell, const double *deps, int nparam, double step,
+                              double *grad)
+{
+    LOCK(s);
+    if (nparam <= 0 || !grad) EMF_FAIL(EMF_ERR_ARG, "nparam must be > 0");
+    if (n < 0 || (n > 0 && (!pidx || !comp || !cell || !deps))) EMF_FAIL(EMF_ERR_ARG, "null entries");
+    if (!(step != 0)) EMF_FAIL(EMF_ERR_ARG, "step must be non-zero");
+    if (!s->res[0][0] || !s->res[1][0]) EMF_FAIL(EMF_ERR_STATE, "forward and adjoint results needed");
+    if (s->dev < 0 || s->dev >= kMaxDevices) EMF_FAIL(EMF_ERR_ARG, "device %d out of range", s->dev);
+    for (int64_t e = 0; e < n; ++e) {
+        if (pidx[e] < 0 || pidx[e] >= nparam) EMF_FAIL(EMF_ERR_ARG, "parameter index out of range");
+        if (e > 0 && pidx[e] < pidx[e - 1]) EMF_FAIL(EMF_ERR_ARG, "entries must be grouped by parameter");
+        if (comp[e] < 0 || comp[e] > 5) EMF_FAIL(EMF_ERR_ARG, "component must be 0..5");
+        if (cell[e] < 0 || cell[e] >= s->n) EMF_FAIL(EMF_ERR_ARG, "cell index out of range");
+    }
+    // layout: segments, gradient, cells, d_material, parameter ids, components
+    const size_t o_seg = 0;
+    const size_t o_grad = align_up(o_seg + (size_t(nparam) + 1) * sizeof(long long));
+    const size_t o_cell = align_up(o_grad + size_t(nparam) * sizeof(double));
+    const size_t o_deps = align_up(o_cell + size_t(n) * sizeof(long long));
+    const size_t o_pidx = align_up(o_deps + size_t(n) * sizeof(double2));
+    const size_t o_comp = align_up(o_pidx + size_t(n) * sizeof(int));
+    const size_t need = o_comp + size_t(n) * sizeof(int);
+    if (need > kNoAllocScratchBytes)
+        EMF_FAIL(EMF_ERR_ARG, "%lld entries and %d parameters need %zu bytes of scratch; the "
+                 "allocation-free entry holds %zu (use emf_grad_assemble)", (long long)n, nparam,
+                 need, kNoAllocScratchBytes);
+    std::lock_guard<std::mutex> g(g_noalloc_mu[s->dev]);
+    unsigned char *base = nullptr;
+    CHECK(noalloc_scratch(s->dev, &base));
+    long long *dseg = reinterpret_cast<long long *>(base + o_seg);
+    double *dgrad = reinterpret_cast<double *>(base + o_grad);
+    long long *dcell = reinterpret_cast<long long *>(base + o_cell);
+    double2 *ddeps = reinterpret_cast<double2 *>(base + o_deps);
+    int *dpidx = reinterpret_cast<int *>(base + o_pidx);
+    int *dcomp = reinterpret_cast<int *>(base + o_comp);
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(dcell, cell, n * sizeof(long long), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(ddeps, deps, n * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(dpidx, pidx, n * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(dcomp, comp, n * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    }
+    // every segment and every gradient entry is written below: nothing left
+    // from an earlier call is read
+    hipLaunchKernelGGL(k_segments, dim3(grid1d(nparam + 1, 256)), dim3(256), 0, s->stream, dpidx,
+                       (long long)n, nparam, dseg);
+    hipLaunchKernelGGL(k_grad, dim3(nparam), dim3(RED_THREADS), 0, s->stream, ydax_args(s), dseg,
+                       dcomp, dcell, ddeps, step, dgrad);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(grad, dgrad, nparam * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EMF_OK;
 }
 
 int emf_pml_stretch(emf_solver *s, int64_t n, const int32_t *comp, const int64_t *cell,

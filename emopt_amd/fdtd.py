@@ -560,12 +560,14 @@ class FDTD(MaxwellSolver):
         self._lib.emf_pml_stretch(self._h, len(comp), ptr(comp), ptr(cell), ptr(out))
         return out[0::2] + 1j * out[1::2]
 
-    def grad_assemble(self, pidx, comp, cell, deps, nparam, step=1.0):
+    def grad_assemble(self, pidx, comp, cell, deps, nparam, step=1.0, noalloc=False):
         """grad[p] = -2 Re(sum E_adj * 1j * deps * E) / step over sparse
         (parameter, component, cell, dEps) entries, on the device
         (k_grad): the reference's per-parameter loop
         (emopt/adjoint_method.py:504-533) in one pass.  Components 3..5 are
-        mu_x..mu_z entries (term -1j * dmu * H_adj * H)."""
+        mu_x..mu_z entries (term -1j * dmu * H_adj * H).  noalloc=True uses
+        the entry point that allocates no device memory (up to ~1 M
+        entries)."""
         pidx = np.asarray(pidx, dtype=np.int32)
         order = np.argsort(pidx, kind="stable")
         pidx = np.ascontiguousarray(pidx[order])
@@ -573,8 +575,9 @@ class FDTD(MaxwellSolver):
         cell = np.ascontiguousarray(np.asarray(cell, dtype=np.int64)[order])
         deps = c128(np.asarray(deps, dtype=np.complex128)[order])
         grad = np.zeros(int(nparam))
-        self._lib.emf_grad_assemble(self._h, len(pidx), ptr(pidx), ptr(comp), ptr(cell),
-                                    ptr(deps), int(nparam), float(step), ptr(grad))
+        fn = self._lib.emf_grad_assemble_noalloc if noalloc else self._lib.emf_grad_assemble
+        fn(self._h, len(pidx), ptr(pidx), ptr(comp), ptr(cell), ptr(deps), int(nparam),
+           float(step), ptr(grad))
         return grad
 
     # ---------------------------------------------------------- step-level

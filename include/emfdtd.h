@@ -448,6 +448,19 @@ EMF_API int emf_grad_assemble(emf_solver *s, int64_t n, const int32_t *pidx,
                       const int32_t *comp, const int64_t *cell, const double *deps,
                       int nparam, double step, double *grad);
 
+/* emf_grad_assemble without allocating device memory: the same kernel and
+ * the same result (bit for bit), for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  The entries, segments and gradient are
+ * staged in a file-scope device array of 32 MiB per GPU, held until the
+ * process ends; calls needing more (32 B per entry plus 16 B per parameter,
+ * about 1 M entries) are rejected with EMF_ERR_ARG.  Calls on one GPU share that
+ * array: the library serialises them within the process (a per-device lock),
+ * and nothing a call reads is left over from an earlier one. */
+EMF_API int emf_grad_assemble_noalloc(emf_solver *s, int64_t n, const int32_t *pidx,
+                              const int32_t *comp, const int64_t *cell, const double *deps,
+                              int nparam, double step, double *grad);
+
 /* CPML stretch product s_x*s_y*s_z at the position of field component comp
  * (0..2 = Ex, Ey, Ez) of each cell, as complex (re, im) pairs in out[2n].
  * No counterpart in the reference.  In the frequency domain the CPML
