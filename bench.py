@@ -436,8 +436,10 @@ def _read_counters(path, counter):
 def profile_summary(args):
     lines = []
     cmd = args.command or "(command not given)"
+    title = args.workload or "400x200x120 step workload"
+    step_workload = args.workload is None
     if args.stats:
-        lines += ["# Kernel durations, 400x200x120 step workload", "",
+        lines += ["# Kernel durations, " + title, "",
                   "Command:", "", "    " + cmd, "",
                   "| kernel | calls | avg us | min us | max us | total % |",
                   "|---|---|---|---|---|---|"]
@@ -447,16 +449,17 @@ def profile_summary(args):
                     _short(row["Name"]), row["Calls"], float(row["AverageNs"]) / 1e3,
                     float(row["MinNs"]) / 1e3, float(row["MaxNs"]) / 1e3,
                     float(row["Percentage"])))
-        cells = 400 * 200 * 120
-        lines += ["", "Algorithmic bytes per launch: k_pass_h %d B/cell, k_pass_e %d B/cell "
-                  "x %d cells." % (B_ALG_H, B_ALG_E, cells)]
+        if step_workload:
+            cells = 400 * 200 * 120
+            lines += ["", "Algorithmic bytes per launch: k_pass_h %d B/cell, k_pass_e %d B/cell "
+                      "x %d cells." % (B_ALG_H, B_ALG_E, cells)]
     if args.fetch and args.write:
         fetch = _read_counters(args.fetch, "FETCH_SIZE")
         write = _read_counters(args.write, "WRITE_SIZE")
         ff = COPY_BYTES / (fetch["k_copy"] * 1024.0)
         fw = COPY_BYTES / (write["k_copy"] * 1024.0)
         cells = 400 * 200 * 120
-        lines += ["# HBM traffic per launch, 400x200x120 step workload", "",
+        lines += ["# HBM traffic per launch, " + title, "",
                   "Commands (one counter per run):", "", "    " + (args.fetch_command or "?"),
                   "    " + (args.write_command or "?"), "",
                   "Scale: k_copy moves %d B each way; FETCH_SIZE reads %.1f KiB and WRITE_SIZE "
@@ -467,7 +470,8 @@ def profile_summary(args):
         res = []
         for k in sorted(set(fetch) & set(write)):
             rd, wr = fetch[k] * 1024 * ff, write[k] * 1024 * fw
-            alg = KERNELS.get(k, 0) * cells if k in KERNELS else (2 * COPY_BYTES if k == "k_copy" else 0)
+            alg = (KERNELS.get(k, 0) * cells if (k in KERNELS and step_workload)
+                   else (2 * COPY_BYTES if k == "k_copy" else 0))
             lines.append("| %s | %.0f | %.0f | %.4g | %.4g | %.4g | %s |" % (
                 k, fetch[k], write[k], rd, wr, rd + wr, "%.4g" % alg if alg else "-"))
             res.append((k, rd + wr))
@@ -502,6 +506,7 @@ def main():
     ap.add_argument("--fetch")
     ap.add_argument("--write")
     ap.add_argument("--command")
+    ap.add_argument("--workload", help="title of the profiled workload (default: the step one)")
     ap.add_argument("--fetch-command")
     ap.add_argument("--write-command")
     args = ap.parse_args()
@@ -515,6 +520,10 @@ def main():
     if args.only:
         if args.only == "batched":
             out = bench_batched(args.batch)
+            from emopt_amd import _lib
+            gbs = np.zeros(1)     # the copy kernel that scales the traffic counters
+            _lib.load().emf_copy_bandwidth(0, COPY_BYTES, 3, gbs.ctypes.data_as(_lib.c_void_p))
+            out["copy_GBps"] = float(gbs[0])
         elif args.only == "serving":
             out = bench_serving([float(r) for r in args.serve_rates.split(",")],
                                 args.serve_requests, max_running=args.serve_running)

@@ -267,16 +267,17 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_phasor(const BTable *__restr
     }
 }
 
-// One step of every solve's loop (emopt/fdtd.py:1049-1194), one work-group
-// per table entry, lane 0 working (the convergence norms are summed in a
-// fixed order, as in k_conv_check).
+// One step of every solve's loop (emopt/fdtd.py:1049-1194), one wave per
+// table entry: every lane follows the member's state machine (the same
+// values), the convergence check is shared by the wave (conv_check), and
+// lane 0 stores the new state.
 __global__ void k_batch_tick(const BTable *__restrict__ T, const BMember *__restrict__ M,
                              BState *__restrict__ S)
 {
     int t_lo, t_hi, e_lo, e_hi;
     batch_group(T, t_lo, t_hi, e_lo, e_hi);
     const int e = e_lo + (int)blockIdx.x;
-    if (e >= e_hi || threadIdx.x != 0) return;
+    if (e >= e_hi || threadIdx.x >= 64) return;
     const int m = T->mem[e];
     const BMember &B = M[m];
     BState st = S[m];
@@ -294,13 +295,15 @@ __global__ void k_batch_tick(const BTable *__restrict__ T, const BMember *__rest
                     reinterpret_cast<const char *>(B.A.Eo) + B.A.cs);
                 const double *Ez = reinterpret_cast<const double *>(
                     reinterpret_cast<const char *>(B.A.Eo) + 2 * (size_t)B.A.cs);
-                conv_check(Ex, Ey, Ez, B.samples, B.nsamples, (double)(nd - 1) * B.dt,
-                           B.amp_rtol, B.phi_rtol, nd - 1, B.conv, B.hist, B.hist_cap);
-                const auto *cv = gptr(B.conv);
-                st.nchecks = cv->nchecks;
-                st.a_change = cv->a_change;
-                st.phi_change = cv->phi_change;
-                if (cv->done) { fin = true; st.status = 0; }
+                int nck;
+                double ac, pc;
+                const int conv = conv_check(Ex, Ey, Ez, B.samples, B.nsamples,
+                                            (double)(nd - 1) * B.dt, B.amp_rtol, B.phi_rtol,
+                                            nd - 1, B.conv, B.hist, B.hist_cap, &nck, &ac, &pc);
+                st.nchecks = nck;
+                st.a_change = ac;
+                st.phi_change = pc;
+                if (conv) { fin = true; st.status = 0; }
             }
             if (!fin && nd >= B.n_stop) { fin = true; st.status = 1; }
         }
@@ -319,7 +322,7 @@ __global__ void k_batch_tick(const BTable *__restrict__ T, const BMember *__rest
         }
     }
     st.mode = mode;
-    S[m] = st;
+    if (threadIdx.x == 0) S[m] = st;
 }
 
 // After the replay's phasor pass: the solves that took it are finished.

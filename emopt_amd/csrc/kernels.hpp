@@ -241,27 +241,36 @@ __device__ inline double amp3(double t0, double t1, double t2, double f0, double
     return f10 / (cos(ph) * (sin(t1) - sin(t0)) + sin(ph) * (cos(t1) - cos(t0)));
 }
 
+// One check of emopt/fdtd.py:1072-1144 by one 64-lane wave (every lane of it
+// must call): lane q handles sample point q (nc <= 64), the four norms are
+// summed across the wave by xor butterflies (each lane ends with the same
+// sums, in a fixed order), and lane 0 writes the shared state.  Returns the
+// convergence flag; *nchecks, *a_change, *phi_change receive the check's
+// count and changes in every lane.
 __device__ inline int conv_check(const double *__restrict__ Ex, const double *__restrict__ Ey,
                                  const double *__restrict__ Ez,
                                  const long long *__restrict__ sample, int nc, double tnow,
                                  double amp_rtol, double phi_rtol, long long nstep, ConvState *st,
-                                 double *hist, int hist_cap)
+                                 double *hist, int hist_cap, int *nchecks, double *a_change,
+                                 double *phi_change)
 {
     // all through the global address space (the pointers may come from memory)
     typedef __attribute__((address_space(1))) ConvState gconv;
     gconv *const S = (gconv *)st;
-    S->t0 = S->t1; S->t1 = S->t2; S->t2 = tnow;
-    const double t0 = S->t0, t1 = S->t1, t2 = S->t2;
-    const double *E[3] = { Ex, Ey, Ez };
+    const int q = (int)(threadIdx.x & 63);
+    const double t0 = S->t1, t1 = S->t2, t2 = tnow;
+    const int h = S->nchecks;
     double sa = 0, s0 = 0, sp = 0, sp0 = 0;
-    for (int q = 0; q < nc; ++q) {
+    if (q < nc) {
+        const double *E[3] = { Ex, Ey, Ez };
+        const long long at = *gptr(sample + q);
         double ph[3], am[3];
         for (int c = 0; c < 3; ++c) {
-            S->eh[0][c][q] = S->eh[1][c][q];
-            S->eh[1][c][q] = S->eh[2][c][q];
-            const long long at = *gptr(sample + q);
-            S->eh[2][c][q] = at >= 0 ? *gptr(E[c] + at) : 0.0;
-            const double f0 = S->eh[0][c][q], f1 = S->eh[1][c][q], f2 = S->eh[2][c][q];
+            const double f0 = S->eh[1][c][q], f1 = S->eh[2][c][q];
+            const double f2 = at >= 0 ? *gptr(E[c] + at) : 0.0;
+            S->eh[0][c][q] = f0;
+            S->eh[1][c][q] = f1;
+            S->eh[2][c][q] = f2;
             ph[c] = phase3(t0, t1, t2, f0, f1, f2);
             am[c] = amp3(t0, t1, t2, f0, f1, f2, ph[c]);
         }
@@ -271,32 +280,48 @@ __device__ inline int conv_check(const double *__restrict__ Ex, const double *__
         const double phi1 = ph[0] + ph[1] + ph[2];
         const double a1 = am[0] + am[1] + am[2];
         S->phi1[q] = phi1; S->a1[q] = a1;
-        sa += (a1 - a0) * (a1 - a0); s0 += a0 * a0;
-        sp += (phi1 - phi0) * (phi1 - phi0); sp0 += phi0 * phi0;
+        sa = (a1 - a0) * (a1 - a0); s0 = a0 * a0;
+        sp = (phi1 - phi0) * (phi1 - phi0); sp0 = phi0 * phi0;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sa += __shfl_xor(sa, o, 64);
+        s0 += __shfl_xor(s0, o, 64);
+        sp += __shfl_xor(sp, o, 64);
+        sp0 += __shfl_xor(sp0, o, 64);
     }
     const double ac = sqrt(sa) / sqrt(s0), pc = sqrt(sp) / sqrt(sp0);
-    S->a_change = ac; S->phi_change = pc;
-    const int h = S->nchecks++;
-    if (hist && h < hist_cap) {
-        *gptr(hist + 3 * h) = (double)nstep; *gptr(hist + 3 * h + 1) = ac;
-        *gptr(hist + 3 * h + 2) = pc;
-    }
     const int conv = (ac <= amp_rtol) && (pc <= phi_rtol);
-    S->done = conv;
+    if (q == 0) {
+        S->t0 = t0; S->t1 = t1; S->t2 = t2;
+        S->a_change = ac; S->phi_change = pc;
+        S->nchecks = h + 1;
+        if (hist && h < hist_cap) {
+            *gptr(hist + 3 * h) = (double)nstep; *gptr(hist + 3 * h + 1) = ac;
+            *gptr(hist + 3 * h + 2) = pc;
+        }
+        S->done = conv;
+    }
+    *nchecks = h + 1;
+    *a_change = ac;
+    *phi_change = pc;
     return conv;
 }
 
+// one wave (64 threads): the check of the single-solve loop
 __global__ void k_conv_check(const double *__restrict__ Ex, const double *__restrict__ Ey,
                              const double *__restrict__ Ez, const long long *__restrict__ sample,
                              int nc, double tnow, double amp_rtol, double phi_rtol,
                              long long nstep, ConvState *st, double *hist, int hist_cap,
                              int *done_out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (blockIdx.x != 0 || threadIdx.x >= 64) return;
     if (st->done) return;
+    int nck;
+    double ac, pc;
     const int conv = conv_check(Ex, Ey, Ez, sample, nc, tnow, amp_rtol, phi_rtol, nstep, st, hist,
-                                hist_cap);
-    if (done_out) *done_out = conv;
+                                hist_cap, &nck, &ac, &pc);
+    if (done_out && threadIdx.x == 0) *done_out = conv;
 }
 
 // -------------------------------------------------------- capture / phasor
