@@ -215,7 +215,24 @@ def _plain_solve(sim, adjoint):
             sim._solve(0)
 
 
-def bench_batched(n, seed=3):
+def copy_curve():
+    """Copy rate (GB/s, read + write) against the working set (both buffers),
+    from 16 MB to 4 GB: working sets inside the 256 MB Infinity Cache are
+    served from it when the copy is repeated."""
+    from emopt_amd import _lib
+    L = _lib.load()
+    out = []
+    for mb in (8, 16, 32, 48, 64, 96, 128, 192, 256, 512, 1024, 2048):
+        b = mb << 20
+        reps = int(min(2000, max(10, (40 << 30) // b)))
+        g = np.zeros(1)
+        L.emf_copy_bandwidth(0, b, reps, g.ctypes.data_as(_lib.c_void_p))
+        out.append({"working_set_MB": 2 * mb, "GBps": float(g[0]), "reps": reps})
+        log("[bench] copy, working set %5d MB: %.0f GB/s" % (2 * mb, g[0]))
+    return out
+
+
+def bench_batched(n, seed=3, direct=False, plain=True):
     """The batched workload: n solves, dims uniform in [64,256]x[64,160]x
     [48,96] at d = 0.04 um, lambda 1.50-1.60 um, 8-12 cell CPML, half with a
     forward Jz dipole, half with an adjoint plane source.  All submitted at
@@ -231,22 +248,39 @@ def bench_batched(n, seed=3):
     sims = [build_sim(c) for c in cases]
     _plain_solve(build_sim(cases[0]), cases[0]["adjoint"])        # warm-up
     plain_t, plain_info = [], []
-    t0 = time.perf_counter()
-    for sim, c in zip(sims, cases):
-        _plain_solve(sim, c["adjoint"])
-        plain_t.append(time.perf_counter() - t0)
-        plain_info.append(dict(sim.last_solve))
-    tp = time.perf_counter() - t0
     sample = list(range(0, n, max(1, n // 8)))
-    ref = {q: all_fields(sims[q], cases[q], cases[q]["adjoint"]) for q in sample}
-    log("[bench] batched workload, single-solve path: %.2fs" % tp)
-    b = Batch()
+    if plain:
+        t0 = time.perf_counter()
+        for sim, c in zip(sims, cases):
+            _plain_solve(sim, c["adjoint"])
+            plain_t.append(time.perf_counter() - t0)
+            plain_info.append(dict(sim.last_solve))
+        tp = time.perf_counter() - t0
+        ref = {q: all_fields(sims[q], cases[q], cases[q]["adjoint"]) for q in sample}
+        log("[bench] batched workload, single-solve path: %.2fs" % tp)
+    b = Batch(use_graph=not direct)
     t0 = time.perf_counter()
     for sim, c in zip(sims, cases):
         b.add(sim, adjoint=c["adjoint"], postprocess=False)
-    b.run()
+    if direct:
+        # under a tracer: report progress every 50 replays
+        while b.running:
+            b.run(until="graphs", max_graphs=50)
+            log("[bench] batched workload: %d solves running after %.1fs"
+                % (b.running, time.perf_counter() - t0))
+    else:
+        b.run()
     tb = time.perf_counter() - t0
-    log("[bench] batched workload, one batch: %.2fs" % tb)
+    log("[bench] batched workload, one batch%s: %.2fs" % (" (direct launches)" if direct else "", tb))
+    if not plain:
+        info = [dict(sims[q].last_solve) for q in range(n)]
+        steps = sum(i["steps"] * c for i, c in zip(info, cells))
+        b.close()
+        return {"workload": "batch_%d_dims64-256x64-160x48-96_dx0.04_wl1.50-1.60_cpml8-12" % n,
+                "solves": n, "cells_total": int(sum(cells)), "cell_updates_total": int(steps),
+                "direct_launches": bool(direct),
+                "batched": dict(seconds=tb, solves_per_hr=n * 3600 / tb,
+                                cell_updates_per_s=steps / tb)}
     n0_same = all(sims[q].last_solve["n0"] == plain_info[q]["n0"] for q in range(n))
     err = max(max(rel_l2(x, y) for x, y in zip(all_fields(sims[q], cases[q], cases[q]["adjoint"]),
                                                ref[q])) for q in sample)
@@ -259,6 +293,7 @@ def bench_batched(n, seed=3):
                           **_pcts(plain_t)),
             "batched": dict(seconds=tb, solves_per_hr=n * 3600 / tb, cell_updates_per_s=steps / tb,
                             **_pcts(lat_b)),
+            "direct_launches": bool(direct),
             "speedup_solves_per_hr": tp / tb, "n0_identical": bool(n0_same),
             "max_rel_l2_vs_plain_sample": err, "sample": sample}
 
@@ -419,9 +454,9 @@ def _short(name):
     return name.split("(")[0][:48]
 
 
-def _read_counters(path, counter):
-    """Average counter value per dispatch, by kernel, from a rocprofv3
-    counter_collection.csv."""
+def _read_counters(path, counter, totals=False):
+    """Average counter value per dispatch (totals: (sum, dispatches)), by
+    kernel, from a rocprofv3 counter_collection.csv."""
     acc = {}
     with open(path) as f:
         for row in csv.DictReader(f):
@@ -430,6 +465,8 @@ def _read_counters(path, counter):
             k = _short(row["Kernel_Name"])
             s, n = acc.get(k, (0.0, 0))
             acc[k] = (s + float(row["Counter_Value"]), n + 1)
+    if totals:
+        return acc
     return {k: s / n for k, (s, n) in acc.items()}
 
 
@@ -453,7 +490,37 @@ def profile_summary(args):
             cells = 400 * 200 * 120
             lines += ["", "Algorithmic bytes per launch: k_pass_h %d B/cell, k_pass_e %d B/cell "
                       "x %d cells." % (B_ALG_H, B_ALG_E, cells)]
-    if args.fetch and args.write:
+        if args.cell_updates:
+            lines += ["", "Time per cell-update (all launches of the kernel / %.6g cell-updates):"
+                      % args.cell_updates, ""]
+            with open(args.stats) as f:
+                for row in csv.DictReader(f):
+                    lines.append("- %s: %.2f ps" % (_short(row["Name"]),
+                                                   float(row["TotalDurationNs"]) * 1e3
+                                                   / args.cell_updates))
+    if args.fetch and args.write and args.cell_updates:
+        # a workload whose launches differ in size: total bytes per kernel
+        fetch = _read_counters(args.fetch, "FETCH_SIZE", totals=True)
+        write = _read_counters(args.write, "WRITE_SIZE", totals=True)
+        ff = COPY_BYTES / (fetch["k_copy"][0] / fetch["k_copy"][1] * 1024.0)
+        fw = COPY_BYTES / (write["k_copy"][0] / write["k_copy"][1] * 1024.0)
+        lines += ["# HBM traffic, " + title, "",
+                  "Commands (one counter per run):", "", "    " + (args.fetch_command or "?"),
+                  "    " + (args.write_command or "?"), "",
+                  "Scale: k_copy moves %d B each way; FETCH_SIZE reads %.1f KiB and WRITE_SIZE "
+                  "%.1f KiB for it, so the factors are %.3f (fetch) and %.3f (write)."
+                  % (COPY_BYTES, fetch["k_copy"][0] / fetch["k_copy"][1],
+                     write["k_copy"][0] / write["k_copy"][1], ff, fw), "",
+                  "Totals over all launches; per cell-update = total / %.6g cell-updates." %
+                  args.cell_updates, "",
+                  "| kernel | launches | scaled read GB | scaled write GB | B per cell-update |",
+                  "|---|---|---|---|---|"]
+        for k in sorted(set(fetch) & set(write)):
+            rd, wr = fetch[k][0] * 1024 * ff, write[k][0] * 1024 * fw
+            lines.append("| %s | %d | %.4g | %.4g | %.2f |" % (
+                k, fetch[k][1], rd / 1e9, wr / 1e9,
+                0.0 if k == "k_copy" else (rd + wr) / args.cell_updates))
+    elif args.fetch and args.write:
         fetch = _read_counters(args.fetch, "FETCH_SIZE")
         write = _read_counters(args.write, "WRITE_SIZE")
         ff = COPY_BYTES / (fetch["k_copy"] * 1024.0)
@@ -494,6 +561,12 @@ def main():
                     help="compare the workload's phasors with the oracle's at n0 = this many "
                          "Tn (0: skip); 28 Tn is past the source ramp")
     ap.add_argument("--batch", type=int, default=32, help="solves in the batched section (0: skip)")
+    ap.add_argument("--batch-direct", action="store_true",
+                    help="batched section: launch each replay's kernels directly (tracing)")
+    ap.add_argument("--batch-no-plain", action="store_true",
+                    help="batched section: skip the single-solve leg")
+    ap.add_argument("--copy-curve", action="store_true",
+                    help="with --only batched: copy rate against working-set size")
     ap.add_argument("--serve-requests", type=int, default=48,
                     help="requests per arrival rate in the serving section (0: skip)")
     ap.add_argument("--serve-rates", default="2,8,32", help="Poisson arrival rates, per second")
@@ -507,6 +580,9 @@ def main():
     ap.add_argument("--write")
     ap.add_argument("--command")
     ap.add_argument("--workload", help="title of the profiled workload (default: the step one)")
+    ap.add_argument("--cell-updates", type=float,
+                    help="profile summary of a workload whose launches differ in size: report "
+                         "totals per cell-update")
     ap.add_argument("--fetch-command")
     ap.add_argument("--write-command")
     args = ap.parse_args()
@@ -519,7 +595,10 @@ def main():
         sys.exit(2)
     if args.only:
         if args.only == "batched":
-            out = bench_batched(args.batch)
+            out = bench_batched(args.batch, direct=args.batch_direct,
+                                plain=not args.batch_no_plain)
+            if args.copy_curve:
+                out["copy_curve"] = copy_curve()
             from emopt_amd import _lib
             gbs = np.zeros(1)     # the copy kernel that scales the traffic counters
             _lib.load().emf_copy_bandwidth(0, COPY_BYTES, 3, gbs.ctypes.data_as(_lib.c_void_p))

@@ -103,6 +103,8 @@ _SIGS = {
     "emf_batch_delete": ([_P], c_int),
     "emf_batch_set_tuning": ([_P, c_int, c_int, c_int], c_int),
     "emf_batch_set_grouping": ([_P, c_int64, c_int], c_int),
+    "emf_batch_set_graph": ([_P, c_int], c_int),
+    "emf_batch_set_lanes": ([_P, c_int], c_int),
     "emf_batch_add": ([_P, _P, c_int, c_double, c_double, c_int64, c_int64, POINTER(c_int)], c_int),
     "emf_batch_run": ([_P, c_int, c_int, POINTER(c_int)], c_int),
     "emf_batch_poll": ([_P, c_int, POINTER(SolveInfo)], c_int),

@@ -22,18 +22,20 @@ import ctypes
 from . import _lib
 from ._lib import SolveInfo
 
-__all__ = ["Batch", "GROUP_BYTES"]
+__all__ = ["Batch", "GROUP_BYTES", "LANES"]
 
 # Members are stepped in groups whose hot state (~80 B/cell) stays within
-# this many bytes -- about MI355X's 256 MB Infinity Cache -- so each group's
-# E pass re-reads from cache what its H pass just wrote (batch of 32 mixed
-# solves: 0.98x the one-after-the-other time, against 0.91x with one group).
-GROUP_BYTES = 260 * 10**6
+# GROUP_BYTES, and LANES groups are stepped at once (one stream each), so the
+# groups in flight together -- about MI355X's 256 MB Infinity Cache -- let each
+# group's E pass re-read from cache what its H pass just wrote, while one
+# lane's kernels fill the CUs that the other's leave idle at their tails.
+GROUP_BYTES = 130 * 10**6
+LANES = 2
 
 
 class Batch:
     def __init__(self, device=0, steps_per_graph=64, zc=2, blocks_per_cu=16,
-                 group_bytes=GROUP_BYTES, rotate_groups=False):
+                 group_bytes=GROUP_BYTES, rotate_groups=False, use_graph=True, lanes=LANES):
         self._lib = _lib.load()
         self.device = device
         h = ctypes.c_void_p()
@@ -43,6 +45,19 @@ class Batch:
         self.finished = {}          # id -> solve info dict of finished members
         self.set_tuning(steps_per_graph, zc, blocks_per_cu)
         self.set_grouping(group_bytes, rotate_groups)
+        self.set_graph(use_graph)
+        self.set_lanes(lanes)
+
+    def set_lanes(self, lanes):
+        """Step this many groups at once (1..4), each on its own stream."""
+        self._lib.emf_batch_set_lanes(self._h, int(lanes))
+        self.lanes = int(lanes)
+
+    def set_graph(self, use_graph):
+        """True (default): one graph launch per replay.  False: the replay's
+        kernels are launched one by one (for per-kernel tracing)."""
+        self._lib.emf_batch_set_graph(self._h, 1 if use_graph else 0)
+        self.use_graph = bool(use_graph)
 
     def set_grouping(self, group_bytes, rotate=False):
         """Step members in groups whose hot state fits `group_bytes`, one

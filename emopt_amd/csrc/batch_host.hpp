@@ -1,13 +1,19 @@
 // Host side of batched solves (emf_batch_* entry points).  Included at the end
 // of emfdtd.hip: it uses the solver internals defined there.
 //
-// A batch owns a stream, three small device arrays (work table, member
-// descriptors, member states) and one hipGraph of `spg` steps, captured on
-// first use and replayed until every member has finished.  The graph's
-// kernels read everything that varies (which solves, which step, what the
-// next step does) from those device arrays, so admitting or retiring a solve
-// only rewrites them between replays -- the graph is never re-captured and
-// records no memset or copy nodes.
+// A batch owns small device arrays (work table, member descriptors, member
+// states) and one hipGraph of `spg` steps per lane, captured on first use and
+// replayed until every member has finished.  The graph's kernels read
+// everything that varies (which solves, which step, what the next step does)
+// from those device arrays, so admitting or retiring a solve only rewrites
+// them between replays -- the graphs are never re-captured and record no
+// memset or copy nodes.
+//
+// Lanes: the members are split into cache-sized groups, and `lanes` groups
+// are stepped at once, each by its own stream, copy of the table and graph.
+// Lanes never step the same group, so they share no data; the kernels of one
+// lane fill the CUs that another lane's launch leaves idle at its tail, and
+// one lane's small per-step kernels (the tick) overlap the other's passes.
 #pragma once
 
 namespace {
@@ -24,31 +30,46 @@ struct BatchSlot {
 
 }  // namespace
 
+constexpr int BATCH_LANES_MAX = 4;
+
 struct emf_batch {
     std::mutex mtx;
     int dev = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // lane 0; admissions and table uploads
+    hipStream_t lstream[BATCH_LANES_MAX] = {};   // lstream[0] == stream
+    int lanes = 2;                      // groups stepped at once
+    int lnext[BATCH_LANES_MAX] = {};    // next group of each lane (rotate)
     int spg = 64;                       // steps per graph replay
     int zc = 2;                         // z planes per work-group
     int bpc = 16;                       // persistent work-groups per CU
+    int use_graph = 1;                  // 0: launch each replay's kernels directly
     long long group_bytes = 0;          // hot-state bytes per group (0: one group)
-    int gnext = 0;                      // group the next replay steps
     int rotate = 0;                     // 1: groups take turns; 0: first group until done
     long long next_seq = 0;             // admission counter
     int ncu = 0;
-    BTable *T = nullptr;
+    BTable *T = nullptr;                // one copy of the table per lane
     BMember *M = nullptr;
     BState *S = nullptr;
     BTable hT{};
     BMember hM[BATCH_MAX]{};
-    BState *hS = nullptr;               // pinned ring of 3 state snapshots
+    BState *hS = nullptr;               // pinned ring of state snapshots
     int ring = 0;
     BatchSlot sl[BATCH_MAX];
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph[BATCH_LANES_MAX] = {};
+    hipGraphExec_t exec[BATCH_LANES_MAX] = {};
 };
 
+constexpr int BATCH_RING = 3 * BATCH_LANES_MAX;   // > replays in flight (2 per lane)
+
 namespace {
+
+// wait for every lane's stream
+int batch_sync(emf_batch *b)
+{
+    for (int l = 0; l < BATCH_LANES_MAX; ++l)
+        if (b->lstream[l]) HIPCHK(hipStreamSynchronize(b->lstream[l]));
+    return EMF_OK;
+}
 
 int batch_upload_table(emf_batch *b)
 {
@@ -87,39 +108,69 @@ int batch_upload_table(emf_batch *b)
     if (t.ngroups == 0) t.gent[t.ngroups++] = 0;
     t.gent[t.ngroups] = t.nact;
     t.cur = 0;
-    b->gnext = 0;
-    HIPCHK(hipMemcpyAsync(b->T, &t, sizeof(BTable), hipMemcpyHostToDevice, b->stream));
+    for (int l = 0; l < BATCH_LANES_MAX; ++l) {
+        b->lnext[l] = l;
+        HIPCHK(hipMemcpyAsync(b->T + l, &t, sizeof(BTable), hipMemcpyHostToDevice, b->stream));
+    }
     HIPCHK(hipStreamSynchronize(b->stream));   // t is rewritten by the next call
     return EMF_OK;
 }
 
+// the group lane l steps in its next replay (-1: none).  Lane l steps the
+// groups g with g % lanes == l: always the same one (the first `lanes`
+// groups until their members finish), or those in turn with rotate.
+int batch_lane_group(emf_batch *b, int l)
+{
+    const int ng = b->hT.ngroups;
+    if (l >= ng) return -1;
+    if (!b->rotate) return l;
+    int g = b->lnext[l];
+    if (g >= ng) g = l;
+    b->lnext[l] = g + b->lanes < ng ? g + b->lanes : l;
+    return g;
+}
+
+// the kernels of one replay of lane l: spg steps (H, E, tick), then the
+// phasor pass and the finish marks, on the lane's stream and table
+void batch_enqueue(emf_batch *b, int l)
+{
+    const unsigned blocks = (unsigned)(b->ncu * b->bpc);
+    hipStream_t st = b->lstream[l];
+    BTable *T = b->T + l;
+    for (int q = 0; q < b->spg; ++q) {
+        hipLaunchKernelGGL(k_batch_h, dim3(blocks), dim3(PX, BTY), 0, st, T, b->M, b->S);
+        hipLaunchKernelGGL(k_batch_e, dim3(blocks), dim3(PX, BTY), 0, st, T, b->M, b->S);
+        hipLaunchKernelGGL(k_batch_tick, dim3(BATCH_MAX), dim3(64), 0, st, T, b->M, b->S);
+    }
+    hipLaunchKernelGGL(k_batch_phasor, dim3(blocks), dim3(PX, BTY), 0, st, T, b->M, b->S);
+    hipLaunchKernelGGL(k_batch_finish, dim3(BATCH_MAX / 64), dim3(64), 0, st, T, b->S);
+}
+
 int batch_graph(emf_batch *b)
 {
-    if (b->exec) return EMF_OK;
-    const unsigned blocks = (unsigned)(b->ncu * b->bpc);
-    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (int q = 0; q < b->spg; ++q) {
-        hipLaunchKernelGGL(k_batch_h, dim3(blocks), dim3(PX, BTY), 0, b->stream, b->T, b->M, b->S);
-        hipLaunchKernelGGL(k_batch_e, dim3(blocks), dim3(PX, BTY), 0, b->stream, b->T, b->M, b->S);
-        hipLaunchKernelGGL(k_batch_tick, dim3(BATCH_MAX), dim3(64), 0, b->stream, b->T, b->M,
-                           b->S);
+    if (!b->use_graph) return EMF_OK;
+    for (int l = 0; l < b->lanes; ++l) {
+        if (b->exec[l]) continue;
+        HIPCHK(hipStreamBeginCapture(b->lstream[l], hipStreamCaptureModeThreadLocal));
+        batch_enqueue(b, l);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(b->lstream[l], &g);
+        if (ec != hipSuccess || !g)
+            EMF_FAIL(EMF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+        b->graph[l] = g;
+        HIPCHK(hipGraphInstantiate(&b->exec[l], b->graph[l], nullptr, nullptr, 0));
     }
-    hipLaunchKernelGGL(k_batch_phasor, dim3(blocks), dim3(PX, BTY), 0, b->stream, b->T, b->M, b->S);
-    hipLaunchKernelGGL(k_batch_finish, dim3(BATCH_MAX / 64), dim3(64), 0, b->stream, b->T, b->S);
-    hipGraph_t g = nullptr;
-    const hipError_t ec = hipStreamEndCapture(b->stream, &g);
-    if (ec != hipSuccess || !g) EMF_FAIL(EMF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
-    b->graph = g;
-    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     return EMF_OK;
 }
 
 void batch_drop_graph(emf_batch *b)
 {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
+    for (int l = 0; l < BATCH_LANES_MAX; ++l) {
+        if (b->exec[l]) (void)hipGraphExecDestroy(b->exec[l]);
+        if (b->graph[l]) (void)hipGraphDestroy(b->graph[l]);
+        b->exec[l] = nullptr;
+        b->graph[l] = nullptr;
+    }
 }
 
 // a member whose state shows it finished: fill its solve info and hand the
@@ -154,22 +205,28 @@ int emf_batch_new(int device, emf_batch **out)
     emf_batch *b = new emf_batch();
     b->dev = device;
     int rc = EMF_OK;
-    if (hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
-        hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         rc = EMF_ERR_HIP;
-    if (rc == EMF_OK) rc = dalloc(&b->T, 1);
+    // every lane's stream at the default priority
+    for (int l = 0; l < BATCH_LANES_MAX && rc == EMF_OK; ++l)
+        if (hipStreamCreateWithFlags(&b->lstream[l], hipStreamNonBlocking) != hipSuccess)
+            rc = EMF_ERR_HIP;
+    b->stream = b->lstream[0];
+    if (rc == EMF_OK) rc = dalloc(&b->T, BATCH_LANES_MAX);
     if (rc == EMF_OK) rc = dalloc(&b->M, BATCH_MAX);
     if (rc == EMF_OK) rc = dalloc(&b->S, BATCH_MAX);
-    if (rc == EMF_OK && hipHostMalloc((void **)&b->hS, 3 * BATCH_MAX * sizeof(BState)) != hipSuccess)
+    if (rc == EMF_OK &&
+        hipHostMalloc((void **)&b->hS, BATCH_RING * BATCH_MAX * sizeof(BState)) != hipSuccess)
         rc = EMF_ERR_NOMEM;
     if (rc == EMF_OK) {
-        memset(b->hS, 0, 3 * BATCH_MAX * sizeof(BState));
+        memset(b->hS, 0, BATCH_RING * BATCH_MAX * sizeof(BState));
         rc = batch_upload_table(b);
     }
     if (rc != EMF_OK) {
         dfree(b->T); dfree(b->M); dfree(b->S);
         if (b->hS) (void)hipHostFree(b->hS);
-        if (b->stream) (void)hipStreamDestroy(b->stream);
+        for (auto &st : b->lstream)
+            if (st) (void)hipStreamDestroy(st);
         delete b;
         EMF_FAIL(rc, "batch creation failed");
     }
@@ -183,7 +240,7 @@ int emf_batch_delete(emf_batch *b)
     {
         std::lock_guard<std::mutex> lk(b->mtx);
         (void)hipSetDevice(b->dev);
-        (void)hipStreamSynchronize(b->stream);
+        (void)batch_sync(b);
         batch_drop_graph(b);
         for (auto &q : b->sl) {
             if (q.s) {
@@ -194,7 +251,8 @@ int emf_batch_delete(emf_batch *b)
         }
         dfree(b->T); dfree(b->M); dfree(b->S);
         if (b->hS) (void)hipHostFree(b->hS);
-        (void)hipStreamDestroy(b->stream);
+        for (auto &st : b->lstream)
+            if (st) (void)hipStreamDestroy(st);
     }
     delete b;
     return EMF_OK;
@@ -214,11 +272,31 @@ int emf_batch_set_tuning(emf_batch *b, int steps_per_graph, int zc, int blocks_p
     if (blocks_per_cu < 1 || blocks_per_cu > 64) EMF_FAIL(EMF_ERR_ARG, "blocks_per_cu must be 1..64");
     for (auto &q : b->sl)
         if (q.s && !q.finished) EMF_FAIL(EMF_ERR_STATE, "set the tuning while no solve is running");
-    HIPCHK(hipStreamSynchronize(b->stream));
+    CHECK(batch_sync(b));
     if (steps_per_graph != b->spg || blocks_per_cu != b->bpc) batch_drop_graph(b);
     b->spg = steps_per_graph;
     b->zc = zc;
     b->bpc = blocks_per_cu;
+    return EMF_OK;
+}
+
+int emf_batch_set_graph(emf_batch *b, int use_graph)
+{
+    BLOCK(b);
+    if (use_graph != 0 && use_graph != 1) EMF_FAIL(EMF_ERR_ARG, "use_graph must be 0 or 1");
+    CHECK(batch_sync(b));
+    b->use_graph = use_graph;
+    return EMF_OK;
+}
+
+int emf_batch_set_lanes(emf_batch *b, int lanes)
+{
+    BLOCK(b);
+    if (lanes < 1 || lanes > BATCH_LANES_MAX)
+        EMF_FAIL(EMF_ERR_ARG, "lanes must be 1..%d", BATCH_LANES_MAX);
+    CHECK(batch_sync(b));
+    b->lanes = lanes;
+    for (int l = 0; l < BATCH_LANES_MAX; ++l) b->lnext[l] = l;
     return EMF_OK;
 }
 
@@ -227,7 +305,7 @@ int emf_batch_set_grouping(emf_batch *b, int64_t group_bytes, int rotate)
     BLOCK(b);
     if (group_bytes < 0) EMF_FAIL(EMF_ERR_ARG, "group_bytes must be >= 0");
     if (rotate != 0 && rotate != 1) EMF_FAIL(EMF_ERR_ARG, "rotate must be 0 or 1");
-    HIPCHK(hipStreamSynchronize(b->stream));
+    CHECK(batch_sync(b));
     b->group_bytes = group_bytes;
     b->rotate = rotate;
     return batch_upload_table(b);
@@ -328,7 +406,7 @@ int emf_batch_add(emf_batch *b, emf_solver *s, int slot, double rtol, double nma
         st.status = fixed_n0 >= 0 ? 2 : 1;
     }
     // the batch's kernels may still be reading M/S/T: wait for them
-    HIPCHK(hipStreamSynchronize(b->stream));
+    CHECK(batch_sync(b));
     HIPCHK(hipMemcpyAsync(&b->M[id], &B, sizeof(BMember), hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipMemcpyAsync(&b->S[id], &st, sizeof(BState), hipMemcpyHostToDevice, b->stream));
     q.s = s;
@@ -361,16 +439,23 @@ int emf_batch_run(emf_batch *b, int max_graphs, int until, int *finished)
     };
     if (running() > 0) {
         CHECK(batch_graph(b));
-        struct Pend { int r; hipEvent_t ev; };
+        // a replay in flight: its lane, snapshot slot, group and end event
+        struct Pend { int lane, r, g; hipEvent_t ev; };
         std::vector<Pend> pend;
+        int inflight[BATCH_LANES_MAX] = {};
         int launched = 0;
         bool dirty = false, stop = false;
         int rc = EMF_OK;
         auto retire = [&](const Pend &p) {
             if (hipEventSynchronize(p.ev) != hipSuccess) rc = EMF_ERR_HIP;
             (void)hipEventDestroy(p.ev);
+            --inflight[p.lane];
+            // only the replay's own group: the snapshot was taken while other
+            // lanes were stepping theirs.  The table is the one the replay
+            // ran on (it is only rewritten when nothing is in flight).
             const BState *snap = b->hS + (size_t)p.r * BATCH_MAX;
-            for (int m = 0; m < BATCH_MAX; ++m) {
+            for (int e = b->hT.gent[p.g]; e < b->hT.gent[p.g + 1]; ++e) {
+                const int m = b->hT.mem[e];
                 BatchSlot &q = b->sl[m];
                 if (q.s && !q.finished && snap[m].phase == BP_DONE) {
                     batch_finish(b, m, snap[m]);
@@ -391,24 +476,31 @@ int emf_batch_run(emf_batch *b, int max_graphs, int until, int *finished)
                 if (running() == 0) stop = true;
                 continue;
             }
-            while (!stop && rc == EMF_OK && (int)pend.size() < 2 &&
-                   (until == 2 || launched < max_graphs)) {
-                hipEvent_t ev;
-                hipLaunchKernelGGL(k_batch_set_group, dim3(1), dim3(64), 0, b->stream, b->T,
-                                   b->gnext);
-                b->gnext = b->rotate ? (b->gnext + 1) % std::max(1, b->hT.ngroups) : 0;
-                if (hipGetLastError() != hipSuccess || hipGraphLaunch(b->exec, b->stream) != hipSuccess ||
-                    hipMemcpyAsync(b->hS + (size_t)b->ring * BATCH_MAX, b->S,
-                                   BATCH_MAX * sizeof(BState), hipMemcpyDeviceToHost,
-                                   b->stream) != hipSuccess ||
-                    hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
-                    hipEventRecord(ev, b->stream) != hipSuccess) {
-                    rc = EMF_ERR_HIP;
-                    break;
+            // up to two replays in flight per lane, each lane on its group
+            for (int l = 0; l < b->lanes && !stop && rc == EMF_OK; ++l) {
+                while (rc == EMF_OK && inflight[l] < 2 && (until == 2 || launched < max_graphs)) {
+                    const int g = batch_lane_group(b, l);
+                    if (g < 0) break;
+                    hipStream_t st = b->lstream[l];
+                    hipEvent_t ev;
+                    hipLaunchKernelGGL(k_batch_set_group, dim3(1), dim3(64), 0, st, b->T + l, g);
+                    hipError_t le = hipSuccess;
+                    if (b->use_graph) le = hipGraphLaunch(b->exec[l], st);
+                    else batch_enqueue(b, l);
+                    if (hipGetLastError() != hipSuccess || le != hipSuccess ||
+                        hipMemcpyAsync(b->hS + (size_t)b->ring * BATCH_MAX, b->S,
+                                       BATCH_MAX * sizeof(BState), hipMemcpyDeviceToHost,
+                                       st) != hipSuccess ||
+                        hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+                        hipEventRecord(ev, st) != hipSuccess) {
+                        rc = EMF_ERR_HIP;
+                        break;
+                    }
+                    pend.push_back({ l, b->ring, g, ev });
+                    ++inflight[l];
+                    b->ring = (b->ring + 1) % BATCH_RING;
+                    ++launched;
                 }
-                pend.push_back({ b->ring, ev });
-                b->ring = (b->ring + 1) % 3;
-                ++launched;
             }
             if (pend.empty()) break;
             const Pend p = pend.front();
@@ -441,7 +533,7 @@ int emf_batch_remove(emf_batch *b, int member)
         EMF_FAIL(EMF_ERR_ARG, "no member %d in this batch", member);
     BatchSlot &q = b->sl[member];
     const bool was_running = !q.finished;
-    HIPCHK(hipStreamSynchronize(b->stream));
+    CHECK(batch_sync(b));
     {
         std::lock_guard<std::mutex> lks(q.s->mtx);
         q.s->in_batch = nullptr;

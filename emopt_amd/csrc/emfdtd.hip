@@ -2226,17 +2226,20 @@ int emf_copy_bandwidth(int device, int64_t bytes, int reps, double *gbps)
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
         hipMemsetAsync(a, 0, n * sizeof(double2), st) != hipSuccess)
         rc = EMF_ERR_HIP;
-    std::vector<float> t;
+    // one warm-up launch, then `reps` launches back to back between two
+    // events: the average per launch, without per-launch event overhead (a
+    // copy of a few MB takes a few microseconds)
     const unsigned grid = (unsigned)((n + 255) / 256);
-    for (int r = 0; r < reps + 1 && rc == EMF_OK; ++r) {
-        if (hipEventRecord(e0, st) != hipSuccess) { rc = EMF_ERR_HIP; break; }
+    float ms = 0;
+    if (rc == EMF_OK) {
         hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, st, a, b, n);
-        float ms = 0;
-        if (hipGetLastError() != hipSuccess || hipEventRecord(e1, st) != hipSuccess ||
-            hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+        if (hipEventRecord(e0, st) != hipSuccess) rc = EMF_ERR_HIP;
+        for (int r = 0; r < reps && rc == EMF_OK; ++r)
+            hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, st, a, b, n);
+        if (rc == EMF_OK &&
+            (hipGetLastError() != hipSuccess || hipEventRecord(e1, st) != hipSuccess ||
+             hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
             rc = EMF_ERR_HIP;
-        else if (r > 0)
-            t.push_back(ms);   // first launch is a warm-up
     }
     (void)hipStreamSynchronize(st);
     if (e0) (void)hipEventDestroy(e0);
@@ -2244,9 +2247,7 @@ int emf_copy_bandwidth(int device, int64_t bytes, int reps, double *gbps)
     (void)hipStreamDestroy(st);
     dfree(a); dfree(b);
     if (rc != EMF_OK) EMF_FAIL(rc, "copy kernel failed");
-    std::sort(t.begin(), t.end());
-    double ms = t[t.size() / 2];
-    *gbps = 2.0 * n * sizeof(double2) / (ms * 1e-3) / 1e9;
+    *gbps = 2.0 * n * sizeof(double2) * reps / (ms * 1e-3) / 1e9;
     return EMF_OK;
 }
 

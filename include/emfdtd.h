@@ -322,7 +322,9 @@ EMF_API int emf_set_streaming(emf_solver *s, int mode);
 EMF_API int emf_mem_info(int device, int64_t *free_bytes, int64_t *total_bytes);
 
 /* Device copy bandwidth (GB/s, read + write bytes) of a plain vector copy of
- * `bytes` bytes, median of `reps` launches; a roofline reference point. */
+ * `bytes` bytes: one warm-up launch, then the average of `reps` launches back
+ * to back over the same two buffers; a roofline reference point (and, for
+ * buffers smaller than the Infinity Cache, the cache's copy rate). */
 EMF_API int emf_copy_bandwidth(int device, int64_t bytes, int reps, double *gbps);
 
 /* ---- batched solves (no counterpart in the reference) ------------------ */
@@ -398,6 +400,22 @@ EMF_API int emf_batch_set_tuning(emf_batch *b, int steps_per_graph, int zc, int 
  * earlier results for the first admitted).  group_bytes = 0 (default): one
  * group, every replay steps all members.  Results never depend on either. */
 EMF_API int emf_batch_set_grouping(emf_batch *b, int64_t group_bytes, int rotate);
+
+/* Groups stepped at once (1..4, default 2), each by its own stream with its
+ * own copy of the work table and graph: lane l steps the groups g with
+ * g % lanes == l (the first `lanes` groups until their members finish, or
+ * those in turn with rotate).  Lanes share no data, so results do not depend
+ * on the number of lanes; the hot state of all lanes' groups together should
+ * fit the Infinity Cache (lanes x group_bytes).  No counterpart in the
+ * reference (emopt/fdtd.py runs one solve at a time). */
+EMF_API int emf_batch_set_lanes(emf_batch *b, int lanes);
+
+/* use_graph = 1 (default): each replay is one launch of the captured graph;
+ * 0: the same kernels are launched one by one on the batch's stream (for
+ * per-kernel tracing; slower by the launch overhead).  Results are the same
+ * either way.  No counterpart in the reference (emopt/fdtd.py runs one solve
+ * at a time). */
+EMF_API int emf_batch_set_graph(emf_batch *b, int use_graph);
 
 /* Admits solver s: resets its fields and CPML as emf_solve does and queues
  * the solve (result slot, rtol, nmax, Tn >= 1, fixed_n0 as for emf_solve).

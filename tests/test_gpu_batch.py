@@ -150,6 +150,56 @@ def test_batch_grouped_stepping(cases, refs, rotate):
     flat.close()
 
 
+def test_batch_lanes_give_identical_results(cases, refs):
+    """One group per member, stepped by 1, 2 or 3 lanes (streams) at once,
+    the same groups throughout or in turns: every member's result is exactly
+    the same, and matches its single solve."""
+    from emopt_amd.batch import Batch
+    one = max(80 * c["Nx"] * c["Ny"] * c["Nz"] for c in cases)
+    out = {}
+    for lanes, rotate in ((1, False), (2, False), (3, True), (2, True)):
+        b = Batch(steps_per_graph=8, group_bytes=one, rotate_groups=rotate, lanes=lanes)
+        sims = [build_sim(c) for c in cases]
+        for s, c in zip(sims, cases):
+            b.add(s, adjoint=c["adjoint"], postprocess=False)
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            b.run()
+        assert b.running == 0
+        out[lanes, rotate] = [all_fields(s, c, c["adjoint"]) for s, c in zip(sims, cases)]
+        if (lanes, rotate) == (2, False):
+            for q, (s, c, r) in enumerate(zip(sims, cases, refs)):
+                _check(s, c, r, "2 lanes, member %d" % q)
+        b.close()
+    base = out[1, False]
+    for key, f in out.items():
+        for q in range(len(cases)):
+            for x, y in zip(f[q], base[q]):
+                assert np.array_equal(x, y), (key, q)
+
+
+def test_batch_direct_launches_equal_graph_replays(cases):
+    """The replay's kernels launched one by one (for tracing) give exactly
+    the graph's result."""
+    from emopt_amd.batch import Batch
+    out = []
+    for graph in (True, False):
+        b = Batch(steps_per_graph=16, use_graph=graph)
+        sims = [build_sim(cases[q]) for q in (1, 4)]
+        for s, q in zip(sims, (1, 4)):
+            b.add(s, adjoint=cases[q]["adjoint"], postprocess=False)
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            b.run()
+        out.append([(dict(s.last_solve), all_fields(s, cases[q], cases[q]["adjoint"]))
+                    for s, q in zip(sims, (1, 4))])
+        b.close()
+    for (la, fa), (lb, fb) in zip(*out):
+        assert la["n0"] == lb["n0"] and la["steps"] == lb["steps"]
+        for x, y in zip(fa, fb):
+            assert np.array_equal(x, y)
+
+
 def test_batch_fixed_n0_and_nmax(cases):
     """fixed_n0 members (status 'fixed') and a member stopped by Nmax
     (status 'nmax') against the single-solve path."""
