@@ -104,6 +104,13 @@ _SIGS = {
     "emf_batch_set_tuning": ([_P, c_int, c_int, c_int], c_int),
     "emf_batch_set_grouping": ([_P, c_int64, c_int], c_int),
     "emf_batch_set_graph": ([_P, c_int], c_int),
+    "emf_stage_clear": ([_P], c_int),
+    "emf_stage_count": ([_P, _P], c_int),
+    "emf_stage_interp_T": ([_P, c_int, _P, c_int, c_double] + [c_int] * 6 + [_P], c_int),
+    "emf_stage_power_face": ([_P, c_int, _P, c_double, c_double, c_double] + [c_int] * 12 + [_P],
+                             c_int),
+    "emf_stage_register": ([_P], c_int),
+    "emf_stage_get": ([_P, c_int, _P, _P], c_int),
     "emf_batch_set_lanes": ([_P, c_int], c_int),
     "emf_batch_add": ([_P, _P, c_int, c_double, c_double, c_int64, c_int64, POINTER(c_int)], c_int),
     "emf_batch_run": ([_P, c_int, c_int, POINTER(c_int)], c_int),

@@ -33,7 +33,8 @@ import numpy as np
 
 from .misc import DomainCoordinates
 
-__all__ = ["ModeMatch", "interpolated_dFdx_3D", "power_norm_dFdx_3D", "AdjointMethod",
+__all__ = ["ModeMatch", "interpolated_dFdx_3D", "power_norm_dFdx_3D",
+           "power_norm_dFdx_3D_device", "AdjointMethod",
            "AdjointMethodPNF3D", "FDTDAdjointFunction", "fdtd_fom", "rect_edge_deps"]
 
 
@@ -108,6 +109,18 @@ class ModeMatch:
         self.bm = self.am1 - self.am2
         self.mode_match_fwd = 0.5 * self.am * np.conj(self.am) * np.real(self.Pm)
         self.mode_match_back = 0.5 * self.bm * np.conj(self.bm) * np.real(self.Pm)
+
+    def dFdx_device(self, sim, weight_set=0, mask=0b111111):
+        """The get_dFd* planes in the form the device staging takes:
+        (weight_set, scale, mask) with dF/dfield_q = scale_q * W_q for the
+        weight planes W already on `sim` (compute_device uploads them).  Call
+        after compute_device; a component whose mask bit is clear is zero."""
+        key = (id(sim), weight_set)
+        if getattr(self, "_dev_key", None) != key:
+            sim.set_plane_weights(weight_set, self._weights())
+            self._dev_key = key
+        kE, kH = self._k(False), self._k(True)
+        return weight_set, (kE, kE, kE, kH, kH, kH), mask
 
     def get_mode_match_forward(self, P_in):
         return np.real(self.mode_match_fwd) / P_in
@@ -261,6 +274,37 @@ def power_norm_dFdx_3D(sim, f, domain, dfdEx, dfdEy, dfdEz, dfdHx, dfdHy, dfdHz)
     return [srcs, doms]
 
 
+_FACE_COMPS = (("Ey", "Ez", "Hy", "Hz"), ("Ez", "Ex", "Hz", "Hx"), ("Ex", "Ey", "Hx", "Hy"))
+
+
+def power_norm_dFdx_3D_device(sim, f, domain, weight_set, scale, mask=0b111111):
+    """power_norm_dFdx_3D built on the GPU into sim's staged sources: the FOM
+    term from the derivative planes scale_q * W_q (W the weight set, as
+    ModeMatch.dFdx_device gives them) and the six face terms from the solved
+    fields, each mapped back onto the Yee grid there.  The same blocks as
+    power_norm_dFdx_3D's host arrays (the same arithmetic, in the same order
+    as numpy evaluates it on an FMA-capable host).  Returns False, with
+    nothing staged for the faces from the first face whose grown box leaves
+    the grid on, when a face needs the host path."""
+    from .defs import FIELD_INDEX
+    Psrc = sim.source_power
+    sim.stage_planes(weight_set, scale, Psrc, domain, mask)
+    dx, dy, dz = sim.dx, sim.dy, sim.dz
+    x1, x2, y1, y2, z1, z2 = _box_planes(sim)
+    faces = ((x1, -1, 0, 0, dy * dz), (x2, +1, None, 0, dy * dz),
+             (y1, -1, 1, 1, dx * dz), (y2, +1, None, 1, dx * dz),
+             (z1, -1, 2, 2, dx * dy), (z2, +1, None, 2, dx * dy))
+    mf = -1 * f
+    rp2 = 1.0 / Psrc ** 2
+    for dom, sgn, sym_axis, axis, area in faces:
+        if sym_axis is not None and sim.bc[sym_axis] in "EH":
+            continue
+        comps = [FIELD_INDEX[c] for c in _FACE_COMPS[axis]]
+        if not sim.stage_power_face(comps, sgn * 0.25 * area, mf, rp2, dom):
+            return False
+    return True
+
+
 def rect_edge_deps(sim, eps, prim, attr, delta, bbox):
     """Sparse dEps/dp of moving one coordinate of a primitive: the cells of
     `bbox` (index box k1, k2, j1, j2, i1, i2) whose sampled eps changes,
@@ -314,6 +358,10 @@ class AdjointMethod:
 
     exact_adjoint = True
 
+    # build the adjoint sources on the GPU when the method offers it
+    # (stage_dFdx); False: always the host path of calc_dFdx
+    device_sources = True
+
     def __init__(self, sim, step=1e-8):
         self.sim = sim
         self._step = step
@@ -331,6 +379,12 @@ class AdjointMethod:
 
     def calc_grad_p(self, sim, params):
         return np.zeros(len(params))
+
+    def stage_dFdx(self, sim, params):
+        """Build the adjoint sources on the GPU (sim's staged sources) and
+        return True, or return False for the host path (calc_dFdx).  The
+        base class has no device form."""
+        return False
 
     def get_update_boxes(self, sim, params):
         N = len(params)
@@ -358,8 +412,11 @@ class AdjointMethod:
         return np.array_equal(self.prev_params, np.asarray(params, dtype=np.double))
 
     def solve_adjoint_for(self, params):
-        src = self.calc_dFdx(self.sim, params)
-        self.sim.set_adjoint_sources(src)
+        if self.device_sources and self.stage_dFdx(self.sim, params):
+            self.sim.set_adjoint_sources_staged()
+        else:
+            src = self.calc_dFdx(self.sim, params)
+            self.sim.set_adjoint_sources(src)
         self.sim.solve_adjoint()
 
     def calc_gradient(self, sim, params):
@@ -494,8 +551,28 @@ class AdjointMethodPNF3D(AdjointMethod):
     def calc_dfdx(self, sim, params):
         raise NotImplementedError
 
+    def calc_dfdx_device(self, sim, params):
+        """Optional device form of calc_dfdx: per FOM domain, (weight_set,
+        scale, mask) with df/dfield_q = scale_q * W_q for weight planes W on
+        the solver (ModeMatch.dFdx_device).  None: host path only."""
+        return None
+
     def get_fom_domains(self):
         raise NotImplementedError
+
+    def stage_dFdx(self, sim, params):
+        """calc_dFdx's sources built on the GPU (power_norm_dFdx_3D_device)
+        when calc_dfdx_device gives the planes."""
+        specs = self.calc_dfdx_device(sim, params)
+        if specs is None:
+            return False
+        f = self.calc_f(sim, params)
+        sim.stage_clear()
+        for spec, dom in zip(specs, self.get_fom_domains()):
+            if not power_norm_dFdx_3D_device(sim, f, dom, *spec):
+                sim.stage_clear()
+                return False
+        return True
 
     def calc_penalty(self, sim, params):
         return 0.0

@@ -190,6 +190,7 @@ struct emf_solver {
     double2 *res[2][6] = {};       // result slots: fwd, adj (one 6n block per slot)
 
     std::vector<SourceDev> srcs;
+    std::vector<SourceDev> staged; // adjoint sources built on the device (emf_stage_*)
     SrcBlk *src_desc = nullptr;    // device copy of the source descriptors
     size_t src_desc_cap = 0;
     bool src_dirty = true;
@@ -1068,6 +1069,9 @@ int emf_delete(emf_solver *s)
         for (auto &d : s->srcs)
             for (int c = 0; c < 6; ++c) dfree(d.f[c]);
         s->srcs.clear();
+        for (auto &d : s->staged)
+            for (int c = 0; c < 6; ++c) dfree(d.f[c]);
+        s->staged.clear();
         dfree(s->src_desc);
         dfree(s->conv);
         dfree(s->samples);
@@ -1324,15 +1328,17 @@ int emf_add_source(emf_solver *s, const double *Jx, const double *Jy, const doub
     std::vector<double> buf(2 * cnt);
     for (int c = 0; c < 6; ++c) {
         for (long long m = 0; m < cnt; ++m) {
-            double re = f[c][2 * m], im = f[c][2 * m + 1];
-            double amp = re, ph = im;
-            if (calc_phase) {   // (|v|, -atan2(im, re)), src/fdtd.cpp:1517-1576
-                amp = sqrt(re * re + im * im);
-                ph = (im == 0 && re == 0) ? 0.0 : -1 * atan2(im, re);
-            }
+            const double re = f[c][2 * m], im = f[c][2 * m + 1];
             // stored as (A cos(phi), A sin(phi)); see SrcBlk
-            buf[2 * m] = amp * cos(ph);
-            buf[2 * m + 1] = amp * sin(ph);
+            if (calc_phase) {
+                // the reference keeps (|v|, -atan2(im, re)) (src/fdtd.cpp:1517-1576),
+                // whose (A cos(phi), A sin(phi)) is (re, -im)
+                buf[2 * m] = re;
+                buf[2 * m + 1] = -im;
+            } else {
+                buf[2 * m] = re * cos(im);
+                buf[2 * m + 1] = re * sin(im);
+            }
         }
         int rc = dalloc(&d.f[c], cnt);
         if (rc != EMF_OK) {
@@ -1794,6 +1800,246 @@ int emf_interp_dot(emf_solver *s, int slot, int id, int k1, int k2, int j1, int 
         out[0] += part[2 * b].x; out[1] += part[2 * b].y;
         out[2] += part[2 * b + 1].x; out[3] += part[2 * b + 1].y;
     }
+    return EMF_OK;
+}
+
+// ------------------------------------------------ staged (device-built) sources
+
+namespace {
+
+void free_blocks(std::vector<SourceDev> &v)
+{
+    for (auto &d : v)
+        for (int c = 0; c < 6; ++c) dfree(d.f[c]);
+    v.clear();
+}
+
+// The transposed interpolation of interpolated_dFdx_3D
+// (emopt/fomutils.py:1282-1429) of six planes P (device arrays of the
+// domain's Ni x Nj x Nk cells; null = zero) of a domain whose min x and y
+// indices are k1 and j1: each plane is added, scaled, into a zero array
+// padded by one cell on every side, in the reference's order of terms; the
+// padded arrays, cropped to the grown domain and then to place = (i0, j0,
+// k0, id1, jd1, kd1, sI, sJ, sK) (the grid block and its offset into the
+// grown domain), become one staged source block.
+int stage_transposed(emf_solver *s, const double2 *const P[6], int Ni, int Nj, int Nk, int k1,
+                     int j1, const int *place)
+{
+    const int Pz = Ni + 2, Py = Nj + 2, Px = Nk + 2;
+    const long long np = (long long)Pz * Py * Px;
+    double2 *pad[6] = {};
+    int rc = EMF_OK;
+    for (int c = 0; c < 6 && rc == EMF_OK; ++c) {
+        rc = dalloc(&pad[c], (size_t)np);
+        if (rc == EMF_OK)
+            hipLaunchKernelGGL(k_fill2, dim3(grid1d(np, 256)), dim3(256), 0, s->stream, pad[c], np,
+                               make_double2(0.0, 0.0));
+    }
+    // one term: pad[c][dz.., dy.., dx..] += w * P[c][sz.., sy.., sx..] over nz x ny x nx
+    auto term = [&](int c, int dz, int dy, int dx, int sz, int sy, int sx, int nz, int ny, int nx,
+                    double w, int assign) {
+        if (rc != EMF_OK || !P[c] || nz <= 0 || ny <= 0 || nx <= 0) return;
+        BoxMap b{Py, Px, dz, dy, dx, Nj, Nk, sz, sy, sx, nz, ny, nx};
+        const long long n = (long long)nz * ny * nx;
+        hipLaunchKernelGGL(k_box_axpy, dim3(grid1d(n, 256)), dim3(256), 0, s->stream, pad[c], P[c],
+                           b, w, assign);
+    };
+    // the whole plane at slice origins (z, y, x): I = 1, O = 0, M = 2
+    auto full = [&](int c, int z, int y, int x, double w) { term(c, z, y, x, 0, 0, 0, Ni, Nj, Nk, w, 0); };
+    const char bx = s->bc[0], by = s->bc[1];
+    const double sgn = by == 'H' ? 1.0 : -1.0;
+    const bool ymir = j1 == 0 && (by == 'H' || by == 'E');
+    // Ex
+    full(0, 1, 1, 1, 0.25); full(0, 1, 1, 0, 0.25); full(0, 2, 1, 1, 0.25); full(0, 2, 1, 0, 0.25);
+    if (k1 == 0 && bx == 'H') {
+        term(0, 1, 1, 1, 0, 0, 0, Ni, Nj, 1, 0.25, 0);
+        term(0, 2, 1, 1, 0, 0, 0, Ni, Nj, 1, 0.25, 0);
+    } else if (k1 == 0 && bx == 'E') {
+        // as in the reference, this branch indexes the y axis
+        term(0, 1, 1, 1, 0, 0, 0, Ni, 1, Nk, -0.25, 0);
+        term(0, 2, 1, 1, 0, 0, 0, Ni, 1, Nk, -0.25, 0);
+    }
+    // Ey
+    full(1, 1, 1, 1, 0.25); full(1, 1, 0, 1, 0.25); full(1, 2, 1, 1, 0.25); full(1, 2, 0, 1, 0.25);
+    if (ymir) {
+        term(1, 1, 1, 1, 0, 0, 0, Ni, 1, Nk, sgn * 0.25, 0);
+        term(1, 2, 1, 1, 0, 0, 0, Ni, 1, Nk, sgn * 0.25, 0);
+    }
+    // Ez: assigned, not added
+    term(2, 1, 1, 1, 0, 0, 0, Ni, Nj, Nk, 1.0, 1);
+    // Hx
+    full(3, 1, 1, 1, 0.5); full(3, 1, 0, 1, 0.5);
+    if (ymir) term(3, 1, 1, 1, 0, 0, 0, Ni, 1, Nk, sgn * 0.5, 0);
+    // Hy
+    full(4, 1, 1, 1, 0.5); full(4, 1, 1, 0, 0.5);
+    // Hz
+    full(5, 1, 1, 1, 0.125); full(5, 1, 0, 1, 0.125); full(5, 1, 1, 0, 0.125);
+    full(5, 1, 0, 0, 0.125); full(5, 2, 1, 1, 0.125); full(5, 2, 0, 1, 0.125);
+    full(5, 2, 1, 0, 0.125); full(5, 2, 0, 0, 0.125);
+    if (ymir) {
+        term(5, 1, 1, 1, 0, 0, 0, Ni, 1, Nk, sgn * 0.125, 0);
+        term(5, 1, 1, 0, 0, 0, 0, Ni, 1, Nk, sgn * 0.125, 0);
+        term(5, 2, 1, 1, 0, 0, 0, Ni, 1, Nk, sgn * 0.125, 0);
+        term(5, 2, 1, 0, 0, 0, 0, Ni, 1, Nk, sgn * 0.125, 0);
+    }
+    if (rc == EMF_OK && hipGetLastError() != hipSuccess) rc = EMF_ERR_HIP;
+    // crop: grown cell (a, b, c) is pad[a + 1][b][c]; the block starts at
+    // (id1, jd1, kd1) of the grown domain
+    const int sI = place[6], sJ = place[7], sK = place[8];
+    const long long cnt = (long long)sI * sJ * sK;
+    if (rc == EMF_OK && cnt > 0) {
+        if (place[3] < 0 || place[4] < 0 || place[5] < 0 || place[3] + sI > Ni + 1 ||
+            place[4] + sJ > Nj + 1 || place[5] + sK > Nk + 1 || place[0] < 0 || place[1] < 0 ||
+            place[2] < 0 || place[0] + sI > s->Nz || place[1] + sJ > s->Ny || place[2] + sK > s->Nx) {
+            rc = EMF_ERR_ARG;
+        } else {
+            SourceDev d;
+            d.i0 = place[0]; d.j0 = place[1]; d.k0 = place[2]; d.I = sI; d.J = sJ; d.K = sK;
+            for (int c = 0; c < 6; ++c) d.f[c] = nullptr;
+            for (int c = 0; c < 6 && rc == EMF_OK; ++c) {
+                rc = dalloc(&d.f[c], (size_t)cnt);
+                if (rc == EMF_OK)
+                    hipLaunchKernelGGL(k_stage_block, dim3(grid1d(cnt, 256)), dim3(256), 0,
+                                       s->stream, d.f[c], pad[c], Py, Px, place[3] + 1, place[4],
+                                       place[5], sI, sJ, sK);
+            }
+            if (rc == EMF_OK && hipGetLastError() != hipSuccess) rc = EMF_ERR_HIP;
+            if (rc == EMF_OK) s->staged.push_back(d);
+            else for (int c = 0; c < 6; ++c) dfree(d.f[c]);
+        }
+    }
+    (void)hipStreamSynchronize(s->stream);
+    for (int c = 0; c < 6; ++c) dfree(pad[c]);
+    if (rc == EMF_ERR_ARG) EMF_FAIL(rc, "source placement outside the grown domain or the grid");
+    if (rc != EMF_OK) EMF_FAIL(rc, "staging a source block failed");
+    return EMF_OK;
+}
+
+}  // namespace
+
+int emf_stage_clear(emf_solver *s)
+{
+    LOCK(s);
+    (void)hipStreamSynchronize(s->stream);
+    free_blocks(s->staged);
+    return EMF_OK;
+}
+
+int emf_stage_count(emf_solver *s, int *count)
+{
+    LOCK(s);
+    if (!count) EMF_FAIL(EMF_ERR_ARG, "null output");
+    *count = (int)s->staged.size();
+    return EMF_OK;
+}
+
+int emf_stage_interp_T(emf_solver *s, int id, const double *scale, int mask, double rdiv, int k1,
+                       int k2, int j1, int j2, int i1, int i2, const int *place)
+{
+    LOCK(s);
+    if (id < 0 || id >= emf_solver::kWeightSets || !scale || !place)
+        EMF_FAIL(EMF_ERR_ARG, "bad weight set or null argument");
+    if (k1 > k2 || j1 > j2 || i1 > i2) EMF_FAIL(EMF_ERR_ARG, "empty or inverted domain");
+    const int Nk = k2 - k1, Nj = j2 - j1, Ni = i2 - i1;
+    const long long cnt = (long long)Nk * Nj * Ni;
+    if (cnt != s->wcount[id])
+        EMF_FAIL(EMF_ERR_ARG, "domain has %lld cells, weight set %d has %lld", cnt, id,
+                 (long long)s->wcount[id]);
+    if (cnt == 0) return EMF_OK;
+    double2 *P[6] = {};
+    int rc = EMF_OK;
+    for (int q = 0; q < 6 && rc == EMF_OK; ++q) {
+        if (!(mask >> q & 1) || !s->wplane[id][q]) continue;
+        rc = dalloc(&P[q], (size_t)cnt);
+        if (rc == EMF_OK)
+            hipLaunchKernelGGL(k_plane_scale, dim3(grid1d(cnt, 256)), dim3(256), 0, s->stream, P[q],
+                               s->wplane[id][q], scale[2 * q], scale[2 * q + 1], rdiv, cnt);
+    }
+    if (rc == EMF_OK && hipGetLastError() != hipSuccess) rc = EMF_ERR_HIP;
+    if (rc == EMF_OK) rc = stage_transposed(s, P, Ni, Nj, Nk, k1, j1, place);
+    (void)hipStreamSynchronize(s->stream);
+    for (int q = 0; q < 6; ++q) dfree(P[q]);
+    return rc;
+}
+
+int emf_stage_power_face(emf_solver *s, int slot, const int *comps, double s1, double mf,
+                         double rp2, int k1, int k2, int j1, int j2, int i1, int i2, int ek1,
+                         int ek2, int ej1, int ej2, int ei1, int ei2, const int *place)
+{
+    LOCK(s);
+    if (!comps || !place) EMF_FAIL(EMF_ERR_ARG, "null argument");
+    for (int q = 0; q < 4; ++q)
+        if (comps[q] < 0 || comps[q] > 5) EMF_FAIL(EMF_ERR_ARG, "component must be 0..5");
+    CHECK(check_interp_boxes(s, slot, k1, k2, j1, j2, i1, i2, ek1, ek2, ej1, ej2, ei1, ei2));
+    const int Nk = k2 - k1, Nj = j2 - j1, Ni = i2 - i1;
+    const long long cnt = (long long)Nk * Nj * Ni;
+    if (cnt == 0) return EMF_OK;
+    FaceArgs fa{};
+    double2 *P[6] = {};
+    int rc = EMF_OK;
+    for (int q = 0; q < 4 && rc == EMF_OK; ++q) {
+        fa.F[q] = s->res[slot][comps[q]];
+        fa.st[q] = interp_stencil(s, comps[q], ek1, ej1);
+        if (P[comps[q]]) { rc = EMF_ERR_ARG; break; }
+        rc = dalloc(&P[comps[q]], (size_t)cnt);
+        fa.out[q] = P[comps[q]];
+    }
+    if (rc == EMF_OK) {
+        fa.s1 = s1; fa.mf = mf; fa.rp2 = rp2;
+        const IBox dom{k1, k2, j1, j2, i1, i2}, ext{ek1, ek2, ej1, ej2, ei1, ei2};
+        hipLaunchKernelGGL(k_face_dP, dim3(grid1d(cnt, 256)), dim3(256), 0, s->stream, fa, s->Nx,
+                           s->Ny, dom, ext);
+        if (hipGetLastError() != hipSuccess) rc = EMF_ERR_HIP;
+    }
+    if (rc == EMF_OK) rc = stage_transposed(s, P, Ni, Nj, Nk, k1, j1, place);
+    (void)hipStreamSynchronize(s->stream);
+    for (int q = 0; q < 6; ++q) dfree(P[q]);
+    if (rc == EMF_ERR_ARG) EMF_FAIL(rc, "the four components must differ");
+    return rc;
+}
+
+int emf_stage_register(emf_solver *s)
+{
+    LOCK(s);
+    (void)hipStreamSynchronize(s->stream);
+    free_blocks(s->srcs);
+    for (const auto &b : s->staged) {
+        SourceDev d = b;
+        const long long cnt = (long long)b.I * b.J * b.K;
+        int rc = EMF_OK;
+        for (int c = 0; c < 6; ++c) d.f[c] = nullptr;
+        for (int c = 0; c < 6 && rc == EMF_OK; ++c) {
+            rc = dalloc(&d.f[c], (size_t)cnt);
+            if (rc == EMF_OK &&
+                hipMemcpyAsync(d.f[c], b.f[c], cnt * sizeof(double2), hipMemcpyDeviceToDevice,
+                               s->stream) != hipSuccess)
+                rc = EMF_ERR_HIP;
+        }
+        if (rc != EMF_OK) {
+            for (int c = 0; c < 6; ++c) dfree(d.f[c]);
+            s->src_dirty = true;
+            EMF_FAIL(rc, "registering the staged sources failed");
+        }
+        s->srcs.push_back(d);
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->src_dirty = true;
+    return EMF_OK;
+}
+
+int emf_stage_get(emf_solver *s, int idx, int *geom, double *out)
+{
+    LOCK(s);
+    if (idx < 0 || idx >= (int)s->staged.size() || !geom)
+        EMF_FAIL(EMF_ERR_ARG, "no staged block %d", idx);
+    const SourceDev &d = s->staged[idx];
+    geom[0] = d.i0; geom[1] = d.j0; geom[2] = d.k0; geom[3] = d.I; geom[4] = d.J; geom[5] = d.K;
+    if (!out) return EMF_OK;
+    const long long cnt = (long long)d.I * d.J * d.K;
+    for (int c = 0; c < 6; ++c)
+        HIPCHK(hipMemcpyAsync(out + 2 * c * cnt, d.f[c], cnt * sizeof(double2),
+                              hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
     return EMF_OK;
 }
 

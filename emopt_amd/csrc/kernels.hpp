@@ -694,4 +694,108 @@ __global__ __launch_bounds__(RED_THREADS) void k_interp_dot(DotArgs da, int Nx, 
     }
 }
 
+// ------------------------------------------- adjoint sources built on the device
+//
+// The reference forms adjoint sources on the host (emopt/fomutils.py:1282-1429,
+// 1654-1920): derivative planes on the Ez grid, mapped back onto the Yee
+// grid by the transpose of the interpolation stencils.  Here the planes are
+// formed and transposed on the device, and the result is appended to the
+// solver's staged sources without crossing to the host.
+
+// P = (k x W) / div for a plane W of n cells (k complex, div real), rounded
+// as numpy rounds the host formula k * W / div on an FMA-capable host: the
+// product as fma(kr, wr, -(ki wi)) + i fma(kr, wi, ki wr), the quotient as a
+// product with the reciprocal rdiv = 1 / div
+__global__ void k_plane_scale(double2 *__restrict__ out, const double2 *__restrict__ W, double kr,
+                              double ki, double rdiv, long long n)
+{
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < n;
+         g += (long long)gridDim.x * blockDim.x) {
+        const double2 w = W[g];
+        const double tr = fma(kr, w.x, -(ki * w.y)), ti = fma(kr, w.y, ki * w.x);
+        out[g] = make_double2(tr * rdiv, ti * rdiv);
+    }
+}
+
+// The derivative planes of one face of the source-power box
+// (emopt/fomutils.py:1654-1920): with S = 0.5 Re(E1 H2* - E2 H1*) on the
+// face, dS/dE1 = s1 H2*, dS/dE2 = -s1 H1*, dS/dH1 = -s1 E2*, dS/dH2 = s1 E1*
+// (s1 = +-area/4 for the face's outward sign), each times mf = -f and
+// divided by Psrc^2 (as a product with rp2 = 1 / Psrc^2, the way numpy
+// divides a complex array by a real scalar).  F: the slot's E1, E2, H1, H2;
+// out: their planes.
+struct FaceArgs {
+    const double2 *F[4];
+    IStencil st[4];
+    double2 *out[4];
+    double s1, mf, rp2;
+};
+
+__global__ void k_face_dP(FaceArgs fa, int Nx, int Ny, IBox dom, IBox ext)
+{
+    const int nk = dom.k2 - dom.k1, nj = dom.j2 - dom.j1;
+    const long long total = (long long)nk * nj * (dom.i2 - dom.i1);
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < total;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int k = dom.k1 + (int)(g % nk), j = dom.j1 + (int)((g / nk) % nj);
+        const int i = dom.i1 + (int)(g / ((long long)nk * nj));
+        double2 f[4];
+        for (int q = 0; q < 4; ++q) f[q] = interp_at(fa.F[q], Nx, Ny, ext, fa.st[q], k, j, i);
+        // (coefficient, conjugated field) of E1, E2, H1, H2
+        const double c[4] = { fa.s1, -fa.s1, -fa.s1, fa.s1 };
+        const int src[4] = { 3, 2, 1, 0 };
+        for (int q = 0; q < 4; ++q) {
+            const double2 v = f[src[q]];
+            const double dr = c[q] * v.x, di = c[q] * (-v.y);
+            fa.out[q][g] = make_double2((fa.mf * dr) * fa.rp2, (fa.mf * di) * fa.rp2);
+        }
+    }
+}
+
+// dst[box] += w * src[box] (or dst[box] = src[box] with assign) over an
+// nz x ny x nx box of two (z, y, x) row-major arrays: one term of the
+// transposed interpolation.  w is +-1/2^m, so w * src is exact and the sum
+// rounds once, as src * w then += does.
+struct BoxMap {
+    int dNy, dNx, dz, dy, dx;     // destination row lengths and box origin
+    int sNy, sNx, sz, sy, sx;     // source row lengths and box origin
+    int nz, ny, nx;
+};
+
+__global__ void k_box_axpy(double2 *__restrict__ dst, const double2 *__restrict__ src, BoxMap b,
+                           double w, int assign)
+{
+    const long long total = (long long)b.nz * b.ny * b.nx;
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < total;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(g % b.nx), r = (int)((g / b.nx) % b.ny);
+        const int a = (int)(g / ((long long)b.nx * b.ny));
+        const long long d = ((long long)(b.dz + a) * b.dNy + (b.dy + r)) * b.dNx + (b.dx + c);
+        const long long q = ((long long)(b.sz + a) * b.sNy + (b.sy + r)) * b.sNx + (b.sx + c);
+        const double2 v = src[q];
+        if (assign) {
+            dst[d] = v;
+        } else {
+            const double2 o = dst[d];
+            dst[d] = make_double2(o.x + v.x * w, o.y + v.y * w);
+        }
+    }
+}
+
+// One source block from a padded transposed array: block cell (a, b, c) is
+// pad[oz + a][oy + b][ox + c], stored as the update reads sources: a complex
+// value v becomes (Re v, -Im v) = (|v| cos phi, |v| sin phi) with
+// phi = -arg v (src/fdtd.cpp:1517-1576).
+__global__ void k_stage_block(double2 *__restrict__ blk, const double2 *__restrict__ pad, int Py,
+                              int Px, int oz, int oy, int ox, int I, int J, int K)
+{
+    const long long total = (long long)I * J * K;
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < total;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(g % K), b = (int)((g / K) % J), a = (int)(g / ((long long)J * K));
+        const double2 v = pad[((long long)(oz + a) * Py + (oy + b)) * Px + (ox + c)];
+        blk[g] = make_double2(v.x, -v.y);
+    }
+}
+
 }  // namespace emf

@@ -110,6 +110,7 @@ class FDTD(MaxwellSolver):
 
         self._sources = []
         self._adj_sources = []
+        self._adj_staged = False       # adjoint sources include the device-staged blocks
         self._bc = ["0", "0", "0"]
         L.emf_set_bc(h, b"000")
         self._eps = None
@@ -386,12 +387,82 @@ class FDTD(MaxwellSolver):
 
     def clear_adjoint_sources(self):
         self._adj_sources = []
+        self._adj_staged = False
+
+    def set_adjoint_sources_staged(self):
+        """Make the blocks staged on the device (stage_planes, stage_power_face)
+        the adjoint sources; sources added with add_adjoint_sources join them."""
+        self._adj_sources = []
+        self._adj_staged = True
 
     def _register(self, sources):
         L, h = self._lib, self._h
-        L.emf_clear_sources(h)
+        if sources is self._adj_sources and self._adj_staged:
+            L.emf_stage_register(h)          # replaces the current sources
+        else:
+            L.emf_clear_sources(h)
         for s in sources:
             L.emf_add_source(h, *[ptr(a) for a in s.arrays], s.i0, s.j0, s.k0, s.I, s.J, s.K, 1)
+
+    # ------------------------------------- adjoint sources built on the device
+    def _grown_place(self, domain):
+        """(i0, j0, k0, id1, jd1, kd1, I, J, K) of interpolated_dFdx_3D's grown
+        domain (one cell more at -x, -y and +z) clipped to the grid, as
+        add_adjoint_sources would place it; None when it misses the grid."""
+        g = domain.copy()
+        g.set_region(domain.k1 - 1, domain.k2, domain.j1 - 1, domain.j2, domain.i1,
+                     domain.i2 + 1, self._dx, self._dy, self._dz)
+        ov = self._local_overlap(g)
+        if ov is None:
+            return None
+        return np.array([v for t in ov for v in t], dtype=np.int32)
+
+    def stage_clear(self):
+        self._lib.emf_stage_clear(self._h)
+
+    def stage_count(self):
+        n = _lib.ctypes.c_int()
+        self._lib.emf_stage_count(self._h, _lib.ctypes.byref(n))
+        return n.value
+
+    def stage_planes(self, weight_set, scale, div, domain, mask=0b111111):
+        """Stage interpolated_dFdx_3D of the planes (scale_q * W_q) / div on
+        `domain`, W the weight set (set_plane_weights) and scale six complex
+        factors (Ex..Hz); components whose mask bit is clear are zero."""
+        pl = self._grown_place(domain)
+        if pl is None:
+            return
+        sc = np.zeros(12)
+        sc[0::2] = np.real(scale)
+        sc[1::2] = np.imag(scale)
+        self._lib.emf_stage_interp_T(self._h, int(weight_set), ptr(sc), int(mask), 1.0 / div,
+                                     domain.k1, domain.k2, domain.j1, domain.j2, domain.i1,
+                                     domain.i2, ptr(pl))
+
+    def stage_power_face(self, comps, s1, mf, rp2, domain, adjoint=False):
+        """Stage one face term of power_norm_dFdx_3D: comps = indices of
+        (E1, E2, H1, H2), s1 = sign * area / 4, mf = -f, rp2 = 1 / Psrc^2.
+        False (nothing staged) when the face's grown box leaves the grid."""
+        from .fields import grown_boxes
+        _, _, dom, ext, ok = grown_boxes(self, domain)
+        pl = self._grown_place(domain)
+        if not ok:
+            return False
+        if pl is None:
+            return True
+        cs = np.array(comps, dtype=np.int32)
+        self._lib.emf_stage_power_face(self._h, 1 if adjoint else 0, ptr(cs), float(s1), float(mf),
+                                       float(rp2), *dom, *ext, ptr(pl))
+        return True
+
+    def staged_block(self, idx):
+        """(i0, j0, k0, I, J, K) and the six stored arrays of staged block idx."""
+        geom = np.zeros(6, dtype=np.int32)
+        self._lib.emf_stage_get(self._h, int(idx), ptr(geom), None)
+        I, J, K = int(geom[3]), int(geom[4]), int(geom[5])
+        out = np.zeros((6, I, J, K), dtype=np.complex128)
+        self._lib.emf_stage_get(self._h, int(idx), ptr(geom), ptr(out))
+        return tuple(int(v) for v in geom), out
 
     # ----------------------------------------------------------------- solve
     def _solve(self, slot):

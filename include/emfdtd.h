@@ -279,6 +279,54 @@ EMF_API int emf_set_plane_weights(emf_solver *s, int id, int64_t count, const do
 EMF_API int emf_interp_dot(emf_solver *s, int slot, int id, int k1, int k2, int j1, int j2, int i1,
                    int i2, int ek1, int ek2, int ej1, int ej2, int ei1, int ei2, double *out);
 
+/* ---- adjoint sources built on the device ------------------------------ */
+/* The reference forms adjoint sources on the host: derivative planes on the
+ * Ez grid (ModeMatch.get_dFd*, emopt/fomutils.py:1118-1196; the face terms
+ * of power_norm_dFdx_3D, :1654-1920) mapped back onto the Yee grid by
+ * interpolated_dFdx_3D (:1282-1429), then passed to FDTD_add_source
+ * (src/fdtd.hpp:628).  These entry points build the same blocks on the
+ * device, into a staged list, and emf_stage_register makes them the
+ * solver's sources, so no plane crosses to the host.  A staged block holds
+ * what emf_add_source would store for the same complex arrays. */
+
+/* Empties the staged list. */
+EMF_API int emf_stage_clear(emf_solver *s);
+
+/* *count = blocks in the staged list. */
+EMF_API int emf_stage_count(emf_solver *s, int *count);
+
+/* Stages interpolated_dFdx_3D of the planes P_q = (scale_q x W_q) / div,
+ * q = Ex..Hz, with W the weight set `id` (emf_set_plane_weights; its cell
+ * count must be the domain's), scale[2q], scale[2q+1] the re and im of
+ * scale_q, rdiv = 1 / div, and P_q = 0 where bit q of mask is clear.  The
+ * domain (k1..i2, Ez-grid indices) fixes the transposed stencils and the
+ * symmetric-boundary terms (at k1 = 0 or j1 = 0); place = (i0, j0, k0, id1,
+ * jd1, kd1, I, J, K) is the grid block the result covers and its offset in
+ * the grown domain (one cell more at -x, -y and +z).  For the PNF3D form:
+ * scale = ModeMatch's dF/dfield factor, div = the source power. */
+EMF_API int emf_stage_interp_T(emf_solver *s, int id, const double *scale, int mask, double rdiv,
+                               int k1, int k2, int j1, int j2, int i1, int i2, const int *place);
+
+/* Stages the source of one face of the source-power box
+ * (power_norm_dFdx_3D, emopt/fomutils.py:1654-1920): with E1, E2, H1, H2 =
+ * components comps[0..3] of result slot `slot` interpolated on the face's
+ * domain (k1..i2; grown box ek1..ei2 as for emf_get_field_interp), the planes
+ * dE1 = s1 H2*, dE2 = -s1 H1*, dH1 = -s1 E2*, dH2 = s1 E1*, each times mf and
+ * rp2 (the reference's -f dS/dx / Psrc^2 with s1 = +-area/4, mf = -f,
+ * rp2 = 1 / Psrc^2), transposed and placed as for emf_stage_interp_T. */
+EMF_API int emf_stage_power_face(emf_solver *s, int slot, const int *comps, double s1, double mf,
+                                 double rp2, int k1, int k2, int j1, int j2, int i1, int i2,
+                                 int ek1, int ek2, int ej1, int ej2, int ei1, int ei2,
+                                 const int *place);
+
+/* Replaces the solver's sources with copies of the staged blocks (the
+ * staged list is kept). */
+EMF_API int emf_stage_register(emf_solver *s);
+
+/* geom = (i0, j0, k0, I, J, K) of staged block idx; with out != NULL also
+ * its six stored components (complex, I*J*K each, Jx..Mz). */
+EMF_API int emf_stage_get(emf_solver *s, int idx, int *geom, double *out);
+
 /* ---- measurement (no counterpart in the reference) --------------------- */
 
 /* Runs `nsteps` time steps starting at step index n_start on the current

@@ -404,6 +404,11 @@ def grating_problem(Nx=360, Ny=60, Nz=120, d=0.03, ntr=32, w=10, wl=1.55, seed=4
             zero = np.zeros(mon.shape, dtype=np.complex128)
             return [(mm.get_dFdEx(), mm.get_dFdEy(), zero, mm.get_dFdHx(), mm.get_dFdHy(), zero)]
 
+        def calc_dfdx_device(self, sim, p):
+            # the same planes as calc_dfdx, left on the GPU (Ez and Hz zero)
+            mm.compute_device(sim, mon)
+            return [mm.dFdx_device(sim, 0, mask=0b011011)]
+
         def calc_deps(self, sim, p):
             """d eps / d edge for every edge: analytic from the overlap rule
             (deps="analytic", the default), or central differences of the

@@ -60,6 +60,55 @@ def _step_oracle(case, nsteps):
     return [o.padded(q)[1:-1, 1:-1, 1:-1] for q in range(6)]
 
 
+def test_two_sample_phasors_match_oracle():
+    """emf_calc_complex_fields_2T (k_phasor2; the reference's
+    FDTD::calc_complex_fields(t0, t1), src/fdtd.cpp:1346-1390) against the
+    oracle on the same state: the t0 snapshot after 12 steps, the live fields
+    after 25 steps."""
+    import warnings
+    from emopt_amd.fdtd import FDTD, FIELD_INDEX, ptr
+    from emopt_amd.misc import DomainCoordinates
+    from oracle import driver as D
+    from tests.cases import _configure
+    case = random_case(13, 11, 9, seed=21, nsrc=2)
+    d, n_cap, n_end = case["d"], 12, 25
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim = FDTD(case["X"], case["Y"], case["Z"], d, d, d, case["wl"])
+        _configure(sim, case, DomainCoordinates)
+        o = D.OracleFDTD(D.load(D.ORACLE_LIB), case["X"], case["Y"], case["Z"], d, d, d,
+                         case["wl"])
+        _configure(o, case, D.Domain)
+    sim.set_material_arrays(case["mats"][:3], case["mats"][3:])
+    sim._register(sim._sources)
+    L, h, dt = sim._lib, sim._h, sim.dt
+    L.emf_reset_pml(h)
+    L.emf_zero_fields(h)
+    o.set_material_arrays(case["mats"][:3], case["mats"][3:])
+    o._bind(o.fwd)
+    o.lib.FDTD_clear_sources(o.h)
+    for arrs, i0, j0, k0, sI, sJ, sK in o.sources:
+        o.lib.FDTD_add_source(o.h, *arrs, i0, j0, k0, sI, sJ, sK, False)
+    o.lib.FDTD_reset_pml(o.h)
+    for n in range(n_end):
+        if n == n_cap:
+            L.emf_capture_t0_fields(h, 0)
+            o.lib.FDTD_capture_t0_fields(o.h)
+        L.emf_update_H(h, n, n * dt)
+        L.emf_update_E(h, n, (n + 0.5) * dt)
+        o._step(n)
+    L.emf_calc_complex_fields_2T(h, 0, n_cap * dt, n_end * dt)
+    o.lib.FDTD_calc_complex_fields_2T(o.h, n_cap * dt, n_end * dt)
+    ref = o.phasors()
+    errs = []
+    for q, c in enumerate(("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")):
+        g = np.zeros((case["Nz"], case["Ny"], case["Nx"]), dtype=np.complex128)
+        L.emf_get_field(h, 0, FIELD_INDEX[c], 0, case["Nx"], 0, case["Ny"], 0, case["Nz"], ptr(g))
+        errs.append(rel_l2(g, ref[q]))
+    print("2T rel L2 Ex..Hz:", " ".join("%.2e" % e for e in errs))
+    assert max(errs) < 1e-9, errs
+
+
 @pytest.mark.parametrize("bc", ["000", "0H0", "E00", "HE0", "EEH"])
 @pytest.mark.parametrize("ceps", [False, True])
 def test_step_level(bc, ceps):
