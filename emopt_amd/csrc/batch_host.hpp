@@ -328,6 +328,9 @@ int emf_batch_add(emf_batch *b, emf_solver *s, int slot, double rtol, double nma
     CHECK(check_ready(s));
     if (s->ceps || !s->mu_uniform)
         EMF_FAIL(EMF_ERR_ARG, "batched solves need real eps and uniform mu");
+    if (s->wide)
+        EMF_FAIL(EMF_ERR_ARG, "a batch member takes at most %llu cells (32-bit offsets)",
+                 ((1ULL << 32) - 1) / 24ULL);
     const long long n_stop = nmax < 0 ? 0 : (long long)floor(std::min(nmax, 1e15)) + 1;
     const long long nlast = fixed_n0 >= 0 ? fixed_n0 : n_stop;
     const long long nfac = nlast + 2 * Tn + 1;

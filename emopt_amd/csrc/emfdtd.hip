@@ -198,6 +198,8 @@ struct emf_solver {
     int kind = 1;                  // 0, 2 = fused step kernels; 1, 3 = separate H / E passes
     int tyg = 4;                   // waves (rows) per work-group
     int tyg_h = 0, zc_h = 0;       // the H pass's rows and planes per work-group (0: as tyg, zc)
+    bool wide = false;             // the passes address cells by 64-bit offsets (set by check_ready)
+    int wide_force = 0;            // 1: 64-bit offsets at any size (emf_set_wide_offsets)
 
     ConvState *conv = nullptr;
     long long *samples = nullptr;
@@ -599,12 +601,19 @@ int check_ready(emf_solver *s)
         EMF_FAIL(EMF_ERR_ARG, "opposite pml slabs overlap");
     if (s->R <= 0 || s->dx <= 0 || s->dy <= 0 || s->dz <= 0)
         EMF_FAIL(EMF_ERR_STATE, "wavelength / physical dims not set");
-    // the step kernels address every array by a 32-bit byte offset
-    if (3ULL * (unsigned long long)s->n * 8ULL >= (1ULL << 32) ||
-        (unsigned long long)s->psi_tot * 8ULL >= (1ULL << 32))
+    // The step passes address a cell of a field by a byte offset from the
+    // field's base: 32-bit while three components fit in 4 GiB (178 M cells),
+    // 64-bit above.  The fused step kernels have only the 32-bit form; the
+    // psi slabs always use 32-bit offsets.
+    const bool over = 3ULL * (unsigned long long)s->n * 8ULL >= (1ULL << 32);
+    if ((unsigned long long)s->psi_tot * 8ULL >= (1ULL << 32))
+        EMF_FAIL(EMF_ERR_ARG, "CPML slabs of %lld cells exceed the kernels' 32-bit offset range",
+                 (long long)s->psi_tot);
+    if ((over || s->wide_force) && (s->kind == 0 || s->kind == 2))
         EMF_FAIL(EMF_ERR_ARG,
-                 "grid of %lld cells exceeds the kernels' 32-bit offset range (at most %llu cells)",
-                 s->n, ((1ULL << 32) - 1) / 24ULL);
+                 "the fused step kernels take at most %llu cells and no 64-bit offsets; use the "
+                 "two passes (emf_set_tuning kind 1)", ((1ULL << 32) - 1) / 24ULL);
+    s->wide = over || s->wide_force;
     CHECK(refresh_coeffs(s));
     return EMF_OK;
 }
@@ -643,6 +652,7 @@ FusedArgs make_args(const emf_solver *s, int from, int to)
     A.pho = s->psiHv[from]; A.phn = s->psiHv[to]; A.pe = s->psiEv;
     A.tab = s->tab;
     A.cs = (unsigned)(s->n * 8);
+    A.csl = (unsigned long long)s->n * 8ULL;
     A.m_s = s->dt / s->mu_re;
     A.mu_s = s->mu_re;
     const bool comp = s->cidx_bytes != 0 && s->compress && s->cidx_valid && !s->ceps;
@@ -733,12 +743,43 @@ void launch_e_occ(const emf_solver *s, const FusedArgs &A, const Geo &g, const P
     else launch_e_zu<CM, OCC, 0, NT>(s, A, g, ps, done);
 }
 
+// the passes with 64-bit cell offsets (grids above 178 M cells): 64 x 4
+// work-groups, 4 waves per SIMD, the E pass's column unrolled as for 32 bits
+template <bool MU, bool NT>
+void launch_h_wide(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
+                   const int *done)
+{
+    const dim3 gr((s->Nx + PX - 1) / PX, (s->Ny + 3) / 4, (s->Nz + g.zc - 1) / g.zc);
+    PASS_LAUNCH((k_pass_h<MU, 4, 4, NT, true>), gr, dim3(PX, 4), A, g, ps, done);
+}
+
+template <int CM, int ZU, bool NT>
+void launch_e_wide_zu(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
+                      const int *done)
+{
+    const dim3 gr((s->Nx + PX - 1) / PX, (s->Ny + 3) / 4, (s->Nz + s->zc - 1) / s->zc);
+    PASS_LAUNCH((k_pass_e<CM, 4, 4, ZU, NT, true>), gr, dim3(PX, 4), A, g, ps, done);
+}
+
+template <int CM, bool NT>
+void launch_e_wide(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
+                   const int *done)
+{
+    if (CM == CM_INDEX && s->zc <= 2) launch_e_wide_zu<CM, 2, NT>(s, A, g, ps, done);
+    else if (CM == CM_INDEX && s->zc <= 4) launch_e_wide_zu<CM, 4, NT>(s, A, g, ps, done);
+    else if (CM == CM_INDEX && s->zc <= 8) launch_e_wide_zu<CM, 8, NT>(s, A, g, ps, done);
+    else launch_e_wide_zu<CM, 0, NT>(s, A, g, ps, done);
+}
+
 // kind 3 = the two passes compiled for 5 waves per SIMD
 template <bool MU>
 void launch_h_t(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
                 const int *done)
 {
-    if (s->kind == 3) launch_h_occ<MU, 5, false>(s, A, g, ps, done);
+    if (s->wide) {
+        if (stream_psi(s)) launch_h_wide<MU, true>(s, A, g, ps, done);
+        else launch_h_wide<MU, false>(s, A, g, ps, done);
+    } else if (s->kind == 3) launch_h_occ<MU, 5, false>(s, A, g, ps, done);
     else if (stream_psi(s)) launch_h_occ<MU, 4, true>(s, A, g, ps, done);
     else launch_h_occ<MU, 4, false>(s, A, g, ps, done);
 }
@@ -747,7 +788,10 @@ template <int CM>
 void launch_e_t(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
                 const int *done)
 {
-    if (s->kind == 3) launch_e_occ<CM, 5, false>(s, A, g, ps, done);
+    if (s->wide) {
+        if (stream_psi(s)) launch_e_wide<CM, true>(s, A, g, ps, done);
+        else launch_e_wide<CM, false>(s, A, g, ps, done);
+    } else if (s->kind == 3) launch_e_occ<CM, 5, false>(s, A, g, ps, done);
     else if (stream_psi(s)) launch_e_occ<CM, 4, true>(s, A, g, ps, done);
     else launch_e_occ<CM, 4, false>(s, A, g, ps, done);
 }
@@ -2329,6 +2373,14 @@ int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc, int order)
     s->tyg = tyg;
     s->zc = zc;
     s->order = order;
+    return EMF_OK;
+}
+
+int emf_set_wide_offsets(emf_solver *s, int force)
+{
+    LOCK(s);
+    if (force != 0 && force != 1) EMF_FAIL(EMF_ERR_ARG, "force must be 0 or 1");
+    s->wide_force = force;
     return EMF_OK;
 }
 

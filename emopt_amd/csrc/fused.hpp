@@ -40,7 +40,8 @@ struct FusedArgs {
     double *phn;             // H-pass psi, new set
     double *pe;              // E-pass psi (in place)
     const double *tab;       // CPML tables
-    unsigned cs;             // byte stride between components
+    unsigned cs;             // byte stride between components (32-bit offset kernels)
+    unsigned long long csl;  // the same stride for the 64-bit offset kernels
     double m_s, mu_s;        // uniform dt/mu.re and mu.re
     const void *bI;          // real eps only, optional: per-cell index into bT
     const double *bT;        // (b_x, b_y, b_z, 0) per distinct eps triple
@@ -51,14 +52,28 @@ struct FusedArgs {
 // (a and b arrays), real eps with b gathered through a per-cell index.
 enum { CM_REAL = 0, CM_COMPLEX = 1, CM_INDEX = 2 };
 
+// Byte offsets of cells within a field component are 32-bit (OT =
+// unsigned) while 3 components of the grid fit in 4 GiB (178 M cells), and
+// 64-bit (OT = unsigned long long) above; the two forms of a kernel differ
+// only in that arithmetic.  cstride gives the component stride in OT.
+template <typename OT>
+__device__ __forceinline__ OT cstride(const FusedArgs &A)
+{
+    if constexpr (sizeof(OT) == 8) return (OT)A.csl;
+    else return (OT)A.cs;
+}
+
 // Entry of the per-cell coefficient index at byte offset o of a field
 // component: one aligned dword holds the 2- or 4-byte entry (the 2-byte index
 // is allocated with two spare entries for the last dword).
-__device__ __forceinline__ unsigned coef_index(const FusedArgs &A, unsigned o)
+template <typename OT = unsigned>
+__device__ __forceinline__ unsigned coef_index(const FusedArgs &A, OT o)
 {
-    const unsigned e = o >> 3, w2 = A.bIw == 2;
-    const unsigned v = *gptr((const unsigned *)((const char *)A.bI + (w2 ? (e << 1) & ~3u : e << 2)));
-    return w2 ? (v >> ((e & 1u) << 4)) & 0xffffu : v;
+    const OT e = o >> 3;
+    const unsigned w2 = A.bIw == 2;
+    const unsigned v = *gptr((const unsigned *)((const char *)A.bI +
+                                                (w2 ? (e << 1) & ~(OT)3 : e << 2)));
+    return w2 ? (v >> (((unsigned)e & 1u) << 4)) & 0xffffu : v;
 }
 
 // (b_x, b_y, b_z) of table entry ci: 32-byte entries, one 16-byte and one
@@ -126,6 +141,11 @@ __device__ __forceinline__ double ldo(const double *base, unsigned off)
     return *(const gdouble *)(reinterpret_cast<const char *>(base) + off);
 }
 
+__device__ __forceinline__ double ldo(const double *base, unsigned long long off)
+{
+    return *(const gdouble *)(reinterpret_cast<const char *>(base) + off);
+}
+
 // Load from data that no kernel writes while a step runs (the CPML tables),
 // through the constant address space: where the offset is the same for the
 // whole wave the backend then loads it with a scalar load, which keeps it off
@@ -152,6 +172,11 @@ __device__ __forceinline__ void stn(double *base, unsigned off, double v)
 }
 
 __device__ __forceinline__ void sto(double *base, unsigned off, double v)
+{
+    *(gdouble *)(reinterpret_cast<char *>(base) + off) = v;
+}
+
+__device__ __forceinline__ void sto(double *base, unsigned long long off, double v)
 {
     *(gdouble *)(reinterpret_cast<char *>(base) + off) = v;
 }
@@ -229,13 +254,14 @@ struct HIn {
     Slab sx, sy, sz;
 };
 
-template <bool MU_ARRAY, bool NT = false>
+template <bool MU_ARRAY, bool NT = false, typename OT = unsigned>
 __device__ __forceinline__ void h_load_aux(const Geo &g, const Layout &L, const FusedArgs &A,
-                                           int i, int j, int k, unsigned o, HIn &d)
+                                           int i, int j, int k, OT o, HIn &d)
 {
+    const OT cs = cstride<OT>(A);
     d.mx = MU_ARRAY ? ldo(A.mH, o) : A.m_s;
-    d.my = MU_ARRAY ? ldo(A.mH, o + A.cs) : A.m_s;
-    d.mz = MU_ARRAY ? ldo(A.mH, o + 2 * A.cs) : A.m_s;
+    d.my = MU_ARRAY ? ldo(A.mH, o + cs) : A.m_s;
+    d.mz = MU_ARRAY ? ldo(A.mH, o + 2 * cs) : A.m_s;
     d.sx = slab_x(g, i, j, k);
     d.sy = slab_y(g, i, j, k);
     d.sz = slab_z(g, i, j, k);
@@ -295,19 +321,20 @@ __device__ __forceinline__ void h_compute(const Geo &g, const HIn &d, double &hx
 }
 
 // magnetic sources of src/fdtd.cpp:313-366 at one cell
-template <bool MU_ARRAY>
+template <bool MU_ARRAY, typename OT = unsigned>
 __device__ __forceinline__ void h_sources(const Geo &g, const PassSrc &sh, const FusedArgs &A,
-                                          int i, int j, int k, unsigned o, double &hx,
+                                          int i, int j, int k, OT o, double &hx,
                                           double &hy, double &hz)
 {
+    const OT cs = cstride<OT>(A);
     for (int q = 0; q < sh.n; ++q) {
         long long m;
         const SrcBlk sb = src_blk(sh.blk, q);
         if (in_box(sb, i, j, k, m)) {
             const double2 vx = ld2(sb.f[3] + m), vy = ld2(sb.f[4] + m), vz = ld2(sb.f[5] + m);
             const double ux = MU_ARRAY ? ldo(A.muR, o) : A.mu_s;
-            const double uy = MU_ARRAY ? ldo(A.muR, o + A.cs) : A.mu_s;
-            const double uz = MU_ARRAY ? ldo(A.muR, o + 2 * A.cs) : A.mu_s;
+            const double uy = MU_ARRAY ? ldo(A.muR, o + cs) : A.mu_s;
+            const double uz = MU_ARRAY ? ldo(A.muR, o + 2 * cs) : A.mu_s;
             hx = hx + src_val(sh, vx) * g.dt / ux;
             hy = hy + src_val(sh, vy) * g.dt / uy;
             hz = hz + src_val(sh, vz) * g.dt / uz;

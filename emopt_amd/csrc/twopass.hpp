@@ -17,6 +17,8 @@
 // adjacent lane (the wave's edge lane loads it); row and plane neighbours
 // come through L1/L2.
 #pragma once
+#include <type_traits>
+
 #include "fused.hpp"
 
 namespace emf {
@@ -59,15 +61,15 @@ __device__ __forceinline__ void pass_tile(const Geo &g, int ty, int &k, int &j, 
 // the pre-update E and H (the state after the steps taken so far) are also
 // stored as a snapshot: component c of cell g goes to cap[2 * (c * n + g)]
 // (cap points at the .x or .y half of a six-component result block).
-template <bool MU_ARRAY, bool NT = false>
+template <bool MU_ARRAY, bool NT = false, typename OT = unsigned>
 __device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const PassSrc &sh,
                                          int k, int j, int z0, int z1, double *cap,
                                          long long capn)
 {
     const Layout L = make_layout(g);
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
-    const unsigned cs = A.cs, cs2 = 2 * A.cs;
-    const unsigned oz = (unsigned)g.nxy * 8u, oy = (unsigned)Nx * 8u;
+    const OT cs = cstride<OT>(A), cs2 = 2 * cstride<OT>(A);
+    const OT oz = (OT)g.nxy * 8u, oy = (OT)Nx * 8u;
     const bool jn = j + 1 < Ny, kn = k + 1 < Nx;
     bool col_src = false;   // does this column meet a source block?
     for (int q = 0; q < sh.n; ++q) {
@@ -75,7 +77,7 @@ __device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const
         if (k >= s.k0 && k < s.k0 + s.K && j >= s.j0 && j < s.j0 + s.J && s.i0 < z1 && s.i0 + s.I > z0)
             col_src = true;
     }
-    unsigned o = ((unsigned)z0 * (unsigned)Ny + (unsigned)j) * (unsigned)Nx * 8u + (unsigned)k * 8u;
+    OT o = ((OT)z0 * (OT)Ny + (OT)j) * (OT)Nx * 8u + (OT)k * 8u;
     double ex = ldo(A.Eo, o), ey = ldo(A.Eo, o + cs);
     for (int i = z0; i < z1; ++i, o += oz) {
         HIn d;
@@ -88,7 +90,7 @@ __device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const
         // E(k+1) comes from the next lane; the last lane loads it
         double ezk = 0, eyk = 0;
         if (threadIdx.x == PX - 1 && kn) { ezk = ldo(A.Eo, o + 8u + cs2); eyk = ldo(A.Eo, o + 8u + cs); }
-        h_load_aux<MU_ARRAY, NT>(g, L, A, i, j, k, o, d);
+        h_load_aux<MU_ARRAY, NT, OT>(g, L, A, i, j, k, o, d);
         d.ezk = __shfl_down(d.ez, 1);
         d.eyk = __shfl_down(d.ey, 1);
         if (threadIdx.x == PX - 1) { d.ezk = ezk; d.eyk = eyk; }
@@ -101,7 +103,7 @@ __device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const
         }
         double hx, hy, hz, np[6];
         h_compute(g, d, hx, hy, hz, np);
-        if (col_src) h_sources<MU_ARRAY>(g, sh, A, i, j, k, o, hx, hy, hz);
+        if (col_src) h_sources<MU_ARRAY, OT>(g, sh, A, i, j, k, o, hx, hy, hz);
         sto(A.Hn, o, hx); sto(A.Hn, o + cs, hy); sto(A.Hn, o + cs2, hz);
         if (d.sx.in) { stp<NT>(A.phn, psof(L, 0, d.sx.lo, 0) + d.sx.io, np[0]); stp<NT>(A.phn, psof(L, 0, d.sx.lo, 1) + d.sx.io, np[1]); }
         if (d.sy.in) { stp<NT>(A.phn, psof(L, 1, d.sy.lo, 0) + d.sy.io, np[2]); stp<NT>(A.phn, psof(L, 1, d.sy.lo, 1) + d.sy.io, np[3]); }
@@ -112,8 +114,9 @@ __device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const
 
 // OCC = minimum waves per SIMD the register allocation must allow (4 fits
 // the passes without spilling; 5 trades a few spilled registers for 25 %
-// more waves in flight).  NT: streaming psi access (see ldn).
-template <bool MU_ARRAY, int TY, int OCC = 4, bool NT = false>
+// more waves in flight).  NT: streaming psi access (see ldn).  BIG: 64-bit
+// cell offsets (grids above 178 M cells).
+template <bool MU_ARRAY, int TY, int OCC = 4, bool NT = false, bool BIG = false>
 __global__ __launch_bounds__(PX * TY, OCC) void k_pass_h(FusedArgs A, Geo g, PassSrc sh,
                                                         const int *__restrict__ done)
 {
@@ -122,7 +125,8 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_h(FusedArgs A, Geo g, Pas
     pass_tile(g, TY, k, j, z0, z1);
     j = __builtin_amdgcn_readfirstlane(j);   // one row per wave
     if (k >= g.Nx || j >= g.Ny) return;
-    h_column<MU_ARRAY, NT>(A, g, sh, k, j, z0, z1, nullptr, 0);
+    using OT = typename std::conditional<BIG, unsigned long long, unsigned>::type;
+    h_column<MU_ARRAY, NT, OT>(A, g, sh, k, j, z0, z1, nullptr, 0);
 }
 
 // ------------------------------------------------------------------ E pass
@@ -130,13 +134,13 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_h(FusedArgs A, Geo g, Pas
 // advanced to plane i.  With CM_INDEX, ci is the cell's coefficient index,
 // loaded by the caller ahead of the plane's other loads: the table gather is
 // issued last, after every other load of the plane (psi included).
-template <int CM, bool NT = false>
+template <int CM, bool NT = false, typename OT = unsigned>
 __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const PassSrc &se,
-                                        const Layout &L, int k, int j, int i, unsigned o,
+                                        const Layout &L, int k, int j, int i, OT o,
                                         unsigned ci, bool col_src, double &hxm, double &hym)
 {
-    const unsigned cs = A.cs, cs2 = 2 * A.cs;
-    const unsigned oy = (unsigned)g.Nx * 8u;
+    const OT cs = cstride<OT>(A), cs2 = 2 * cstride<OT>(A);
+    const OT oy = (OT)g.Nx * 8u;
     // min-face ghost rule: '0' -> 0, 'E' -> -H(first), 'H' -> +H(first)
     const double sgx = g.bc[0] == 'E' ? -1.0 : (g.bc[0] == 'H' ? 1.0 : 0.0);
     const double sgy = g.bc[1] == 'E' ? -1.0 : (g.bc[1] == 'H' ? 1.0 : 0.0);
@@ -147,7 +151,7 @@ __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const 
     // row j-1; at j == 0 the own cell stands in (an unconditional load keeps
     // the count of loads behind the index fixed) and the ghost rule below
     // replaces it
-    const unsigned dj = j > 0 ? oy : 0u;
+    const OT dj = j > 0 ? oy : (OT)0;
     double hz_j = ldo(A.Ho, o - dj + cs2), hx_j = ldo(A.Ho, o - dj);
     const double ex = ldo(A.Eo, o), ey = ldo(A.Eo, o + cs), ez = ldo(A.Eo, o + cs2);
     double bx = 0, by = 0, bz = 0;
@@ -243,13 +247,13 @@ __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const 
 // planes are loaded first, ahead of everything else -- a plane's gather then
 // never waits for its own index.  ZU = 0: any number of planes, each plane
 // loading its index first.
-template <int CM, int ZU, bool NT = false>
+template <int CM, int ZU, bool NT = false, typename OT = unsigned>
 __device__ __forceinline__ void e_column(const FusedArgs &A, const Geo &g, const PassSrc &se,
                                          int k, int j, int z0, int z1)
 {
     const Layout L = make_layout(g);
-    const unsigned cs = A.cs;
-    const unsigned oz = (unsigned)g.nxy * 8u;
+    const OT cs = cstride<OT>(A);
+    const OT oz = (OT)g.nxy * 8u;
     const double sgz = g.bc[2] == 'E' ? -1.0 : (g.bc[2] == 'H' ? 1.0 : 0.0);
     const bool bz0 = g.bc[2] == '0';
     bool col_src = false;
@@ -258,34 +262,35 @@ __device__ __forceinline__ void e_column(const FusedArgs &A, const Geo &g, const
         if (k >= s.k0 && k < s.k0 + s.K && j >= s.j0 && j < s.j0 + s.J && s.i0 < z1 && s.i0 + s.I > z0)
             col_src = true;
     }
-    unsigned o = ((unsigned)z0 * (unsigned)g.Ny + (unsigned)j) * (unsigned)g.Nx * 8u + (unsigned)k * 8u;
+    OT o = ((OT)z0 * (OT)g.Ny + (OT)j) * (OT)g.Nx * 8u + (OT)k * 8u;
     if (ZU > 0) {
         unsigned ci[ZU > 0 ? ZU : 1];
 #pragma unroll
         for (int p = 0; p < ZU; ++p)   // a missing plane re-reads the last one
-            ci[p] = CM == CM_INDEX ? coef_index(A, o + (unsigned)min(p, z1 - z0 - 1) * oz) : 0u;
+            ci[p] = CM == CM_INDEX ? coef_index<OT>(A, o + (OT)min(p, z1 - z0 - 1) * oz) : 0u;
         double hxm, hym;   // H at i-1
         if (z0 > 0) { hxm = ldo(A.Ho, o - oz); hym = ldo(A.Ho, o - oz + cs); }
         else { hxm = bz0 ? 0.0 : sgz * ldo(A.Ho, o); hym = bz0 ? 0.0 : sgz * ldo(A.Ho, o + cs); }
 #pragma unroll
         for (int p = 0; p < ZU; ++p) {
             if (z0 + p >= z1) break;
-            e_plane<CM, NT>(A, g, se, L, k, j, z0 + p, o + (unsigned)p * oz, ci[p], col_src, hxm,
-                            hym);
+            e_plane<CM, NT, OT>(A, g, se, L, k, j, z0 + p, o + (OT)p * oz, ci[p], col_src, hxm,
+                                hym);
         }
     } else {
         double hxm, hym;
         if (z0 > 0) { hxm = ldo(A.Ho, o - oz); hym = ldo(A.Ho, o - oz + cs); }
         else { hxm = bz0 ? 0.0 : sgz * ldo(A.Ho, o); hym = bz0 ? 0.0 : sgz * ldo(A.Ho, o + cs); }
         for (int i = z0; i < z1; ++i, o += oz) {
-            const unsigned ci = CM == CM_INDEX ? coef_index(A, o) : 0u;
-            e_plane<CM, NT>(A, g, se, L, k, j, i, o, ci, col_src, hxm, hym);
+            const unsigned ci = CM == CM_INDEX ? coef_index<OT>(A, o) : 0u;
+            e_plane<CM, NT, OT>(A, g, se, L, k, j, i, o, ci, col_src, hxm, hym);
         }
     }
 }
 
-// ZU: see e_column (the launch picks the smallest ZU of 2, 4, 8 that holds zc)
-template <int CM, int TY, int OCC = 4, int ZU = 0, bool NT = false>
+// ZU: see e_column (the launch picks the smallest ZU of 2, 4, 8 that holds
+// zc).  BIG: as for k_pass_h.
+template <int CM, int TY, int OCC = 4, int ZU = 0, bool NT = false, bool BIG = false>
 __global__ __launch_bounds__(PX * TY, OCC) void k_pass_e(FusedArgs A, Geo g, PassSrc se,
                                                         const int *__restrict__ done)
 {
@@ -294,7 +299,8 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_e(FusedArgs A, Geo g, Pas
     pass_tile(g, TY, k, j, z0, z1, true);
     j = __builtin_amdgcn_readfirstlane(j);   // one row per wave
     if (k >= g.Nx || j >= g.Ny) return;
-    e_column<CM, ZU, NT>(A, g, se, k, j, z0, z1);
+    using OT = typename std::conditional<BIG, unsigned long long, unsigned>::type;
+    e_column<CM, ZU, NT, OT>(A, g, se, k, j, z0, z1);
 }
 
 }  // namespace emf

@@ -352,6 +352,13 @@ EMF_API int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64
  * identical; only speed differs. */
 EMF_API int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc, int order);
 
+/* force = 1: the H and E passes address cells by 64-bit byte offsets at any
+ * grid size (the same arithmetic; for tests).  Grids of more than 178 M cells
+ * (three components above 4 GiB) use them regardless, with the two passes'
+ * default shape (64 x 4 work-groups); the fused kernels (kinds 0, 2) and
+ * batch members take only smaller grids.  No counterpart in the reference. */
+EMF_API int emf_set_wide_offsets(emf_solver *s, int force);
+
 /* Work-group shape of the H pass of the two-pass kernels (kinds 1 and 3):
  * rows (waves) per work-group tyg in {1, 2, 4, 8} and z planes per work-group
  * zc >= 1; 0 keeps the E pass's value (emf_set_tuning).  A launch shape only:

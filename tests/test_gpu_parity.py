@@ -188,7 +188,7 @@ def test_gpu_matches_reference_golden_solve():
     assert max(errs) <= TOL, errs
 
 
-def _run_steps(case, nsteps, kind, tyg=4, zc=8, order=0, compress=1, stream=-1):
+def _run_steps(case, nsteps, kind, tyg=4, zc=8, order=0, compress=1, stream=-1, wide=0):
     """nsteps time steps through emf_time_steps with the given step kernel."""
     import ctypes
     import warnings
@@ -207,6 +207,7 @@ def _run_steps(case, nsteps, kind, tyg=4, zc=8, order=0, compress=1, stream=-1):
     L.emf_set_tuning(h, kind, tyg, zc, order)
     L.emf_set_compression(h, compress)
     L.emf_set_streaming(h, stream)
+    L.emf_set_wide_offsets(h, wide)
     L.emf_reset_pml(h)
     L.emf_zero_fields(h)
     out = np.zeros(3)
@@ -327,3 +328,24 @@ def test_h_pass_shape_is_launch_shape_only():
         got = sim.time_fields()
         for a, b in zip(got, ref):
             assert np.array_equal(a, b), (tyg, zc)
+
+
+@pytest.mark.parametrize("variant", ["index4", "ceps", "mu", "bcs", "index2"])
+def test_wide_offsets_are_exact(variant):
+    """The passes with 64-bit cell offsets (which grids above 178 M cells
+    use) give exactly the fields of the 32-bit ones: every coefficient mode
+    (4- and 2-byte compressed index, complex eps), non-uniform mu, symmetric
+    boundaries, streamed and cached psi, unrolled (2 planes) and looped (5)
+    E-pass columns."""
+    if variant == "index2":
+        case = strip_case(40, 28, 22, w=5)
+    else:
+        kw = dict(w=(5, 4, 3, 6, 4, 5), seed=37, nsrc=2, mu_rand=variant == "mu",
+                  ceps=variant == "ceps", bc="EH0" if variant == "bcs" else "000")
+        case = random_case(70, 37, 23, **kw)
+    for zc in (2, 5):
+        for stream in (0, 1):
+            a = _run_steps(case, 29, kind=1, tyg=4, zc=zc, stream=stream, wide=1)
+            b = _run_steps(case, 29, kind=1, tyg=4, zc=zc, stream=stream, wide=0)
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y), (variant, zc, stream)
