@@ -554,21 +554,40 @@ class AdjointMethodPNF3D(AdjointMethod):
     def calc_dfdx_device(self, sim, params):
         """Optional device form of calc_dfdx: per FOM domain, (weight_set,
         scale, mask) with df/dfield_q = scale_q * W_q for weight planes W on
-        the solver (ModeMatch.dFdx_device).  None: host path only."""
+        the solver (ModeMatch.dFdx_device).  None: calc_dfdx's planes are
+        uploaded instead."""
         return None
 
     def get_fom_domains(self):
         raise NotImplementedError
 
+    # weight set that carries calc_dfdx's host planes to the device
+    HOST_PLANES_SET = 7
+
     def stage_dFdx(self, sim, params):
-        """calc_dFdx's sources built on the GPU (power_norm_dFdx_3D_device)
-        when calc_dfdx_device gives the planes."""
+        """calc_dFdx's sources built on the GPU (power_norm_dFdx_3D_device).
+        The derivative planes come from calc_dfdx_device when the subclass
+        has it; otherwise calc_dfdx's host planes are uploaded (weight set
+        HOST_PLANES_SET) with unit factors, which gives the same values, and
+        the face terms and all transpositions still run on the device.
+        False (the host path) when a plane does not have its domain's shape."""
         specs = self.calc_dfdx_device(sim, params)
-        if specs is None:
-            return False
         f = self.calc_f(sim, params)
+        doms = self.get_fom_domains()
+        host = None
+        if specs is None:
+            host = self.calc_dfdx(sim, params)
+            for dfdx, dom in zip(host, doms):
+                n = int(np.prod(dom.shape))
+                if any(np.size(a) != n for a in dfdx):
+                    return False
         sim.stage_clear()
-        for spec, dom in zip(specs, self.get_fom_domains()):
+        for q, dom in enumerate(doms):
+            if host is not None:
+                sim.set_plane_weights(self.HOST_PLANES_SET, host[q])
+                spec = (self.HOST_PLANES_SET, (1.0,) * 6, 0b111111)
+            else:
+                spec = specs[q]
             if not power_norm_dFdx_3D_device(sim, f, dom, *spec):
                 sim.stage_clear()
                 return False

@@ -118,7 +118,9 @@ def test_staged_power_norm_equals_host(bc):
 
 def test_staged_sources_drive_the_adjoint_solve():
     """The grating's gradient with adjoint sources staged on the GPU equals
-    the one with host-built sources (same adjoint fields)."""
+    the one with host-built sources (same adjoint fields): with the FOM's
+    planes formed on the device (calc_dfdx_device) and with calc_dfdx's host
+    planes uploaded (a FOM without the device hook)."""
     from tests.cases import grating_problem
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
@@ -132,8 +134,13 @@ def test_staged_sources_drive_the_adjoint_solve():
             am.device_sources = dev
             gs[dev] = am.gradient_device(p, fom_done=True)
             assert sim.last_solve["status"] == "fixed"
-    assert sim.stage_count() == 7          # FOM plane + six faces
-    err = np.abs(gs[True] - gs[False]).max() / np.abs(gs[False]).max()
-    print("device vs host adjoint sources: max gradient difference %.2e (bit-identical: %s)"
-          % (err, np.array_equal(gs[True], gs[False])))
-    assert err <= 1e-12
+        assert sim.stage_count() == 7          # FOM plane + six faces
+        am.device_sources = True
+        am.calc_dfdx_device = lambda sim, params: None
+        gs["upload"] = am.gradient_device(p, fom_done=True)
+        assert sim.stage_count() == 7
+    for key in (True, "upload"):
+        err = np.abs(gs[key] - gs[False]).max() / np.abs(gs[False]).max()
+        print("device (%s) vs host adjoint sources: max gradient difference %.2e "
+              "(bit-identical: %s)" % (key, err, np.array_equal(gs[key], gs[False])))
+        assert err <= 1e-12
