@@ -322,6 +322,8 @@ def bench_serving(rates, n_requests, seed=5, budget=200e9, max_running=8):
         build_s += t1 - t0
         steps += sim.last_solve["steps"] * c["Nx"] * c["Ny"] * c["Nz"]
         del sim
+        if (q + 1) % 32 == 0:
+            log("[bench] serving: single-solve service, %d of %d requests" % (q + 1, n_requests))
     log("[bench] serving: single-solve service of %d requests %.1fs" % (n_requests, sum(service)))
     out = {"requests": n_requests, "budget_bytes": budget, "max_running": max_running,
            "plain_busy_seconds": float(sum(service)), "plain_build_seconds": build_s,
@@ -330,7 +332,14 @@ def bench_serving(rates, n_requests, seed=5, budget=200e9, max_running=8):
         rng = np.random.default_rng([seed, k])
         arr = list(np.cumsum(rng.exponential(1.0 / rate, size=n_requests)))
         reqs = [serve_request(q, seed=seed) for q in range(n_requests)]
-        srv = Server(hbm_budget=budget, prep_workers=8, max_running=max_running)
+        ndone = [0]
+
+        def progress(req, sim):
+            ndone[0] += 1
+            if ndone[0] % 32 == 0:
+                log("[bench] serving %g/s: %d of %d requests done" % (rate, ndone[0], n_requests))
+
+        srv = Server(hbm_budget=budget, prep_workers=8, max_running=max_running, on_done=progress)
         with warnings.catch_warnings():
             warnings.simplefilter("ignore")
             st = srv.run_trace(list(zip(arr, reqs)))

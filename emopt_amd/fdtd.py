@@ -599,6 +599,24 @@ class FDTD(MaxwellSolver):
         from .fields import source_power
         return source_power(self)
 
+    def src_func_t(self, n, t, phase=0.0):
+        """The source time function at step n, time t (FDTD_src_func_t,
+        src/fdtd.cpp:1595-1602): the ramp envelope times sin(t + phase)."""
+        return self._lib.emf_src_func_t(self._h, int(n), float(t), float(phase))
+
+    def test_src_func(self):
+        """Plot the source time function over 3000 steps (emopt/fdtd.py:1720-1730)."""
+        time = np.arange(0, 3000, 1) * self._dt
+        ramp = np.array([self.src_func_t(i, time[i], 0.0) for i in range(len(time))])
+        try:
+            import matplotlib.pyplot as plt
+        except ImportError:
+            warnings.warn("matplotlib is not available: no plot")
+            return ramp
+        plt.plot(time, ramp)
+        plt.show()
+        return ramp
+
     def get_A_diag(self):
         """Copies of (eps_x, eps_y, eps_z, mu_x, mu_y, mu_z) (emopt/fdtd.py:1669-1692)."""
         n = self._Nx * self._Ny * self._Nz

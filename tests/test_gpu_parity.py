@@ -60,6 +60,34 @@ def _step_oracle(case, nsteps):
     return [o.padded(q)[1:-1, 1:-1, 1:-1] for q in range(6)]
 
 
+def test_src_func_t_matches_oracle():
+    """FDTD.src_func_t (emf_src_func_t) against the oracle's FDTD_src_func_t
+    with the same ramp, through and past the ramp, at several phases;
+    test_src_func returns the same samples."""
+    import warnings
+    from emopt_amd.fdtd import FDTD
+    from oracle import driver as D
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim = FDTD(1.0, 0.8, 0.6, 0.05, 0.05, 0.05, 1.31)
+        o = D.OracleFDTD(D.load(D.ORACLE_LIB), 1.0, 0.8, 0.6, 0.05, 0.05, 0.05, 1.31)
+    for T in (None, 7.5):
+        if T is not None:
+            sim.src_ramp_time = T
+            o.src_ramp_time = T
+        for n in (0, 1, 37, 400, 2999):
+            for ph in (0.0, 0.7, -2.1):
+                t = n * sim.dt
+                a, b = sim.src_func_t(n, t, ph), o.lib.FDTD_src_func_t(o.h, n, t, ph)
+                assert abs(a - b) <= 1e-15 * max(1.0, abs(b)), (T, n, ph, a, b)
+    import importlib.util
+    if importlib.util.find_spec("matplotlib") is None:   # otherwise it would plot
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            ramp = sim.test_src_func()
+        assert ramp.shape == (3000,) and ramp[2999] == sim.src_func_t(2999, 2999 * sim.dt, 0.0)
+
+
 def test_two_sample_phasors_match_oracle():
     """emf_calc_complex_fields_2T (k_phasor2; the reference's
     FDTD::calc_complex_fields(t0, t1), src/fdtd.cpp:1346-1390) against the
