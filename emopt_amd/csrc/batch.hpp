@@ -37,6 +37,7 @@ struct BMember {                  // written by the host when the solve is admit
     Geo g;
     const SrcBlk *blk;
     int nsrc;
+    int sub[6];                   // union box of the source blocks (PassSrc::ub)
     int ntx, nty, ntz;            // tile grid of this solve
     const double *fac;            // [nfac][NFAC]
     long long nfac;
@@ -199,13 +200,16 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_h(const BTable *__restrict__
         if (c.mode == BM_IDLE || c.mode == BM_PHASOR) { t = c.next - 1; continue; }
         const BMember &B = M[c.m];
         int k, j, z0, z1;
-        if (!batch_column(B, t - T->off[c.e], k, j, z0, z1)) continue;
+        const bool on = batch_column(B, t - T->off[c.e], k, j, z0, z1);
         const unsigned fo = (unsigned)(c.step * NFAC) * 8u;   // step factors, read-only here
         PassSrc ps;
-        ps.n = B.nsrc; ps.comp0 = 3; ps.blk = B.blk;
+        ps.n = B.nsrc; ps.comp0 = 3; ps.blk = B.blk; ps.ninl = 0;
+        for (int a = 0; a < 6; ++a) ps.ub[a] = B.sub[a];
         ps.st = ldc(B.fac, fo + 0u); ps.ct = ldc(B.fac, fo + 8u); ps.env = ldc(B.fac, fo + 16u);
+        if (!on) continue;
+        const unsigned long long sm = col_src_mask(ps, k, j, z0, z1);
         double *cap = c.mode == BM_CAP0 ? B.res : (c.mode == BM_CAP1 ? B.res + 1 : nullptr);
-        h_column<false>(B.A, B.g, ps, k, j, z0, z1, cap, B.n);
+        h_column<false>(B.A, B.g, ps, k, j, z0, z1, cap, B.n, sm);
     }
 }
 
@@ -231,14 +235,17 @@ __global__ __launch_bounds__(PX * BTY) void k_batch_e(const BTable *__restrict__
         if (c.mode == BM_IDLE || c.mode == BM_PHASOR) { t = T->off[c.e]; continue; }
         const BMember &B = M[c.m];
         int k, j, z0, z1;
-        if (!batch_column(B, t - T->off[c.e], k, j, z0, z1)) continue;
+        const bool on = batch_column(B, t - T->off[c.e], k, j, z0, z1);
         const unsigned fo = (unsigned)(c.step * NFAC) * 8u;   // step factors, read-only here
         PassSrc ps;
-        ps.n = B.nsrc; ps.comp0 = 0; ps.blk = B.blk;
+        ps.n = B.nsrc; ps.comp0 = 0; ps.blk = B.blk; ps.ninl = 0;
+        for (int a = 0; a < 6; ++a) ps.ub[a] = B.sub[a];
         ps.st = ldc(B.fac, fo + 24u); ps.ct = ldc(B.fac, fo + 32u); ps.env = ldc(B.fac, fo + 40u);
-        if (B.A.bI && z1 - z0 <= 2) e_column<CM_INDEX, 2>(B.A, B.g, ps, k, j, z0, z1);
-        else if (B.A.bI) e_column<CM_INDEX, 0>(B.A, B.g, ps, k, j, z0, z1);
-        else e_column<CM_REAL, 0>(B.A, B.g, ps, k, j, z0, z1);
+        if (!on) continue;
+        const unsigned long long sm = col_src_mask(ps, k, j, z0, z1);
+        if (B.A.bI && z1 - z0 <= 2) e_column<CM_INDEX, 2>(B.A, B.g, ps, k, j, z0, z1, sm);
+        else if (B.A.bI) e_column<CM_INDEX, 0>(B.A, B.g, ps, k, j, z0, z1, sm);
+        else e_column<CM_REAL, 0>(B.A, B.g, ps, k, j, z0, z1, sm);
     }
 }
 

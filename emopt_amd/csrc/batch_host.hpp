@@ -381,6 +381,7 @@ int emf_batch_add(emf_batch *b, emf_solver *s, int slot, double rtol, double nma
     B.g.order = 0;
     B.blk = s->src_desc;
     B.nsrc = (int)s->srcs.size();
+    src_union(s, B.sub);
     B.ntx = (s->Nx + PX - 1) / PX;
     B.nty = (s->Ny + BTY - 1) / BTY;
     B.ntz = (s->Nz + b->zc - 1) / b->zc;

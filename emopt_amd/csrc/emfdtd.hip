@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <complex>
 #include <cstdarg>
@@ -626,15 +627,34 @@ double env_at(const emf_solver *s, double t)
     return 1.0;
 }
 
+// union box (i0, i1, j0, j1, k0, k1) of the solver's source blocks
+void src_union(const emf_solver *s, int ub[6])
+{
+    ub[0] = ub[2] = ub[4] = INT_MAX;
+    ub[1] = ub[3] = ub[5] = INT_MIN;
+    for (const auto &d : s->srcs) {
+        ub[0] = std::min(ub[0], d.i0); ub[1] = std::max(ub[1], d.i0 + d.I);
+        ub[2] = std::min(ub[2], d.j0); ub[3] = std::max(ub[3], d.j0 + d.J);
+        ub[4] = std::min(ub[4], d.k0); ub[5] = std::max(ub[5], d.k0 + d.K);
+    }
+}
+
 PassSrc pass_sources(const emf_solver *s, int comp0, double t)
 {
-    PassSrc ps;
+    PassSrc ps{};
     ps.n = (int)s->srcs.size();
     ps.comp0 = comp0;
     ps.st = sin(t);
     ps.ct = cos(t);
     ps.env = env_at(s, t);
     ps.blk = s->src_desc;
+    src_union(s, ps.ub);
+    ps.ninl = std::min(ps.n, SRC_INLINE);
+    for (int q = 0; q < ps.ninl; ++q) {
+        const SourceDev &d = s->srcs[q];
+        const int b[6] = { d.i0, d.I, d.j0, d.J, d.k0, d.K };
+        for (int a = 0; a < 6; ++a) ps.ib[q][a] = b[a];
+    }
     return ps;
 }
 

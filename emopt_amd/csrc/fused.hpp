@@ -320,14 +320,15 @@ __device__ __forceinline__ void h_compute(const Geo &g, const HIn &d, double &hx
     }
 }
 
-// magnetic sources of src/fdtd.cpp:313-366 at one cell
+// magnetic sources of src/fdtd.cpp:313-366 at one cell: the blocks of mask
+// (wave_src_mask; all blocks by default)
 template <bool MU_ARRAY, typename OT = unsigned>
 __device__ __forceinline__ void h_sources(const Geo &g, const PassSrc &sh, const FusedArgs &A,
                                           int i, int j, int k, OT o, double &hx,
-                                          double &hy, double &hz)
+                                          double &hy, double &hz, unsigned long long mask = ~0ULL)
 {
     const OT cs = cstride<OT>(A);
-    for (int q = 0; q < sh.n; ++q) {
+    src_each(sh, mask, [&](int q) {
         long long m;
         const SrcBlk sb = src_blk(sh.blk, q);
         if (in_box(sb, i, j, k, m)) {
@@ -339,7 +340,7 @@ __device__ __forceinline__ void h_sources(const Geo &g, const PassSrc &sh, const
             hy = hy + src_val(sh, vy) * g.dt / uy;
             hz = hz + src_val(sh, vz) * g.dt / uz;
         }
-    }
+    });
 }
 
 template <bool MU_ARRAY, bool CEPS, int TY>

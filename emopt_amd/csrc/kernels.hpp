@@ -109,18 +109,63 @@ __device__ __forceinline__ SrcBlk src_blk(const SrcBlk *blk, int q)
 // st = sin(t), ct = cos(t), env = ramp envelope (exactly 1 after the ramp)
 // of src_func_t (src/fdtd.cpp:1595-1602).  Blocks are applied in order, as
 // the reference loops over its source list.
+// The boxes of the first SRC_INLINE blocks, and the union of all blocks,
+// travel with the kernel arguments, so that a column decides which blocks
+// can meet it without loading a descriptor.
+constexpr int SRC_INLINE = 8;
+
 struct PassSrc {
     int n;
     int comp0;
     double st, ct;
     double env;
     const SrcBlk *blk;
+    int ninl;                     // inline boxes (the first min(n, SRC_INLINE) blocks, or 0)
+    int ub[6];                    // union of all blocks: i0, i1, j0, j1, k0, k1 (ends excluded)
+    int ib[SRC_INLINE][6];        // i0, I, j0, J, k0, K of the first ninl blocks
 };
 
 // src_func_t(t, phi) * A for one cell
 __device__ __forceinline__ double src_val(const PassSrc &p, double2 v)
 {
     return p.env * (p.st * v.x + p.ct * v.y);
+}
+
+// The source blocks that can meet column (k, j) over planes [z0, z1): bit q
+// for block q < 64.  Nothing outside the union box; among the inline boxes
+// the ones that meet the column; every block from ninl on (tested cell by
+// cell against its descriptor).  Register compares only.
+__device__ __forceinline__ unsigned long long col_src_mask(const PassSrc &ps, int k, int j,
+                                                           int z0, int z1)
+{
+    if (ps.n == 0 || k < ps.ub[4] || k >= ps.ub[5] || j < ps.ub[2] || j >= ps.ub[3] ||
+        z1 <= ps.ub[0] || z0 >= ps.ub[1])
+        return 0ULL;
+    unsigned long long m = 0;
+#pragma unroll
+    for (int q = 0; q < SRC_INLINE; ++q) {
+        if (q < ps.ninl) {
+            const int *b = ps.ib[q];
+            if (k >= b[4] && k < b[4] + b[5] && j >= b[2] && j < b[2] + b[3] && b[0] < z1 &&
+                b[0] + b[1] > z0)
+                m |= 1ULL << q;
+        }
+    }
+    const int hi = ps.n < 64 ? ps.n : 64;
+    if (hi > ps.ninl)
+        m |= (hi >= 64 ? ~0ULL : ((1ULL << hi) - 1ULL)) & ~((1ULL << ps.ninl) - 1ULL);
+    return m;
+}
+
+// f(q) for the blocks a cell may receive, in block order: those of `mask`
+// among the first 64, then (with a nonzero mask) every block from 64 on
+template <typename F>
+__device__ __forceinline__ void src_each(const PassSrc &ps, unsigned long long mask, F &&f)
+{
+    const unsigned long long lim = ps.n >= 64 ? ~0ULL : ((1ULL << ps.n) - 1ULL);
+    for (unsigned long long m = mask & lim; m; m &= m - 1ULL) f(__builtin_ctzll(m));
+    if (mask)
+        for (int q = 64; q < ps.n; ++q) f(q);
 }
 
 // ---------------------------------------------------------------- helpers

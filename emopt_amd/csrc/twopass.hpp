@@ -64,19 +64,15 @@ __device__ __forceinline__ void pass_tile(const Geo &g, int ty, int &k, int &j, 
 template <bool MU_ARRAY, bool NT = false, typename OT = unsigned>
 __device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const PassSrc &sh,
                                          int k, int j, int z0, int z1, double *cap,
-                                         long long capn)
+                                         long long capn, unsigned long long smask)
 {
     const Layout L = make_layout(g);
     const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
     const OT cs = cstride<OT>(A), cs2 = 2 * cstride<OT>(A);
     const OT oz = (OT)g.nxy * 8u, oy = (OT)Nx * 8u;
     const bool jn = j + 1 < Ny, kn = k + 1 < Nx;
-    bool col_src = false;   // does this column meet a source block?
-    for (int q = 0; q < sh.n; ++q) {
-        const SrcBlk s = src_blk(sh.blk, q);
-        if (k >= s.k0 && k < s.k0 + s.K && j >= s.j0 && j < s.j0 + s.J && s.i0 < z1 && s.i0 + s.I > z0)
-            col_src = true;
-    }
+    // may a source block meet this column?  (smask: col_src_mask)
+    const bool col_src = smask != 0;
     OT o = ((OT)z0 * (OT)Ny + (OT)j) * (OT)Nx * 8u + (OT)k * 8u;
     double ex = ldo(A.Eo, o), ey = ldo(A.Eo, o + cs);
     for (int i = z0; i < z1; ++i, o += oz) {
@@ -103,7 +99,7 @@ __device__ __forceinline__ void h_column(const FusedArgs &A, const Geo &g, const
         }
         double hx, hy, hz, np[6];
         h_compute(g, d, hx, hy, hz, np);
-        if (col_src) h_sources<MU_ARRAY, OT>(g, sh, A, i, j, k, o, hx, hy, hz);
+        if (col_src) h_sources<MU_ARRAY, OT>(g, sh, A, i, j, k, o, hx, hy, hz, smask);
         sto(A.Hn, o, hx); sto(A.Hn, o + cs, hy); sto(A.Hn, o + cs2, hz);
         if (d.sx.in) { stp<NT>(A.phn, psof(L, 0, d.sx.lo, 0) + d.sx.io, np[0]); stp<NT>(A.phn, psof(L, 0, d.sx.lo, 1) + d.sx.io, np[1]); }
         if (d.sy.in) { stp<NT>(A.phn, psof(L, 1, d.sy.lo, 0) + d.sy.io, np[2]); stp<NT>(A.phn, psof(L, 1, d.sy.lo, 1) + d.sy.io, np[3]); }
@@ -125,8 +121,9 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_h(FusedArgs A, Geo g, Pas
     pass_tile(g, TY, k, j, z0, z1);
     j = __builtin_amdgcn_readfirstlane(j);   // one row per wave
     if (k >= g.Nx || j >= g.Ny) return;
+    const unsigned long long smask = col_src_mask(sh, k, j, z0, z1);
     using OT = typename std::conditional<BIG, unsigned long long, unsigned>::type;
-    h_column<MU_ARRAY, NT, OT>(A, g, sh, k, j, z0, z1, nullptr, 0);
+    h_column<MU_ARRAY, NT, OT>(A, g, sh, k, j, z0, z1, nullptr, 0, smask);
 }
 
 // ------------------------------------------------------------------ E pass
@@ -137,7 +134,8 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_h(FusedArgs A, Geo g, Pas
 template <int CM, bool NT = false, typename OT = unsigned>
 __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const PassSrc &se,
                                         const Layout &L, int k, int j, int i, OT o,
-                                        unsigned ci, bool col_src, double &hxm, double &hym)
+                                        unsigned ci, unsigned long long smask, double &hxm,
+                                        double &hym)
 {
     const OT cs = cstride<OT>(A), cs2 = 2 * cstride<OT>(A);
     const OT oy = (OT)g.Nx * 8u;
@@ -223,8 +221,8 @@ __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const 
         nex = nex - (qz1 - dHydz + dHydz / kz) * bx;
         ney = ney + (qz0 - dHxdz + dHxdz / kz) * by;
     }
-    if (col_src) {
-        for (int q = 0; q < se.n; ++q) {
+    if (smask != 0) {
+        src_each(se, smask, [&](int q) {
             long long m;
             const SrcBlk sb = src_blk(se.blk, q);
             if (in_box(sb, i, j, k, m)) {
@@ -233,7 +231,7 @@ __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const 
                 ney = ney - src_val(se, vy) * by;
                 nez = nez - src_val(se, vz) * bz;
             }
-        }
+        });
     }
     sto(A.En, o, nex); sto(A.En, o + cs, ney); sto(A.En, o + cs2, nez);
     if (sx.in) { stp<NT>(A.pe, psof(L, 0, sx.lo, 0) + sx.io, qx0); stp<NT>(A.pe, psof(L, 0, sx.lo, 1) + sx.io, qx1); }
@@ -249,19 +247,13 @@ __device__ __forceinline__ void e_plane(const FusedArgs &A, const Geo &g, const 
 // loading its index first.
 template <int CM, int ZU, bool NT = false, typename OT = unsigned>
 __device__ __forceinline__ void e_column(const FusedArgs &A, const Geo &g, const PassSrc &se,
-                                         int k, int j, int z0, int z1)
+                                         int k, int j, int z0, int z1, unsigned long long smask)
 {
     const Layout L = make_layout(g);
     const OT cs = cstride<OT>(A);
     const OT oz = (OT)g.nxy * 8u;
     const double sgz = g.bc[2] == 'E' ? -1.0 : (g.bc[2] == 'H' ? 1.0 : 0.0);
     const bool bz0 = g.bc[2] == '0';
-    bool col_src = false;
-    for (int q = 0; q < se.n; ++q) {
-        const SrcBlk s = src_blk(se.blk, q);
-        if (k >= s.k0 && k < s.k0 + s.K && j >= s.j0 && j < s.j0 + s.J && s.i0 < z1 && s.i0 + s.I > z0)
-            col_src = true;
-    }
     OT o = ((OT)z0 * (OT)g.Ny + (OT)j) * (OT)g.Nx * 8u + (OT)k * 8u;
     if (ZU > 0) {
         unsigned ci[ZU > 0 ? ZU : 1];
@@ -274,7 +266,7 @@ __device__ __forceinline__ void e_column(const FusedArgs &A, const Geo &g, const
 #pragma unroll
         for (int p = 0; p < ZU; ++p) {
             if (z0 + p >= z1) break;
-            e_plane<CM, NT, OT>(A, g, se, L, k, j, z0 + p, o + (OT)p * oz, ci[p], col_src, hxm,
+            e_plane<CM, NT, OT>(A, g, se, L, k, j, z0 + p, o + (OT)p * oz, ci[p], smask, hxm,
                                 hym);
         }
     } else {
@@ -283,7 +275,7 @@ __device__ __forceinline__ void e_column(const FusedArgs &A, const Geo &g, const
         else { hxm = bz0 ? 0.0 : sgz * ldo(A.Ho, o); hym = bz0 ? 0.0 : sgz * ldo(A.Ho, o + cs); }
         for (int i = z0; i < z1; ++i, o += oz) {
             const unsigned ci = CM == CM_INDEX ? coef_index<OT>(A, o) : 0u;
-            e_plane<CM, NT, OT>(A, g, se, L, k, j, i, o, ci, col_src, hxm, hym);
+            e_plane<CM, NT, OT>(A, g, se, L, k, j, i, o, ci, smask, hxm, hym);
         }
     }
 }
@@ -299,8 +291,9 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_e(FusedArgs A, Geo g, Pas
     pass_tile(g, TY, k, j, z0, z1, true);
     j = __builtin_amdgcn_readfirstlane(j);   // one row per wave
     if (k >= g.Nx || j >= g.Ny) return;
+    const unsigned long long smask = col_src_mask(se, k, j, z0, z1);
     using OT = typename std::conditional<BIG, unsigned long long, unsigned>::type;
-    e_column<CM, ZU, NT, OT>(A, g, se, k, j, z0, z1);
+    e_column<CM, ZU, NT, OT>(A, g, se, k, j, z0, z1, smask);
 }
 
 }  // namespace emf

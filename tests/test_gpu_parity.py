@@ -156,6 +156,17 @@ def test_step_level_many_sources_and_no_pml():
     assert max(errs) < STEP_TOL, errs
 
 
+def test_step_level_more_than_64_sources():
+    """A wave tests the first 64 source blocks in one ballot and every later
+    block one by one: 70 overlapping blocks (applied in order) against the
+    oracle, and a grid wider than one wave."""
+    case = random_case(150, 11, 9, w=(3, 4, 2, 3, 2, 3), seed=9, nsrc=70)
+    g = _step_gpu(case, 12)
+    o = _step_oracle(case, 12)
+    errs = rel_l2_set(g, o)
+    assert max(errs) < STEP_TOL, errs
+
+
 def test_step_level_wide_pml_uniform_mu():
     case = random_case(40, 24, 20, w=(6, 5, 4, 7, 5, 6), seed=3, mu_rand=False, nsrc=1)
     g = _step_gpu(case, 40)
