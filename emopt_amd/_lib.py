@@ -105,6 +105,7 @@ _SIGS = {
     "emf_batch_set_grouping": ([_P, c_int64, c_int], c_int),
     "emf_batch_set_graph": ([_P, c_int], c_int),
     "emf_set_wide_offsets": ([_P, c_int], c_int),
+    "emf_field_products": ([_P, c_int] + [c_int] * 6 + [_P], c_int),
     "emf_stage_clear": ([_P], c_int),
     "emf_stage_count": ([_P, _P], c_int),
     "emf_stage_interp_T": ([_P, c_int, _P, c_int, c_double] + [c_int] * 6 + [_P], c_int),

@@ -1867,6 +1867,36 @@ int emf_interp_dot(emf_solver *s, int slot, int id, int k1, int k2, int j1, int 
     return EMF_OK;
 }
 
+int emf_field_products(emf_solver *s, int which, int k1, int k2, int j1, int j2, int i1, int i2,
+                       double *out)
+{
+    LOCK(s);
+    if (which != 0 && which != 1) EMF_FAIL(EMF_ERR_ARG, "which must be 0 (E) or 1 (H)");
+    if (!out) EMF_FAIL(EMF_ERR_ARG, "null output");
+    if (!s->res[0][0] || !s->res[1][0])
+        EMF_FAIL(EMF_ERR_STATE, "needs a forward and an adjoint solution");
+    if (k1 < 0 || j1 < 0 || i1 < 0 || k2 > s->Nx || j2 > s->Ny || i2 > s->Nz || k1 > k2 ||
+        j1 > j2 || i1 > i2)
+        EMF_FAIL(EMF_ERR_ARG, "box outside the grid");
+    const long long cnt = (long long)(k2 - k1) * (j2 - j1) * (i2 - i1);
+    if (cnt == 0) return EMF_OK;
+    ProdArgs pa{};
+    for (int c = 0; c < 3; ++c) {
+        pa.f[c] = s->res[0][3 * which + c];
+        pa.a[c] = s->res[1][3 * which + c];
+    }
+    CHECK(ensure_scratch(s, 3 * cnt));
+    pa.out = s->scratch;
+    const IBox dom{k1, k2, j1, j2, i1, i2};
+    hipLaunchKernelGGL(k_field_products, dim3(grid1d(cnt, 256)), dim3(256), 0, s->stream, pa,
+                       s->Nx, s->Ny, dom);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, s->scratch, 3 * cnt * sizeof(double2), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EMF_OK;
+}
+
 // ------------------------------------------------ staged (device-built) sources
 
 namespace {

@@ -739,6 +739,35 @@ __global__ __launch_bounds__(RED_THREADS) void k_interp_dot(DotArgs da, int Nx, 
     }
 }
 
+// ------------------------------------------- forward x adjoint field products
+//
+// out[c][g] = fwd_c * adj_c over a box of cells for three components (the E
+// or the H ones), the per-cell factors of the topology gradient and of the
+// torch pseudo-loss (emopt/experimental/fdtd.py:37-223).  The product is
+// rounded as numpy rounds an elementwise complex product on an FMA-capable
+// host: fma(ar, br, -(ai bi)) + i fma(ar, bi, ai br).
+struct ProdArgs {
+    const double2 *f[3], *a[3];
+    double2 *out;
+};
+
+__global__ void k_field_products(ProdArgs pa, int Nx, int Ny, IBox dom)
+{
+    const int nk = dom.k2 - dom.k1, nj = dom.j2 - dom.j1;
+    const long long total = (long long)nk * nj * (dom.i2 - dom.i1);
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < total;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int k = dom.k1 + (int)(g % nk), j = dom.j1 + (int)((g / nk) % nj);
+        const int i = dom.i1 + (int)(g / ((long long)nk * nj));
+        const long long c0 = ((long long)i * Ny + j) * Nx + k;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double2 x = pa.f[c][c0], y = pa.a[c][c0];
+            pa.out[c * total + g] = make_double2(fma(x.x, y.x, -(x.y * y.y)), fma(x.x, y.y, x.y * y.x));
+        }
+    }
+}
+
 // ------------------------------------------- adjoint sources built on the device
 //
 // The reference forms adjoint sources on the host (emopt/fomutils.py:1282-1429,

@@ -624,6 +624,67 @@ class FDTD(MaxwellSolver):
         self._lib.emf_get_mat_arrays(self._h, *[ptr(a) for a in arrs])
         return tuple(arrs)
 
+    def field_products(self, domain, which="E"):
+        """(F_x F_adj_x, F_y F_adj_y, F_z F_adj_z) on `domain` for F = E or H, as a
+        numpy array [3, nz, ny, nx]: the elementwise products of the forward
+        and adjoint phasors, formed on the GPU (emf_field_products)."""
+        if domain is None:
+            domain = DomainCoordinates(0, self._X, 0, self._Y, 0, self._Z,
+                                       self._dx, self._dy, self._dz)
+        k1, k2 = max(0, domain.k1), min(self._Nx, domain.k2)
+        j1, j2 = max(0, domain.j1), min(self._Ny, domain.j2)
+        i1, i2 = max(0, domain.i1), min(self._Nz, domain.i2)
+        out = np.zeros((3, max(i2 - i1, 0), max(j2 - j1, 0), max(k2 - k1, 0)),
+                       dtype=np.complex128)
+        if out.size:
+            self._lib.emf_field_products(self._h, {"E": 0, "H": 1}[which], k1, k2, j1, j2, i1, i2,
+                                         ptr(out))
+        return out
+
+    def calc_ydAx_topology(self, domain, update_mu, sig_eps=None, sig_mu=None, del_eps=1.,
+                           del_mu=1., planar=False, lam=0.):
+        """Topology gradient 2 sig (lam / size + del Im(sum_c F_c F_adj_c)) per
+        design cell, eps terms minus mu terms (emopt/experimental/fdtd.py:
+        37-158), with the field products formed on the GPU.  planar sums the
+        products over z first.  sig_eps / sig_mu = None weigh every cell by 1
+        (an unbounded problem, as the reference documents; its code would
+        multiply by None)."""
+        P = self.field_products(domain, "E")
+        s_e = 1.0 if sig_eps is None else sig_eps
+        if planar:
+            grad_eps = s_e * (lam / np.size(s_e) + 2 * del_eps * np.imag(
+                np.sum(P[0], axis=0) + np.sum(P[1], axis=0) + np.sum(P[2], axis=0)))
+        else:
+            grad_eps = s_e * (lam / np.size(s_e) + 2 * del_eps * np.imag(P[0] + P[1] + P[2]))
+        if not update_mu:
+            return np.ravel(grad_eps)
+        Q = self.field_products(domain, "H")
+        s_m = 1.0 if sig_mu is None else sig_mu
+        if planar:
+            grad_mu = s_m * (lam / np.size(s_m) - 2 * del_mu * np.imag(
+                np.sum(Q[0], axis=0) + np.sum(Q[1], axis=0) + np.sum(Q[2], axis=0)))
+        else:
+            grad_mu = s_m * (lam / np.size(s_m) - 2 * del_mu * np.imag(Q[0] + Q[1] + Q[2]))
+        return np.concatenate([np.ravel(grad_eps), np.ravel(grad_mu)], axis=0)
+
+    def calc_ydAx_autograd(self, domain, update_mu, eps_autograd, mu_autograd):
+        """Pseudo-loss 2 Im sum(eps o E o E_adj - mu o H o H_adj) over `domain`
+        for reverse-mode differentiation (emopt/experimental/fdtd.py:160-223):
+        its gradient with respect to the torch tensors eps_autograd (and
+        mu_autograd with update_mu) is the adjoint gradient per cell.  The
+        field products are formed on the GPU; torch moves them to the tensors'
+        device.  Tensors on the GPU need torch imported before this library is
+        first used: a HIP runtime that this library initialised first leaves
+        torch without a device."""
+        import torch
+        dev = eps_autograd[0].device
+        P = torch.as_tensor(self.field_products(domain, "E")).to(dev)
+        loss = eps_autograd[0] * P[0] + eps_autograd[1] * P[1] + eps_autograd[2] * P[2]
+        if update_mu:
+            Q = torch.as_tensor(self.field_products(domain, "H")).to(dev)
+            loss = loss - mu_autograd[0] * Q[0] - mu_autograd[1] * Q[1] - mu_autograd[2] * Q[2]
+        return 2 * loss.imag.sum()
+
     def calc_ydAx(self, Adiag0, box=None):
         """y^T (A - A0) x on the device (emopt/fdtd.py:1694-1717), the eps
         and mu terms.  Adiag0 holds full-grid arrays (as get_A_diag returns

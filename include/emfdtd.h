@@ -279,6 +279,16 @@ EMF_API int emf_set_plane_weights(emf_solver *s, int id, int64_t count, const do
 EMF_API int emf_interp_dot(emf_solver *s, int slot, int id, int k1, int k2, int j1, int j2, int i1,
                    int i2, int ek1, int ek2, int ej1, int ej2, int ei1, int ei2, double *out);
 
+/* out (host) = fwd_c * adj_c (complex, elementwise) over the box k1..k2 x
+ * j1..j2 x i1..i2 for the three E components (which = 0) or H components
+ * (which = 1) of the two result slots: 3 consecutive blocks of
+ * (i2-i1)(j2-j1)(k2-k1) complex values, z-major.  The per-cell factors of
+ * calc_ydAx_topology and calc_ydAx_autograd (emopt/experimental/fdtd.py:
+ * 37-223), which take them from get_field and get_adjoint_field: 3 products
+ * come back instead of 6 fields. */
+EMF_API int emf_field_products(emf_solver *s, int which, int k1, int k2, int j1, int j2, int i1,
+                               int i2, double *out);
+
 /* ---- adjoint sources built on the device ------------------------------ */
 /* The reference forms adjoint sources on the host: derivative planes on the
  * Ez grid (ModeMatch.get_dFd*, emopt/fomutils.py:1118-1196; the face terms
