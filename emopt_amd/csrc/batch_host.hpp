@@ -184,6 +184,8 @@ void batch_finish(emf_batch *b, int m, const BState &st)
     q.info.Tn = B.Tn;
     q.info.steps = st.n0 + 2 * B.Tn;
     q.info.status = st.status;
+    if (st.status == 1 && st.nchecks > 0 && !(std::isfinite(st.a_change) && std::isfinite(st.phi_change)))
+        q.info.status = 3;   // stopped at Nmax with non-finite samples, as emf_solve reports
     q.info.nchecks = st.nchecks;
     q.info.a_change = st.a_change;
     q.info.phi_change = st.phi_change;

@@ -1665,6 +1665,10 @@ int emf_solve(emf_solver *s, int slot, double rtol, double nmax, int64_t Tn, int
     HIPCHK(hipStreamSynchronize(s->stream));
     auto t_end = std::chrono::steady_clock::now();
     s->nhist = std::min(cs.nchecks, s->hist_cap);
+    // the loop runs on past non-finite samples (emopt/fdtd.py:1050-1052,
+    // 1137-1138) and stops at Nmax: say so when the last check saw them
+    if (status == 1 && cs.nchecks > 0 && !(std::isfinite(cs.a_change) && std::isfinite(cs.phi_change)))
+        status = 3;
     if (info) {
         info->n0 = n0;
         info->Tn = Tn;

@@ -32,7 +32,7 @@ from .simulation import MaxwellSolver
 
 __all__ = ["FDTD", "SourceArray"]
 
-STATUS = {0: "converged", 1: "nmax", 2: "fixed"}
+STATUS = {0: "converged", 1: "nmax", 2: "fixed", 3: "nan"}
 
 
 class SourceArray:
@@ -485,8 +485,10 @@ class FDTD(MaxwellSolver):
                                status=STATUS.get(info.status, str(info.status)),
                                nchecks=int(info.nchecks), a_change=float(info.a_change),
                                phi_change=float(info.phi_change), seconds=float(info.seconds))
-        if info.status == 1:
+        if info.status in (1, 3):
             warnings.warn("Maximum number of time steps exceeded.")
+        if info.status == 3:
+            warnings.warn("The fields are not finite (NaN or Inf) at the last convergence check.")
         self._have[slot] = True
 
     @property

@@ -52,7 +52,9 @@ typedef struct {
     int64_t n0;         /* step index at which the t0 snapshot was taken */
     int64_t Tn;         /* steps between snapshots, int(0.75 * Ncycle) */
     int64_t steps;      /* total time steps executed (n0 + 2*Tn) */
-    int32_t status;     /* 0 converged, 1 hit Nmax, 2 fixed step count */
+    int32_t status;     /* 0 converged, 1 hit Nmax, 2 fixed step count, 3 hit Nmax with
+                           non-finite fields at the last check (the reference's loop runs on
+                           past NaN/Inf, emopt/fdtd.py:1050-1052,1137-1138) */
     int32_t nchecks;    /* convergence checks performed */
     double a_change;    /* last relative amplitude change */
     double phi_change;  /* last relative phase change */

@@ -178,7 +178,7 @@ def run_gpu(case, fixed_n0=None, device=0):
     return out
 
 
-def run_oracle(case, fixed_n0=None, lib="oracle", max_steps=None, adjoint=False):
+def run_oracle(case, fixed_n0=None, lib="oracle", max_steps=None, adjoint=False, nmax=None):
     """The oracle's solve of `case`.  lib: "oracle" (serial C restatement),
     "omp" (the same built with OpenMP, bit-identical per cell) or "ref"
     (EMopt's own src/fdtd.cpp compiled from its sources).  adjoint=True
@@ -195,6 +195,8 @@ def run_oracle(case, fixed_n0=None, lib="oracle", max_steps=None, adjoint=False)
         for arrs, (k1, k2, j1, j2, i1, i2) in case.get("adj_sources", []):
             o.add_adjoint_sources(arrs, D.Domain.from_indices(k1, k2, j1, j2, i1, i2))
         o.set_material_arrays(case["mats"][:3], case["mats"][3:])
+        if nmax is not None:
+            o.Nmax = nmax
         if adjoint:
             o.solve_adjoint(fixed_n0=fixed_n0, max_steps=max_steps)
         else:

@@ -188,6 +188,33 @@ def test_strip_solve_converged_matches_oracle():
     print("rel L2 Ex..Hz:", " ".join("%.2e" % e for e in errs))
 
 
+def test_nan_fields_stop_at_nmax_with_status_nan():
+    """A NaN in a source current makes the fields NaN; the reference's loop
+    runs on past NaN/Inf (emopt/fdtd.py:1050-1052,1137-1138) and stops at
+    Nmax.  The device loop stops at the same step as the oracle's, after the
+    same number of checks, and reports status 'nan' (the oracle: 'nmax')."""
+    import warnings
+    from tests.cases import build_sim
+    case = strip_case(24, 20, 16, w=4)
+    case["sources"][0][0][2][0, 3, 0] = complex(np.nan, 0.0)
+    sim = build_sim(case)
+    nmax = 6.5 * sim.Tn
+    sim.Nmax = nmax
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        sim._register(sim._sources)
+        sim._solve(0)
+    o = run_oracle(case, nmax=nmax)
+    g = sim.last_solve
+    print("device: n0 %d status %s checks %d; oracle: n0 %d status %s checks %d"
+          % (g["n0"], g["status"], g["nchecks"], o["n0"], o["status"], len(o["history"])))
+    assert g["status"] == "nan" and o["status"] == "nmax"
+    assert g["n0"] == o["n0"] and g["nchecks"] == len(o["history"])
+    assert any("not finite" in str(x.message) for x in w)
+    full = sim.get_field("Ez")
+    assert np.isnan(full).any()
+
+
 def test_strip_solve_fixed_n0_bc_symmetry():
     case = strip_case(30, 22, 18, w=4, bc="0H0")
     g = run_gpu(case, fixed_n0=600)
