@@ -24,7 +24,7 @@ namespace emf {
 
 constexpr int TX = 64;          // lanes along x: one wavefront row
 constexpr int TY = 4;           // rows along y per work-group
-constexpr int NCONV_MAX = 64;
+constexpr int NCONV_MAX = 1024;
 constexpr int NUM_XCD = 8;
 
 struct Geo {
@@ -257,8 +257,8 @@ __global__ void k_field_interp(const double2 *__restrict__ F, int Nx, int Ny, IB
 
 // ------------------------------------------------------ convergence check
 //
-// One check of emopt/fdtd.py:1072-1144 for all nconv sample points, run by a
-// single thread so that the norms are accumulated in a fixed order.
+// State of the check of emopt/fdtd.py:1072-1144 for up to NCONV_MAX sample
+// points (the reference's nconv=200 of its 3D examples fits with room).
 struct ConvState {
     double eh[3][3][NCONV_MAX];   // [sample slot 0..2][component][point]
     double phi1[NCONV_MAX], a1[NCONV_MAX];
@@ -287,9 +287,10 @@ __device__ inline double amp3(double t0, double t1, double t2, double f0, double
 }
 
 // One check of emopt/fdtd.py:1072-1144 by one 64-lane wave (every lane of it
-// must call): lane q handles sample point q (nc <= 64), the four norms are
-// summed across the wave by xor butterflies (each lane ends with the same
-// sums, in a fixed order), and lane 0 writes the shared state.  Returns the
+// must call): lane l handles sample points l, l + 64, ... (nc <= NCONV_MAX),
+// the four norms are summed per lane and then across the wave by xor
+// butterflies (each lane ends with the same sums, in a fixed order), and
+// lane 0 writes the shared state.  Returns the
 // convergence flag; *nchecks, *a_change, *phi_change receive the check's
 // count and changes in every lane.
 __device__ inline int conv_check(const double *__restrict__ Ex, const double *__restrict__ Ey,
@@ -302,11 +303,12 @@ __device__ inline int conv_check(const double *__restrict__ Ex, const double *__
     // all through the global address space (the pointers may come from memory)
     typedef __attribute__((address_space(1))) ConvState gconv;
     gconv *const S = (gconv *)st;
-    const int q = (int)(threadIdx.x & 63);
+    const int lane = (int)(threadIdx.x & 63);
     const double t0 = S->t1, t1 = S->t2, t2 = tnow;
     const int h = S->nchecks;
     double sa = 0, s0 = 0, sp = 0, sp0 = 0;
-    if (q < nc) {
+    // lane l handles sample points l, l + 64, l + 128, ...
+    for (int q = lane; q < nc; q += 64) {
         const double *E[3] = { Ex, Ey, Ez };
         const long long at = *gptr(sample + q);
         double ph[3], am[3];
@@ -325,8 +327,8 @@ __device__ inline int conv_check(const double *__restrict__ Ex, const double *__
         const double phi1 = ph[0] + ph[1] + ph[2];
         const double a1 = am[0] + am[1] + am[2];
         S->phi1[q] = phi1; S->a1[q] = a1;
-        sa = (a1 - a0) * (a1 - a0); s0 = a0 * a0;
-        sp = (phi1 - phi0) * (phi1 - phi0); sp0 = phi0 * phi0;
+        sa += (a1 - a0) * (a1 - a0); s0 += a0 * a0;
+        sp += (phi1 - phi0) * (phi1 - phi0); sp0 += phi0 * phi0;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -337,7 +339,7 @@ __device__ inline int conv_check(const double *__restrict__ Ex, const double *__
     }
     const double ac = sqrt(sa) / sqrt(s0), pc = sqrt(sp) / sqrt(sp0);
     const int conv = (ac <= amp_rtol) && (pc <= phi_rtol);
-    if (q == 0) {
+    if (lane == 0) {
         S->t0 = t0; S->t1 = t1; S->t2 = t2;
         S->a_change = ac; S->phi_change = pc;
         S->nchecks = h + 1;

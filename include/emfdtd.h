@@ -216,7 +216,8 @@ EMF_API double emf_calc_amplitude_3T(double t0, double t1, double t2, double f0,
 
 /* ---- whole solve ------------------------------------------------------- */
 
-/* Number of convergence sample points (nconv, emopt/fdtd.py:240-241,411-413). */
+/* Number of convergence sample points (nconv, emopt/fdtd.py:240-241,411-413);
+ * the solve refuses a grid on which it yields more than 1024 points. */
 EMF_API int emf_set_nconv(emf_solver *s, int nconv);
 
 /* Runs the complete CW solve of emopt/fdtd.py:994-1194 on the device:

@@ -160,7 +160,8 @@ def run_gpu(case, fixed_n0=None, device=0):
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         sim = FDTD(case["X"], case["Y"], case["Z"], d, d, d, case["wl"], rtol=case["rtol"],
-                   complex_eps=case.get("ceps", False), device=device)
+                   nconv=case.get("nconv"), complex_eps=case.get("ceps", False),
+                   device=device)
         _configure(sim, case, DomainCoordinates)
     sim.set_material_arrays(case["mats"][:3], case["mats"][3:])
     sim.fixed_n0 = fixed_n0
@@ -190,7 +191,8 @@ def run_oracle(case, fixed_n0=None, lib="oracle", max_steps=None, adjoint=False,
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         o = D.OracleFDTD(D.load(path), case["X"], case["Y"], case["Z"], d, d, d, case["wl"],
-                         rtol=case["rtol"], complex_eps=case.get("ceps", False))
+                         rtol=case["rtol"], nconv=case.get("nconv"),
+                         complex_eps=case.get("ceps", False))
         _configure(o, case, D.Domain)
         for arrs, (k1, k2, j1, j2, i1, i2) in case.get("adj_sources", []):
             o.add_adjoint_sources(arrs, D.Domain.from_indices(k1, k2, j1, j2, i1, i2))
@@ -268,7 +270,8 @@ def build_sim(case, device=0):
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         sim = FDTD(case["X"], case["Y"], case["Z"], d, d, d, case["wl"], rtol=case["rtol"],
-                   complex_eps=case.get("ceps", False), device=device)
+                   nconv=case.get("nconv"), complex_eps=case.get("ceps", False),
+                   device=device)
         _configure(sim, case, DomainCoordinates)
     for arrs, (k1, k2, j1, j2, i1, i2) in case.get("adj_sources", []):
         sim.add_adjoint_sources(arrs, DomainCoordinates.from_indices(k1, k2, j1, j2, i1, i2, d, d, d))

@@ -72,6 +72,23 @@ def test_batch_matches_single_solves(cases, refs):
     b.close()
 
 
+def test_batch_many_convergence_points():
+    """A member with nconv=200 (more sample points than one wave's lanes)
+    gives what its single solve gives."""
+    from emopt_amd.batch import Batch
+    case = batch_case(1, n_fwd=2, **SMALL)
+    case["nconv"] = 200
+    ref = _plain(case)
+    sim = build_sim(case)
+    b = Batch(steps_per_graph=32)
+    b.add(sim, adjoint=case["adjoint"], postprocess=False)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        assert b.run() == [0]
+    _check(sim, case, ref, "nconv=200 member")
+    b.close()
+
+
 def test_batch_alone_equals_in_mix(cases, refs):
     """A member alone in a batch and the same problem among others."""
     from emopt_amd.batch import Batch

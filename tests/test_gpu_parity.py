@@ -188,6 +188,24 @@ def test_strip_solve_converged_matches_oracle():
     print("rel L2 Ex..Hz:", " ".join("%.2e" % e for e in errs))
 
 
+def test_solve_with_200_convergence_points_matches_oracle():
+    """nconv=200 (the reference's 3D MMI example, emopt/examples/
+    MMI_splitter_3D/mmi_1x2_splitter_3D_fdtd.py) samples about 200 cells,
+    more than one 64-lane wave holds: each lane takes several points.  The
+    same n0, the same checks and the same phasors as the oracle's."""
+    case = strip_case(24, 20, 16, w=4)
+    case["nconv"] = 200
+    g = run_gpu(case)
+    o = run_oracle(case)
+    assert len(o["sim"].conv_pts) > 128, len(o["sim"].conv_pts)
+    assert g["status"] == "converged"
+    assert g["n0"] == o["n0"], (g["n0"], o["n0"])
+    np.testing.assert_allclose(g["history"][:, 0], o["history"][:, 0])
+    np.testing.assert_allclose(g["history"][:, 1:], o["history"][:, 1:], rtol=1e-8, atol=1e-300)
+    errs = [rel_l2(a, b) for a, b in zip(g["fields"], o["fields"])]
+    assert max(errs) <= TOL, errs
+
+
 def test_nan_fields_stop_at_nmax_with_status_nan():
     """A NaN in a source current makes the fields NaN; the reference's loop
     runs on past NaN/Inf (emopt/fdtd.py:1050-1052,1137-1138) and stops at
