@@ -11,9 +11,18 @@ from .grid import (ConstantMaterial3D, StructuredMaterial3D, FunctionMaterial3D,
 __version__ = "0.1.0"
 
 
+# submodules reachable as attributes (``emopt.fdtd.FDTD``, ``emopt.modes``,
+# ...) as in emopt/__init__.py:2; imported on first use, since the solver
+# modules load the native library
+_SUBMODULES = ("fdtd", "modes", "adjoint", "adjoint_method", "fomutils", "batch", "serve",
+               "shape", "fields", "simulation", "autotune")
+
+
 def __getattr__(name):
-    # the solver pulls in the native library; import it on first use
+    import importlib
     if name in ("FDTD", "SourceArray"):
         from . import fdtd
         return getattr(fdtd, name)
+    if name in _SUBMODULES:
+        return importlib.import_module("." + name, __name__)
     raise AttributeError(name)

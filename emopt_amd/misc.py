@@ -6,6 +6,24 @@ from __future__ import annotations
 
 import numpy as np
 
+# One process drives one GPU: it is the reference's master rank
+# (emopt/misc.py:22-25 with a single MPI process).
+RANK = 0
+N_PROC = 1
+NOT_PARALLEL = True
+
+
+def run_on_master(func):
+    """Decorator of emopt/misc.py:27-38: the function runs on the master
+    rank only, which is this process."""
+    import functools
+
+    @functools.wraps(func)
+    def wrapper(*args, **kwargs):
+        return func(*args, **kwargs) if NOT_PARALLEL else None
+
+    return wrapper
+
 
 class DomainCoordinates:
     def __init__(self, xmin, xmax, ymin, ymax, zmin, zmax, dx, dy, dz):

@@ -160,8 +160,8 @@ def run_gpu(case, fixed_n0=None, device=0):
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         sim = FDTD(case["X"], case["Y"], case["Z"], d, d, d, case["wl"], rtol=case["rtol"],
-                   nconv=case.get("nconv"), complex_eps=case.get("ceps", False),
-                   device=device)
+                   nconv=case.get("nconv"), min_rindex=case.get("min_rindex", 1.0),
+                   complex_eps=case.get("ceps", False), device=device)
         _configure(sim, case, DomainCoordinates)
     sim.set_material_arrays(case["mats"][:3], case["mats"][3:])
     sim.fixed_n0 = fixed_n0
@@ -192,6 +192,7 @@ def run_oracle(case, fixed_n0=None, lib="oracle", max_steps=None, adjoint=False,
         warnings.simplefilter("ignore")
         o = D.OracleFDTD(D.load(path), case["X"], case["Y"], case["Z"], d, d, d, case["wl"],
                          rtol=case["rtol"], nconv=case.get("nconv"),
+                         min_rindex=case.get("min_rindex", 1.0),
                          complex_eps=case.get("ceps", False))
         _configure(o, case, D.Domain)
         for arrs, (k1, k2, j1, j2, i1, i2) in case.get("adj_sources", []):
@@ -270,8 +271,8 @@ def build_sim(case, device=0):
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         sim = FDTD(case["X"], case["Y"], case["Z"], d, d, d, case["wl"], rtol=case["rtol"],
-                   nconv=case.get("nconv"), complex_eps=case.get("ceps", False),
-                   device=device)
+                   nconv=case.get("nconv"), min_rindex=case.get("min_rindex", 1.0),
+                   complex_eps=case.get("ceps", False), device=device)
         _configure(sim, case, DomainCoordinates)
     for arrs, (k1, k2, j1, j2, i1, i2) in case.get("adj_sources", []):
         sim.add_adjoint_sources(arrs, DomainCoordinates.from_indices(k1, k2, j1, j2, i1, i2, d, d, d))
@@ -291,7 +292,8 @@ def case_from_sim(sim, rtol):
     return dict(Nx=sim.Nx, Ny=sim.Ny, Nz=sim.Nz, d=d, wl=sim.wavelength,
                 X=span(sim.Nx, d), Y=span(sim.Ny, d), Z=span(sim.Nz, d), w=list(sim._w_cells),
                 rtol=rtol, ramp=sim.src_ramp_time, bc="".join(sim.bc),
-                mats=[a.reshape(shp) for a in A], sources=srcs)
+                mats=[a.reshape(shp) for a in A], sources=srcs, nconv=sim._nconv,
+                min_rindex=sim._min_rindex)
 
 
 def all_fields(sim, case, adjoint=False):

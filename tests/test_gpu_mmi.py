@@ -1,0 +1,169 @@
+"""The reference's 3D MMI splitter example, run as written against this
+package (emopt/examples/MMI_splitter_3D/mmi_1x2_splitter_3D_fdtd.py up to
+its check_gradient call; the optimizer run and the plot are left out).
+
+Everything the script touches is exercised at the script's own size
+(126 x 51 x 64 cells): FDTD(rtol=1e-5, min_rindex=1.44, nconv=200),
+Nmax = 1000 Ncycle, a CPML of int((15 * 0.04) / 0.04) = 15 cells with none at y-min,
+bc '0H0', structured materials with layer priorities set as attributes,
+full-vector mode sources with bc 'H0', ModeMatch on the FOM plane, the
+power-normalised adjoint method with @run_on_master hooks, and
+check_gradient with its one-sided finite differences (fd_step 1e-10).
+
+Checked:
+  * the adjoint gradient (the reference's per-parameter formula with the
+    example's step 1e-5) agrees with the finite differences to the 1 % the
+    reference's check_gradient asks for;
+  * the converged forward solve equals the oracle's on the same material
+    arrays and mode-source currents: the same n0 and convergence history
+    over the 201 sample points (phase changes of the first three checks,
+    taken while most points hold only rounding-level precursors, to 1e-2),
+    fields within rel L2 1e-6;
+  * the figure of merit is a power fraction: -1 < F < 0.
+"""
+import warnings
+
+import numpy as np
+import pytest
+
+from tests.cases import all_fields, case_from_sim, rel_l2, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _mmi_problem():
+    import emopt_amd as emopt
+    from emopt_amd.misc import run_on_master
+    from emopt_amd.adjoint_method import AdjointMethodPNF3D
+
+    class MMISplitterAdjointMethod(AdjointMethodPNF3D):
+        def __init__(self, sim, mmi, fom_domain, mode_match):
+            super(MMISplitterAdjointMethod, self).__init__(sim, step=1e-5)
+            self.mmi = mmi
+            self.mode_match = mode_match
+            self.fom_domain = fom_domain
+
+        def update_system(self, params):
+            self.mmi.height = params[0]
+            self.mmi.width = params[1]
+
+        @run_on_master
+        def calc_f(self, sim, params):
+            Ex, Ey, Ez, Hx, Hy, Hz = sim.saved_fields[0]
+            self.mode_match.compute(Ex, Ey, Ez, Hx, Hy, Hz)
+            return -1 * self.mode_match.get_mode_match_forward(1.0)
+
+        @run_on_master
+        def calc_dfdx(self, sim, params):
+            mm = self.mode_match
+            return [(-1 * mm.get_dFdEx(), -1 * mm.get_dFdEy(), -1 * mm.get_dFdEz(),
+                     -1 * mm.get_dFdHx(), -1 * mm.get_dFdHy(), -1 * mm.get_dFdHz())]
+
+        def get_fom_domains(self):
+            return [self.fom_domain]
+
+        def calc_grad_p(self, sim, params):
+            return np.zeros(len(params))
+
+    X, Y, Z = 5.0, 4.0 / 2, 2.5
+    dx = dy = dz = 0.04
+    wavelength = 1.55
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim = emopt.fdtd.FDTD(X, Y, Z, dx, dy, dz, wavelength, rtol=1e-5, min_rindex=1.44,
+                              nconv=200)
+        sim.Nmax = 1000 * sim.Ncycle
+        w_pml = dx * 15
+        sim.w_pml = [w_pml, w_pml, 0, w_pml, w_pml, w_pml]
+    sim.bc = "0H0"
+    sim.verbose = 0
+    X, Y, Z = sim.X, sim.Y, sim.Z
+
+    w_wg, L_in, L_out, L_mmi, w_mmi, h_si = 0.45, X / 2 + 1, X / 2 + 1, 2.4, 1.75, 0.22
+    wg_in = emopt.grid.Rectangle(X / 4, 0, L_in, w_wg); wg_in.layer = 1
+    mmi = emopt.grid.Rectangle(X / 2, 0, L_mmi, w_mmi); mmi.layer = 1
+    wg_out = emopt.grid.Rectangle(3 * X / 4, w_wg, L_out, w_wg); wg_out.layer = 1
+    rbg = emopt.grid.Rectangle(X / 2, Y / 2, 2 * X, 2 * Y); rbg.layer = 2
+    wg_in.material_value = 3.45 ** 2
+    mmi.material_value = 3.45 ** 2
+    wg_out.material_value = 3.45 ** 2
+    rbg.material_value = 1.444 ** 2
+
+    eps = emopt.grid.StructuredMaterial3D(X, Y, Z, dx, dy, dz)
+    eps.add_primitive(wg_in, Z / 2 - h_si / 2, Z / 2 + h_si / 2)
+    eps.add_primitive(mmi, Z / 2 - h_si / 2, Z / 2 + h_si / 2)
+    eps.add_primitive(wg_out, Z / 2 - h_si / 2, Z / 2 + h_si / 2)
+    eps.add_primitive(rbg, -Z, 2 * Z)
+    mu = emopt.grid.ConstantMaterial3D(1.0)
+    sim.set_materials(eps, mu)
+    sim.build()
+
+    input_slice = emopt.misc.DomainCoordinates(16 * dx, 16 * dx, 0, Y - w_pml, w_pml, Z - w_pml,
+                                               dx, dy, dz)
+    mode = emopt.modes.ModeFullVector(wavelength, eps, mu, input_slice, n0=3.45, neigs=4,
+                                      verbose=False)
+    mode.bc = "H0"
+    mode.build()
+    mode.solve()
+    sim.set_sources(mode, input_slice)
+
+    fom_slice = emopt.misc.DomainCoordinates(X - w_pml - 4 * dx, X - w_pml - 4 * dx, 0,
+                                             Y - w_pml, w_pml, Z - w_pml, dx, dy, dz)
+    fom_mode = emopt.modes.ModeFullVector(wavelength, eps, mu, fom_slice, n0=3.45, neigs=4,
+                                          verbose=False)
+    fom_mode.bc = "H0"
+    fom_mode.build()
+    fom_mode.solve()
+    m = [fom_mode.get_field_interp(0, c) for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")]
+    mode_match = emopt.fomutils.ModeMatch([1, 0, 0], dy, dz, *m)
+    sim.field_domains = [fom_slice]
+    am = MMISplitterAdjointMethod(sim, mmi, fom_slice, mode_match)
+    params = np.array([w_mmi, L_mmi])
+    return sim, am, params, mode, fom_mode
+
+
+@pytest.fixture(scope="module")
+def mmi():
+    return _mmi_problem()
+
+
+def test_mmi_example_gradient_matches_finite_differences(mmi):
+    sim, am, params, mode, fom_mode = mmi
+    print("grid %d x %d x %d, CPML cells %s, input neff %s"
+          % (sim.Nx, sim.Ny, sim.Nz, list(sim._w_cells), np.round(mode.neff.real, 4)))
+    assert list(sim._w_cells) == [15, 15, 0, 15, 15, 15]
+    err, g_fd, g_am = am.check_gradient(params.copy(), plot=False, verbose=False,
+                                        return_gradients=True)
+    print("n0 %d, steps %d; finite differences %s, adjoint %s, total error %.3e"
+          % (sim.last_solve["n0"], sim.last_solve["steps"], g_fd, g_am, err))
+    assert sim.last_solve["status"] == "converged"
+    assert err < 1e-2
+
+
+def test_mmi_example_forward_solve_matches_oracle(mmi):
+    sim, am, params, mode, fom_mode = mmi
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        F = am.fom(params)
+    print("FOM %.6f, source power %.6e" % (F, sim.source_power))
+    assert -1.0 < F < 0.0
+    g = dict(sim.last_solve)
+    assert g["status"] == "converged"
+    case = case_from_sim(sim, 1e-5)
+    assert case["nconv"] == 200 and case["bc"] == "0H0"
+    got = all_fields(sim, case)
+    ora = run_oracle(case, lib="omp", nmax=sim.Nmax)
+    errs = [rel_l2(a, b) for a, b in zip(got, ora["fields"])]
+    print("n0 %d vs oracle %d over %d sample points; rel L2 Ex..Hz %s"
+          % (g["n0"], ora["n0"], len(ora["sim"].conv_pts), " ".join("%.2e" % e for e in errs)))
+    assert g["n0"] == ora["n0"]
+    h, ho = sim.convergence_history, ora["history"]
+    np.testing.assert_allclose(h[:, :2], ho[:, :2], rtol=1e-8, atol=1e-300)
+    # At the first checks the wave has not reached most of the 200 points:
+    # their fields are precursors made of cancelling stencil terms, known to
+    # a few bits, and the phases atan2 takes of them follow those bits (the
+    # second and third checks' phase changes differ by 0.4 %).  From the
+    # fourth check on every phase change agrees to 1e-8.
+    np.testing.assert_allclose(h[:3, 2], ho[:3, 2], rtol=1e-2)
+    np.testing.assert_allclose(h[3:, 2], ho[3:, 2], rtol=1e-8)
+    assert max(errs) <= 1e-6
