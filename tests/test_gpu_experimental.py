@@ -1,0 +1,204 @@
+"""The experimental adjoint methods on the GPU, on the reference's two
+experimental 3D examples.
+
+* AutoDiffPNF3D on emopt/examples/experimental/MMI_splitter_3D_AutoDiff/
+  mmi_1x2_splitter_3D_fdtd_AutoDiffPNF3D.py as written, at its own size
+  (151 x 126 x 64 cells, a 20-cell CPML, rtol 1e-4, nconv 200): the
+  structure is a torch function of the four MMI edges, the gradient comes
+  from one adjoint solve and torch.autograd through the pseudo-loss, and
+  the script's check_gradient (one-sided differences, fd_step 1e-6) must
+  agree to 1 %.
+* TopologyPNF3D on a smaller version of emopt/examples/experimental/
+  splitter_3D_Topology/splitter_TopologyPNF3D.py (the same structure on a
+  4 x 3 x 2 um grid with a 1.2 x 1.0 um design region, planar, bounded
+  eps): every design column is a parameter; check_gradient on the four
+  columns with the largest gradient, and on a run with a volume penalty.
+
+The gradient formulas are the reference's (-2 Re y^T dA x): they differ
+from the exact derivative of the time-stepped system by the leapfrog's
+frequency factor 2 sin(dt/2)/dt (about 1e-3 here), well inside 1 %.
+"""
+import warnings
+
+import numpy as np
+import pytest
+import torch  # noqa: F401  (torch before the solver library: see FDTD.calc_ydAx_autograd)
+
+pytestmark = pytest.mark.gpu
+
+
+def _modes_and_match(emopt, sim, eps, mu, wavelength, src_slice, fom_slice, n0=3.45):
+    mode = emopt.modes.ModeFullVector(wavelength, eps, mu, src_slice, n0=n0, neigs=4,
+                                      verbose=False)
+    mode.build()
+    mode.solve()
+    sim.set_sources(mode, src_slice)
+    fom_mode = emopt.modes.ModeFullVector(wavelength, eps, mu, fom_slice, n0=n0, neigs=4,
+                                          verbose=False)
+    fom_mode.build()
+    fom_mode.solve()
+    m = [fom_mode.get_field_interp(0, c) for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")]
+    return emopt.fomutils.ModeMatch([1, 0, 0], sim.dy, sim.dz, *m)
+
+
+def _mode_match_hooks(cls):
+    from emopt_amd.misc import run_on_master
+
+    class WithFOM(cls):
+        @run_on_master
+        def calc_f(self, sim, params):
+            Ex, Ey, Ez, Hx, Hy, Hz = sim.saved_fields[0]
+            self.mode_match.compute(Ex, Ey, Ez, Hx, Hy, Hz)
+            return -1 * self.mode_match.get_mode_match_forward(1.0)
+
+        @run_on_master
+        def calc_dfdx(self, sim, params):
+            mm = self.mode_match
+            return [(-1 * mm.get_dFdEx(), -1 * mm.get_dFdEy(), -1 * mm.get_dFdEz(),
+                     -1 * mm.get_dFdHx(), -1 * mm.get_dFdHy(), -1 * mm.get_dFdHz())]
+
+        def get_fom_domains(self):
+            return [self.fom_domain]
+
+    return WithFOM
+
+
+def test_autodiff_mmi_example_gradient():
+    import emopt_amd as emopt
+    from emopt_amd.experimental.fdtd import FDTD
+    from emopt_amd.experimental.adjoint_method import AutoDiffPNF3D
+    from emopt_amd.experimental.grid import AutoDiffMaterial3D
+    import emopt_amd.experimental.autodiff_geometry as adg
+    from functools import partial
+
+    nl = adg.nl_lin
+
+    def create_eps_grid(v, coords, k, zmin, zmax, wg_i, wg_o1, wg_o2, eps_l, delta_eps,
+                        bg=None):
+        z, y, x = coords
+        in_wg = adg.rect2d(k, x, y, wg_i, nl=nl)
+        out_wg1 = adg.rect2d(k, x, y, wg_o1, nl=nl)
+        out_wg2 = adg.rect2d(k, x, y, wg_o2, nl=nl)
+        mmi = adg.rect2d(k, x, y, v, nl=nl)
+        shape = adg.union_b([in_wg, out_wg1, out_wg2, mmi])
+        shape = adg.depth(shape, k, z, [zmin, zmax], nl=nl)
+        return eps_l + delta_eps * shape
+
+    class MMISplitterAdjointMethod(_mode_match_hooks(AutoDiffPNF3D)):
+        def __init__(self, sim, domain, fom_domain, mode_match):
+            super().__init__(sim, domain=domain)
+            self.mode_match = mode_match
+            self.fom_domain = fom_domain
+
+        def calc_grad_p(self, sim, params):
+            return np.zeros_like(params)
+
+    X, Y, Z, dx, dy, dz, wavelength = 6.0, 5.0, 2.5, 0.04, 0.04, 0.04, 1.55
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim = FDTD(X, Y, Z, dx, dy, dz, wavelength, rtol=1e-4, min_rindex=1.44, nconv=200)
+        sim.Nmax = 1000 * sim.Ncycle
+        w_pml = dx * 20
+        sim.w_pml = [w_pml] * 6
+    sim.verbose = 0
+    X, Y, Z = sim.X, sim.Y, sim.Z
+    domain = emopt.misc.DomainCoordinates(w_pml, X - w_pml, w_pml, Y - w_pml, w_pml, Z - w_pml,
+                                          dx, dy, dz)
+    eps_si, eps_clad = 3.45 ** 2, 1.444 ** 2
+    w_wg, L_mmi, w_mmi, h_si = 0.5, 2.5, 2.05, 0.22
+    zmin, zmax = Z / 2.0 - h_si / 2.0, Z / 2.0 + h_si / 2.0
+    L_in = L_out = X / 2.
+    wg_i = [-5.0, L_in, Y / 2 - w_wg / 2.0, Y / 2 + w_wg / 2.0]
+    wg_o1 = [X - L_out, X + 5.0, Y / 2 + w_wg - w_wg / 2.0, Y / 2 + w_wg + w_wg / 2.0]
+    wg_o2 = [X - L_out, X + 5.0, Y / 2 - w_wg - w_wg / 2.0, Y / 2 - w_wg + w_wg / 2.0]
+    design_params = np.array([X / 2.0 - L_mmi / 2, X / 2.0 + L_mmi / 2, Y / 2.0 - w_mmi / 2.0,
+                              Y / 2.0 + w_mmi / 2.0])
+    func = partial(create_eps_grid, k=1.0 / dx, zmin=zmin, zmax=zmax, wg_i=wg_i, wg_o1=wg_o1,
+                   wg_o2=wg_o2, eps_l=eps_clad, delta_eps=eps_si - eps_clad)
+    eps = AutoDiffMaterial3D(sim.dx, sim.dy, sim.dz, func, design_params)
+    mu = emopt.grid.ConstantMaterial3D(1.0)
+    sim.set_materials(eps, mu)
+    sim.build()
+    input_slice = emopt.misc.DomainCoordinates(w_pml + 3 * dx, w_pml + 3 * dx, w_pml, Y - w_pml,
+                                               w_pml, Z - w_pml, dx, dy, dz)
+    fom_slice = emopt.misc.DomainCoordinates(X - w_pml - 4 * dx, X - w_pml - 4 * dx, w_pml,
+                                             Y - w_pml, w_pml, Z - w_pml, dx, dy, dz)
+    mode_match = _modes_and_match(emopt, sim, eps, mu, wavelength, input_slice, fom_slice)
+    sim.field_domains = [fom_slice]
+    am = MMISplitterAdjointMethod(sim, domain, fom_slice, mode_match)
+    am.update_system(design_params)
+    err, g_fd, g_am = am.check_gradient(design_params, fd_step=1e-6, plot=False, verbose=False,
+                                        return_gradients=True)
+    print("grid %d x %d x %d, n0 %d; finite differences %s, autodiff %s, total error %.3e"
+          % (sim.Nx, sim.Ny, sim.Nz, sim.last_solve["n0"], g_fd, g_am, err))
+    assert (sim.Nx, sim.Ny, sim.Nz) == (151, 126, 64)
+    assert sim.last_solve["status"] == "converged"
+    assert err < 1e-2
+
+
+def _topology_problem(planar=True, vol_penalty=0.0):
+    import emopt_amd as emopt
+    from emopt_amd.experimental.adjoint_method import TopologyPNF3D
+    from emopt_amd.experimental.fdtd import FDTD
+    from emopt_amd.experimental.grid import TopologyMaterial3D
+
+    class TopologyAM(_mode_match_hooks(TopologyPNF3D)):
+        def __init__(self, sim, mode_match, mm_line, **kw):
+            super().__init__(sim, **kw)
+            self.mode_match = mode_match
+            self.fom_domain = mm_line
+
+    wavelength, X, Y, Z, dx = 1.31, 4.0, 3.0, 2.0, 0.04
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim = FDTD(X, Y, Z, dx, dx, dx, wavelength, rtol=1e-5, min_rindex=1.44, nconv=200)
+        sim.Nmax = 1000 * sim.Ncycle
+    sim.verbose = 0
+    X, Y, Z = sim.X, sim.Y, sim.Z
+    w_pml = sim.w_pml[0]
+    optdomain = emopt.misc.DomainCoordinates(0.5 * X - 0.6, 0.5 * X + 0.6, 0.5 * Y - 0.5,
+                                             0.5 * Y + 0.5, 0.5 * Z - 0.15, 0.5 * Z + 0.15,
+                                             dx, dx, dx)
+    eps_si, eps_clad = 3.5 ** 2, 1.444 ** 2
+    R = emopt.grid.Rectangle
+    in_wg = R(0.0, Y / 2.0, X, 0.5); in_wg.layer = 2; in_wg.material_value = eps_si
+    out1 = R(X, Y / 2.0 + 0.35, X, 0.4); out1.layer = 2; out1.material_value = eps_si
+    out2 = R(X, Y / 2.0 - 0.35, X, 0.4); out2.layer = 2; out2.material_value = eps_si
+    init = R(X / 2, Y / 2, 1.2 + 2 * dx, 1.0 + 2 * dx); init.layer = 1
+    init.material_value = eps_si / 2
+    bg = R(X / 2, Y / 2, 2 * X, 2 * Y); bg.layer = 3; bg.material_value = eps_clad
+    eps = emopt.grid.StructuredMaterial3D(X, Y, Z, dx, dx, dx)
+    for p in (in_wg, out1, out2, init):
+        eps.add_primitive(p, 0.5 * Z - 0.15, 0.5 * Z + 0.15)
+    eps.add_primitive(bg, -Z, 2 * Z)
+    eps_top = TopologyMaterial3D(eps, optdomain)
+    mu = emopt.grid.ConstantMaterial3D(1.0)
+    sim.set_materials(eps_top, mu)
+    src_line = emopt.misc.DomainCoordinates(w_pml + 5 * dx, w_pml + 5 * dx, Y / 2 - 1.0,
+                                            Y / 2 + 1.0, w_pml, Z - w_pml, dx, dx, dx)
+    mode_line = emopt.misc.DomainCoordinates(X - (w_pml + 5 * dx), X - (w_pml + 5 * dx),
+                                             Y / 2 - 1.2, Y / 2 + 1.2, w_pml, Z - w_pml, dx, dx, dx)
+    mode_match = _modes_and_match(emopt, sim, eps, mu, wavelength, src_line, mode_line, n0=3.5)
+    sim.field_domains = [mode_line]
+    sim.build()
+    am = TopologyAM(sim, mode_match, mode_line, domain=optdomain, update_mu=False,
+                    eps_bounds=[eps_clad, eps_si], planar=planar, vol_penalty=vol_penalty)
+    params = am.get_params(squish=0.02)
+    am.update_system(params)
+    return sim, am, params, optdomain
+
+
+@pytest.mark.parametrize("vol_penalty", [0.0, 0.05])
+def test_topology_splitter_gradient(vol_penalty):
+    sim, am, params, dom = _topology_problem(vol_penalty=vol_penalty)
+    assert params.size == dom.Ny * dom.Nx
+    g = am.gradient(params)
+    assert g.shape == params.shape and np.all(np.isfinite(g))
+    idx = np.argsort(-np.abs(g))[:4]
+    err, g_fd, g_am = am.check_gradient(params, indices=idx, fd_step=1e-3, plot=False,
+                                        verbose=False, return_gradients=True)
+    print("design %d x %d x %d, %d parameters, penalty %.2f; columns %s: finite differences %s,"
+          " adjoint %s, total error %.3e" % (dom.Nx, dom.Ny, dom.Nz, params.size, vol_penalty,
+                                             idx, g_fd, g_am[idx], err))
+    np.testing.assert_allclose(g_am, g, rtol=1e-12, atol=1e-300)
+    assert err < 1e-2
