@@ -97,6 +97,24 @@ class FunctionMaterial3D(Material3D):
         return self._out(vals, arr, reshape)
 
 
+class GridMaterial3D(Material3D):
+    """Values held in an array ``grid[i, j, k]`` over an Nx x Ny x Nz grid of
+    size X x Y x Z (emopt/grid.py:238-251).  get_values returns the cells of
+    a box as they are, for every Yee offset."""
+
+    def __init__(self, X, Y, Z, Nx, Ny, Nz, grid):
+        self.X, self.Y, self.Z = X, Y, Z
+        self.Nx, self.Ny, self.Nz = Nx, Ny, Nz
+        self.grid = grid
+
+    def get_value(self, k, j, i):
+        return self.grid[i, j, k]
+
+    def get_values(self, k1, k2, j1, j2, i1, i2, sx=0.0, sy=0.0, sz=0.0, arr=None, reshape=True):
+        vals = np.asarray(self.grid, dtype=np.complex128)[i1:i2, j1:j2, k1:k2].copy()
+        return self._out(vals, arr, reshape)
+
+
 # ------------------------------------------------------------- primitives
 
 class MaterialPrimitive:
@@ -106,6 +124,21 @@ class MaterialPrimitive:
 
     def set_material(self, mat):
         self.material_value = complex(mat)
+
+    # the deprecated accessors of emopt/grid.py:288-310
+    def get_layer(self):
+        return self.layer
+
+    def set_layer(self, layer):
+        self.layer = layer
+
+    def contains_point(self, x, y):
+        """True when (x, y) lies strictly inside the primitive, the test the
+        overlap rule applies to cell corners (emopt/grid.py:312-328)."""
+        inside = StructuredMaterial3D._ring_of(self, 128)
+        if inside is None:
+            return False
+        return bool(inside.within(np.asarray([float(x)]), np.asarray([float(y)]))[0])
 
     def coverage(self, xc, yc, dx, dy, ss):
         """Area fraction of each cell centred at (yc[:,None], xc[None,:]) of
@@ -122,6 +155,16 @@ class Rectangle(MaterialPrimitive):
     def __init__(self, x0, y0, xspan, yspan, material_value=1.0, layer=1):
         super().__init__(material_value, layer)
         self.x0, self.y0, self.width, self.height = x0, y0, xspan, yspan
+
+    # the setters of emopt/grid.py:465-501
+    def set_position(self, x0, y0):
+        self.x0, self.y0 = x0, y0
+
+    def set_width(self, width):
+        self.width = width
+
+    def set_height(self, height):
+        self.height = height
 
     def bbox(self):
         return (self.x0 - self.width / 2, self.x0 + self.width / 2,
@@ -210,6 +253,36 @@ class Polygon(_Sampled):
     def set_points(self, xs, ys):
         self.xs = np.asarray(xs, dtype=np.double)
         self.ys = np.asarray(ys, dtype=np.double)
+
+    # vertex editing of emopt/grid.py:556-592
+    @property
+    def Np(self):
+        return len(self.xs)
+
+    def add_point(self, x, y):
+        self.xs = np.append(self.xs, float(x))
+        self.ys = np.append(self.ys, float(y))
+
+    def add_points(self, x, y):
+        self.xs = np.concatenate([self.xs, np.asarray(x, dtype=np.double)])
+        self.ys = np.concatenate([self.ys, np.asarray(y, dtype=np.double)])
+
+    def set_point(self, index, x, y):
+        self.xs = np.array(self.xs, dtype=np.double)
+        self.ys = np.array(self.ys, dtype=np.double)
+        self.xs[index], self.ys[index] = x, y
+
+    @staticmethod
+    def populate_lines(xs, ys, ds):
+        """Points spaced at least ds along the connected segments (xs,
+        ys): a segment of length s is cut into max(ceil(s / ds), 2) - 1
+        equal pieces, its far end kept (emopt/grid.py:594-651)."""
+        outx, outy = [np.asarray([xs[0]], dtype=np.double)], [np.asarray([ys[0]], dtype=np.double)]
+        for a in range(len(xs) - 1):
+            n = max(int(np.ceil(np.hypot(xs[a + 1] - xs[a], ys[a + 1] - ys[a]) / ds)), 2)
+            outx.append(np.linspace(xs[a], xs[a + 1], n)[1:])
+            outy.append(np.linspace(ys[a], ys[a + 1], n)[1:])
+        return np.concatenate(outx), np.concatenate(outy)
 
     def bbox(self):
         return self.xs.min(), self.xs.max(), self.ys.min(), self.ys.max()
@@ -447,12 +520,24 @@ class StructuredMaterial3D(Material3D):
         for p in prims:
             self.add_primitive(p, z1, z2)
 
+    @property
+    def zmins(self):
+        return [z1 for _, z1, _, _ in self._prims]
+
+    @property
+    def zmaxs(self):
+        return [z2 for _, _, z2, _ in self._prims]
+
     def _ring(self, prim):
+        return self._ring_of(prim, self.ellipse_vertices)
+
+    @staticmethod
+    def _ring_of(prim, ellipse_vertices):
         """The primitive as the exact rule sees it (None: nothing left)."""
         if isinstance(prim, Rectangle):
             return _Ring(0, *prim.ring(), prim.bbox(), prim.material_value)
         if isinstance(prim, Ellipse):
-            xs, ys = prim.polygon(self.ellipse_vertices)
+            xs, ys = prim.polygon(ellipse_vertices)
         elif isinstance(prim, Polygon):
             xs, ys = prim.ring()
         else:
