@@ -367,6 +367,16 @@ class AdjointMethod:
         self._step = step
         self.prev_params = []          # params of the last forward solve (fom)
 
+    @property
+    def step(self):
+        """Parameter step of the gradient loop (emopt/adjoint_method.py:
+        122-128)."""
+        return self._step
+
+    @step.setter
+    def step(self, val):
+        self._step = val
+
     # --- user hooks (emopt/adjoint_method.py:130-260)
     def update_system(self, params):
         raise NotImplementedError
@@ -411,12 +421,17 @@ class AdjointMethod:
         """True when the last forward solve was at exactly these params."""
         return np.array_equal(self.prev_params, np.asarray(params, dtype=np.double))
 
-    def solve_adjoint_for(self, params):
+    def prepare_adjoint(self, params):
+        """Set the solver's adjoint sources from the last forward solve:
+        built on the device when stage_dFdx can, else calc_dFdx's planes."""
         if self.device_sources and self.stage_dFdx(self.sim, params):
             self.sim.set_adjoint_sources_staged()
         else:
             src = self.calc_dFdx(self.sim, params)
             self.sim.set_adjoint_sources(src)
+
+    def solve_adjoint_for(self, params):
+        self.prepare_adjoint(params)
         self.sim.solve_adjoint()
 
     def calc_gradient(self, sim, params):
@@ -607,6 +622,124 @@ class AdjointMethodPNF3D(AdjointMethod):
             a = power_norm_dFdx_3D(sim, f, dom, *dfdx)
             srcs = [srcs[0] + a[0], srcs[1] + a[1]]
         return srcs
+
+
+class AdjointMethodMO(AdjointMethod):
+    """A figure of merit made of several adjoint methods' figures of merit,
+    e.g. one per wavelength of a broadband design
+    (emopt/adjoint_method.py:723-935).  Subclasses define
+    calc_total_fom(foms) and calc_total_gradient(foms, grads).
+
+    batched=True (not in the reference) runs the members' forward solves
+    together as one batch on the GPU (emopt_amd.batch.Batch), and then their
+    adjoint solves together; each member's result equals its own solve to
+    roundoff.  The members must then drive distinct solver objects."""
+
+    def __init__(self, ams, step=1e-6, batched=False):
+        self._ams = list(ams)
+        self._foms_current = np.zeros(len(self._ams))
+        self._step = step
+        self.prev_params = []
+        self.sim = self._ams[0].sim if self._ams else None
+        self.batched = batched
+        self._batch = None
+
+    @property
+    def adjoint_methods(self):
+        return self._ams
+
+    @adjoint_methods.setter
+    def adjoint_methods(self, new_ams):
+        self._ams = list(new_ams)
+
+    def update_system(self, params):
+        for am in self._ams:
+            am.update_system(params)
+
+    def calc_fom(self, sim, params):
+        return self.calc_total_fom(self._foms_current)
+
+    def calc_dFdx(self, sim, params):
+        return None        # each member forms its own adjoint sources
+
+    def calc_grad_p(self, sim, params):
+        return None        # part of each member's gradient
+
+    def calc_total_fom(self, foms):
+        raise NotImplementedError
+
+    def calc_total_gradient(self, foms, grads):
+        raise NotImplementedError
+
+    # --- batched member solves
+    def _batch_of(self):
+        from .batch import Batch
+        sims = [am.sim for am in self._ams]
+        if len({id(s) for s in sims}) != len(sims):
+            raise ValueError("batched=True needs a distinct solver per member")
+        if self._batch is None:
+            self._batch = Batch(device=sims[0]._device)
+        return self._batch
+
+    def _forward_all(self, params, only_stale):
+        """Update every member's structure and run the forward solves (all
+        of them, or those not already at params) as one batch."""
+        b = self._batch_of()
+        for am in self._ams:
+            am.update_system(params)
+            am.sim.update()
+            if only_stale and am._forward_current(params):
+                continue
+            am.prev_params = np.array(params, dtype=np.double, copy=True)
+            b.add(am.sim)
+        b.run()
+
+    def fom(self, params):
+        """The members' figures of merit (each its own forward solve) and
+        their total (emopt/adjoint_method.py:841-868)."""
+        if self.batched:
+            self._forward_all(params, only_stale=False)
+            foms = [am.calc_fom(am.sim, params) for am in self._ams]
+        else:
+            foms = [am.fom(params) for am in self._ams]
+        self._foms_current = foms
+        self.prev_params = np.array(params, dtype=np.double, copy=True)
+        return self.calc_total_fom(foms)
+
+    def gradient(self, params):
+        """The members' gradients and figures of merit combined by
+        calc_total_gradient (emopt/adjoint_method.py:870-899)."""
+        if not self.batched:
+            grads, foms = [], []
+            for am in self._ams:
+                grads.append(am.gradient(params))
+                foms.append(am.calc_fom(am.sim, params))
+            return self.calc_total_gradient(foms, grads)
+        self._forward_all(params, only_stale=True)
+        b = self._batch_of()
+        for am in self._ams:
+            am.prepare_adjoint(params)
+            b.add(am.sim, adjoint=True)
+        b.run()
+        grads = [am.calc_gradient(am.sim, params) + am.calc_grad_p(am.sim, params)
+                 for am in self._ams]
+        foms = [am.calc_fom(am.sim, params) for am in self._ams]
+        return self.calc_total_gradient(foms, grads)
+
+    def check_gradient(self, params, indices=[], plot=True, verbose=True,
+                       return_gradients=False, fd_step=1e-10):
+        """check_gradient of the total figure of merit
+        (emopt/adjoint_method.py:901-935)."""
+        self.sim = self._ams[0].sim
+        for am in self._ams:
+            am.update_system(params)
+            am.sim.update()
+        return super().check_gradient(params, indices, plot, verbose, return_gradients, fd_step)
+
+    def close(self):
+        if self._batch is not None:
+            self._batch.close()
+            self._batch = None
 
 
 def _torch():

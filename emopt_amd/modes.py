@@ -360,6 +360,38 @@ class ModeFullVector:
         Wtot = sum(W)
         return [w / Wtot for w in W]
 
+    def get_mode_number(self, i):
+        """(P, Q): the numbers of sign changes of mode i's stronger
+        transverse electric component along the slice's first and second
+        axes, through the component's largest value, after a global phase
+        makes that value real.  Samples under 1e-3 of it (the tails, and the
+        node itself) are skipped.  A slice with an E or H symmetry face holds
+        half the mode, and the count is that half's.  The reference declares
+        this method but leaves it empty (emopt/modes.py:2099-2118), so its
+        find_mode_index cannot run."""
+        M, N = self._M, self._N
+        comps = [np.reshape(self._x[i][c * M * N:(c + 1) * M * N], (M, N)) for c in (0, 1)]
+        f = max(comps, key=lambda a: float(np.sum(np.abs(a) ** 2)))
+        peak = int(np.argmax(np.abs(f)))
+        m0, n0 = divmod(peak, N)
+        f = f * np.exp(-1j * np.angle(f.flat[peak]))
+        tol = 1e-3 * abs(f.flat[peak])
+
+        def changes(line):
+            v = np.real(line)[np.abs(line) > tol]
+            return int(np.count_nonzero(np.diff(np.sign(v)) != 0))
+
+        return changes(f[m0, :]), changes(f[:, n0])
+
+    def find_mode_index(self, P, Q):
+        """Index of the first mode whose get_mode_number is (P, Q); 0 with a
+        warning when there is none (emopt/modes.py:2120-2141)."""
+        for i in range(self.neigs):
+            if self.get_mode_number(i) == (P, Q):
+                return i
+        warnings.warn("Desired mode number was not found.")
+        return 0
+
     # -------------------------------------------------------------- source
     def get_source(self, i, dx, dy, dz):
         """The current sources (Jx, Jy, Jz, Mx, My, Mz) on the slice that

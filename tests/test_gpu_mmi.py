@@ -31,7 +31,7 @@ from tests.cases import all_fields, case_from_sim, rel_l2, run_oracle
 pytestmark = pytest.mark.gpu
 
 
-def _mmi_problem():
+def _mmi_problem(wavelength=1.55):
     import emopt_amd as emopt
     from emopt_amd.misc import run_on_master
     from emopt_amd.adjoint_method import AdjointMethodPNF3D
@@ -67,7 +67,6 @@ def _mmi_problem():
 
     X, Y, Z = 5.0, 4.0 / 2, 2.5
     dx = dy = dz = 0.04
-    wavelength = 1.55
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         sim = emopt.fdtd.FDTD(X, Y, Z, dx, dy, dz, wavelength, rtol=1e-5, min_rindex=1.44,
@@ -167,3 +166,46 @@ def test_mmi_example_forward_solve_matches_oracle(mmi):
     np.testing.assert_allclose(h[:3, 2], ho[:3, 2], rtol=1e-2)
     np.testing.assert_allclose(h[3:, 2], ho[3:, 2], rtol=1e-8)
     assert max(errs) <= 1e-6
+
+
+def test_mmi_broadband_batched_members_equal_sequential():
+    """AdjointMethodMO over the MMI at 1.50 and 1.60 um (the mean of the two
+    figures of merit): with batched=True the two forward solves run as one
+    batch and then the two adjoint solves; the figure of merit and the
+    gradient equal the sequential ones (emopt/adjoint_method.py:841-899)
+    to roundoff, with the same n0 per member."""
+    import time
+    from emopt_amd.adjoint_method import AdjointMethodMO
+
+    class Broadband(AdjointMethodMO):
+        def calc_total_fom(self, foms):
+            return float(np.mean(foms))
+
+        def calc_total_gradient(self, foms, grads):
+            return np.mean(grads, axis=0)
+
+    probs = [_mmi_problem(wl) for wl in (1.50, 1.60)]
+    ams = [p[1] for p in probs]
+    params = probs[0][2]
+    seq = Broadband(ams)
+    t0 = time.perf_counter()
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        F_seq = seq.fom(params)
+        n_seq = [am.sim.last_solve["n0"] for am in ams]
+        g_seq = seq.gradient(params)
+    t1 = time.perf_counter()
+    bat = Broadband(ams, batched=True)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        F_bat = bat.fom(params)
+        n_bat = [am.sim.last_solve["n0"] for am in ams]
+        g_bat = bat.gradient(params)
+    t2 = time.perf_counter()
+    bat.close()
+    print("sequential: F %.12f g %s n0 %s (%.2f s); batched: F %.12f g %s n0 %s (%.2f s)"
+          % (F_seq, g_seq, n_seq, t1 - t0, F_bat, g_bat, n_bat, t2 - t1))
+    assert n_bat == n_seq
+    assert abs(F_bat - F_seq) <= 1e-11 * abs(F_seq)
+    np.testing.assert_allclose(g_bat, g_seq, rtol=1e-9, atol=1e-12 * np.abs(g_seq).max())
+    assert all(am.sim.last_solve["status"] == "converged" for am in ams)

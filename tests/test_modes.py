@@ -134,3 +134,37 @@ def test_y_normal_slice_rejected():
     eps = grid.ConstantMaterial3D(2.0)
     with pytest.raises(NotImplementedError):
         ModeFullVector(WL, eps, eps, DomainCoordinates.from_indices(0, 5, 2, 3, 0, 5, d, d, d))
+
+
+def test_mode_numbers_of_a_wide_guide():
+    """A 2 um x 0.22 um Si strip in SiO2 (an MMI section), x-normal slice
+    with PEC walls: its quasi-TE modes in order of decreasing index have 0, 1,
+    2, ... sign changes across the width and none across the height.  The
+    reference leaves get_mode_number empty, so the count is pinned by the
+    guide's mode order instead."""
+    d = 0.04
+    N = (3, 81, 41)
+    X, Y, Z = [(n - 1) * d for n in N]
+    eps = grid.StructuredMaterial3D(X, Y, Z, d, d, d)
+    eps.add_primitive(grid.Rectangle(X / 2, Y / 2, 10 * X, 10 * Y, material_value=N2 ** 2,
+                                     layer=2), -1, 10)
+    eps.add_primitive(grid.Rectangle(X / 2, Y / 2, 10 * X, 2.0, material_value=N1 ** 2, layer=1),
+                      Z / 2 - T / 2, Z / 2 + T / 2)
+    dom = DomainCoordinates.from_indices(1, 2, 0, N[1], 0, N[2], d, d, d)
+    m = ModeFullVector(WL, eps, grid.ConstantMaterial3D(1.0), dom, n0=3.2, neigs=6,
+                       verbose=False)
+    m.build()
+    m.solve()
+    te = []
+    for i in range(m.neigs):
+        W = m.component_energy(i)
+        # quasi-TE in this slice: the field lies along the width (grid y)
+        if W[1] > W[2]:
+            te.append((m.neff[i].real, m.get_mode_number(i)))
+    te.sort(reverse=True)
+    print("quasi-TE neff and mode numbers:", te)
+    assert [pq for _, pq in te[:3]] == [(0, 0), (1, 0), (2, 0)]
+    i1 = m.find_mode_index(1, 0)
+    assert m.get_mode_number(i1) == (1, 0)
+    with pytest.warns(UserWarning):
+        assert m.find_mode_index(9, 9) == 0
