@@ -15,7 +15,7 @@ __version__ = "0.1.0"
 # ...) as in emopt/__init__.py:2; imported on first use, since the solver
 # modules load the native library
 _SUBMODULES = ("fdtd", "modes", "adjoint", "adjoint_method", "fomutils", "batch", "serve",
-               "shape", "fields", "simulation", "autotune")
+               "shape", "fields", "simulation", "autotune", "optimizer", "experimental")
 
 
 def __getattr__(name):

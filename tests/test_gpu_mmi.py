@@ -209,3 +209,22 @@ def test_mmi_broadband_batched_members_equal_sequential():
     assert abs(F_bat - F_seq) <= 1e-11 * abs(F_seq)
     np.testing.assert_allclose(g_bat, g_seq, rtol=1e-9, atol=1e-12 * np.abs(g_seq).max())
     assert all(am.sim.last_solve["status"] == "converged" for am in ams)
+
+
+def test_mmi_example_optimizer_iterations(mmi):
+    """The script's optimisation call, emopt.optimizer.Optimizer(am, params,
+    Nmax=..., opt_method='L-BFGS-B').run(), for two iterations: the figure of
+    merit (minus the power fraction in the output supermode) decreases."""
+    import emopt_amd as emopt
+    sim, am, params, mode, fom_mode = mmi
+    foms = []
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        F0 = am.fom(params)
+        opt = emopt.optimizer.Optimizer(am, params.copy(), Nmax=2, opt_method="L-BFGS-B",
+                                        callback_func=lambda p: foms.append(am.calc_fom(sim, p)),
+                                        scipy_verbose=False)
+        F1, p1 = opt.run()
+    print("FOM %.6f -> %.6f, params %s -> %s" % (F0, F1, params, p1))
+    assert F1 < F0
+    assert len(foms) >= 2
