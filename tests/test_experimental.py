@@ -187,3 +187,15 @@ def test_sigmoid_maps():
     np.testing.assert_allclose(scaled_sigmoid(p, lb, ub), lb + (ub - lb) * sigmoid(p))
     with pytest.raises(ValueError):
         inverse_scaled_sigmoid(np.array([ub + 0.6 * (ub - lb)]), lb, ub, squish=sq)
+
+
+def test_module_surface():
+    """The names scripts reach through the package, as in emopt/__init__.py."""
+    import emopt_amd as emopt
+    assert emopt.experimental.grid.TopologyMaterial3D is TopologyMaterial3D
+    assert emopt.experimental.adjoint_method.sigmoid is sigmoid
+    assert emopt.experimental.autodiff_geometry.rect2d is adg.rect2d
+    assert emopt.optimizer.Optimizer.__name__ == "Optimizer"
+    assert emopt.misc.NOT_PARALLEL and emopt.misc.run_on_master(lambda: 3)() == 3
+    assert emopt.fomutils.ModeMatch is emopt.adjoint.ModeMatch
+    assert emopt.adjoint_method.AdjointMethodMO is emopt.adjoint.AdjointMethodMO
