@@ -633,7 +633,9 @@ class AdjointMethodMO(AdjointMethod):
     batched=True (not in the reference) runs the members' forward solves
     together as one batch on the GPU (emopt_amd.batch.Batch), and then their
     adjoint solves together; each member's result equals its own solve to
-    roundoff.  The members must then drive distinct solver objects."""
+    roundoff.  The members must then drive distinct solver objects.  A member
+    the batch does not take (complex eps, non-uniform mu, a grid above 32-bit
+    offsets) is solved on its own after the batch."""
 
     def __init__(self, ams, step=1e-6, batched=False):
         self._ams = list(ams)
@@ -681,18 +683,33 @@ class AdjointMethodMO(AdjointMethod):
             self._batch = Batch(device=sims[0]._device)
         return self._batch
 
+    def _solve_together(self, sims, adjoint):
+        """The solves of `sims` as one batch; those the batch refuses run on
+        their own afterwards."""
+        from ._lib import EmfError
+        b = self._batch_of()
+        alone = []
+        for sim in sims:
+            try:
+                b.add(sim, adjoint=adjoint)
+            except EmfError:
+                alone.append(sim)
+        b.run()
+        for sim in alone:
+            sim.solve_adjoint() if adjoint else sim.solve_forward()
+
     def _forward_all(self, params, only_stale):
         """Update every member's structure and run the forward solves (all
         of them, or those not already at params) as one batch."""
-        b = self._batch_of()
+        todo = []
         for am in self._ams:
             am.update_system(params)
             am.sim.update()
             if only_stale and am._forward_current(params):
                 continue
             am.prev_params = np.array(params, dtype=np.double, copy=True)
-            b.add(am.sim)
-        b.run()
+            todo.append(am.sim)
+        self._solve_together(todo, adjoint=False)
 
     def fom(self, params):
         """The members' figures of merit (each its own forward solve) and
@@ -716,11 +733,9 @@ class AdjointMethodMO(AdjointMethod):
                 foms.append(am.calc_fom(am.sim, params))
             return self.calc_total_gradient(foms, grads)
         self._forward_all(params, only_stale=True)
-        b = self._batch_of()
         for am in self._ams:
             am.prepare_adjoint(params)
-            b.add(am.sim, adjoint=True)
-        b.run()
+        self._solve_together([am.sim for am in self._ams], adjoint=True)
         grads = [am.calc_gradient(am.sim, params) + am.calc_grad_p(am.sim, params)
                  for am in self._ams]
         foms = [am.calc_fom(am.sim, params) for am in self._ams]
