@@ -147,6 +147,26 @@ def test_step_level(bc, ceps):
     assert max(errs) < STEP_TOL, (bc, ceps, errs)
 
 
+_ALL_BCS = [a + b + c for a in "0EH" for b in "0EH" for c in "0EH"]
+
+
+@pytest.mark.parametrize("bc", _ALL_BCS)
+def test_step_level_every_boundary_combination(bc):
+    """Every combination of PEC / E-symmetry / H-symmetry on the three min
+    faces (src/fdtd.cpp:398-501 ghost rules; the max faces are PEC), each
+    with its own CPML widths (zero on some faces), random materials with
+    non-uniform mu, complex eps on every other case, and two source blocks,
+    20 steps against the oracle."""
+    q = _ALL_BCS.index(bc)
+    rng = np.random.default_rng(100 + q)
+    w = tuple(int(v) for v in rng.integers(0, 4, size=6))
+    case = random_case(12, 11, 10, w=w, seed=200 + q, ceps=bool(q % 2), bc=bc, nsrc=2)
+    g = _step_gpu(case, 20)
+    o = _step_oracle(case, 20)
+    errs = rel_l2_set(g, o)
+    assert max(errs) < STEP_TOL, (bc, w, errs)
+
+
 def test_step_level_many_sources_and_no_pml():
     # more source blocks than the kernel carries inline, and zero CPML widths
     case = random_case(12, 10, 8, w=(0, 0, 0, 0, 0, 0), seed=7, nsrc=6)
