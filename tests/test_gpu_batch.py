@@ -89,6 +89,45 @@ def test_batch_many_convergence_points():
     b.close()
 
 
+def test_batch_members_with_symmetry_boundaries():
+    """Members with E / H symmetry on min faces (no CPML there) and
+    different CPML widths per face: each equals its single solve, and one
+    forward and one adjoint member equal the oracle's solves."""
+    from emopt_amd.batch import Batch
+    from tests.cases import run_oracle
+    bcs = ["0H0", "E00", "HE0", "EEH"]
+    cases = []
+    for q, bc in enumerate(bcs):
+        c = batch_case(q, seed=11, n_fwd=2, **SMALL)      # 0, 1 forward; 2, 3 adjoint
+        w = list(c["w"])
+        for a, f in enumerate(bc):
+            if f in "EH":
+                w[2 * a] = 0
+        w[1] = max(w[1] - 1, 2)
+        c["bc"], c["w"] = bc, w
+        cases.append(c)
+    refs = [_plain(c) for c in cases]
+    b = Batch(steps_per_graph=32)
+    sims = [build_sim(c) for c in cases]
+    for s_, c in zip(sims, cases):
+        b.add(s_, adjoint=c["adjoint"], postprocess=False)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        assert sorted(b.run()) == list(range(len(cases)))
+    for q, (s_, c, r) in enumerate(zip(sims, cases, refs)):
+        _check(s_, c, r, "%s member (%s)" % (c["bc"], "adj" if c["adjoint"] else "fwd"))
+    b.close()
+    for q in (0, 2):                       # one forward, one adjoint member
+        c = cases[q]
+        o = run_oracle(c, lib="omp", adjoint=c["adjoint"])
+        got = all_fields(sims[q], c, c["adjoint"])
+        errs = [rel_l2(x, y) for x, y in zip(got, o["fields"])]
+        print("%s vs oracle: n0 %d / %d, max rel L2 %.2e"
+              % (c["bc"], sims[q].last_solve["n0"], o["n0"], max(errs)))
+        assert sims[q].last_solve["n0"] == o["n0"]
+        assert max(errs) <= 1e-6
+
+
 def test_batch_alone_equals_in_mix(cases, refs):
     """A member alone in a batch and the same problem among others."""
     from emopt_amd.batch import Batch
