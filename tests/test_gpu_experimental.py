@@ -13,6 +13,8 @@ experimental 3D examples.
   4 x 3 x 2 um grid with a 1.2 x 1.0 um design region, planar, bounded
   eps): every design column is a parameter; check_gradient on the four
   columns with the largest gradient, and on a run with a volume penalty.
+* AutoDiff with update_mu on a point detector: eps and mu both torch
+  functions of the parameters, against central differences.
 
 The gradient formulas are the reference's (-2 Re y^T dA x): they differ
 from the exact derivative of the time-stepped system by the leapfrog's
@@ -202,3 +204,66 @@ def test_topology_splitter_gradient(vol_penalty):
                                              idx, g_fd, g_am[idx], err))
     np.testing.assert_allclose(g_am, g, rtol=1e-12, atol=1e-300)
     assert err < 1e-2
+
+
+def test_autodiff_eps_and_mu_gradient_point_detector():
+    """AutoDiff over the plain AdjointMethod with update_mu: eps and mu are
+    torch functions of two parameters (Gaussian blobs in the interior), the
+    figure of merit |Ez|^2 at a detector.  The gradient through the
+    pseudo-loss's eps and mu terms (emopt/experimental/fdtd.py:160-223)
+    against central differences at a fixed step count."""
+    from emopt_amd.experimental.fdtd import FDTD
+    from emopt_amd.experimental.adjoint_method import AutoDiff
+    from emopt_amd.experimental.grid import AutoDiffMaterial3D
+    from emopt_amd.misc import DomainCoordinates as DC
+    from tests.cases import span
+
+    N, d, w = (36, 30, 28), 0.05, 6
+    c_e = torch.tensor([0.9, 0.75, 0.7], dtype=torch.float64)
+    c_m = torch.tensor([1.05, 0.8, 0.75], dtype=torch.float64)
+
+    def blob(coords, c, r=0.15):
+        z, y, x = coords
+        return (torch.exp(-((z - c[2]) / r) ** 2).reshape(-1, 1, 1)
+                * torch.exp(-((y - c[1]) / r) ** 2).reshape(1, -1, 1)
+                * torch.exp(-((x - c[0]) / r) ** 2).reshape(1, 1, -1))
+
+    eps_f = lambda v, coords, bg=None: 2.0 + v[0] * blob(coords, c_e)
+    mu_f = lambda v, coords, bg=None: 1.0 + v[1] * blob(coords, c_m)
+    p = np.array([1.5, 0.6])
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim = FDTD(span(N[0], d), span(N[1], d), span(N[2], d), d, d, d, 1.2, rtol=1e-6)
+        sim.w_pml = [w * d + 1e-9 * d] * 6
+    sim.verbose = 0
+    sim.set_materials(AutoDiffMaterial3D(d, d, d, eps_f, p), AutoDiffMaterial3D(d, d, d, mu_f, p))
+    src = [np.zeros((1, 1, 1), complex) for _ in range(6)]
+    src[2][0, 0, 0] = 1.0
+    sim.set_sources(src, DC.from_indices(12, 13, 14, 15, 13, 14, d, d, d))
+    det = DC.from_indices(24, 25, 16, 17, 15, 16, d, d, d)
+
+    class Detector(AutoDiff):
+        def calc_fom(self, sim, params):
+            return float(abs(sim.get_field("Ez", det)[0, 0, 0]) ** 2)
+
+        def calc_dFdx(self, sim, params):
+            a = [np.zeros((1, 1, 1), complex) for _ in range(6)]
+            a[2][0, 0, 0] = np.conj(sim.get_field("Ez", det)[0, 0, 0])
+            return [[a], [det]]
+
+    am = Detector(sim, update_mu=True)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        am.fom(p)
+        n0 = sim.last_solve["n0"]
+        sim.fixed_n0 = 4 * n0 - (4 * n0) % sim.Tn
+        g = am.gradient(p)
+        h = 1e-3
+        fd = []
+        for q in range(2):
+            e = np.eye(2)[q] * h
+            fd.append((am.fom(p + e) - am.fom(p - e)) / (2 * h))
+    fd = np.array(fd)
+    print("autodiff %s, central differences %s" % (g, fd))
+    assert np.all(np.abs(fd) > 0)
+    np.testing.assert_allclose(g, fd, rtol=1e-2)
