@@ -14,7 +14,8 @@ experimental 3D examples.
   eps): every design column is a parameter; check_gradient on the four
   columns with the largest gradient, and on a run with a volume penalty.
 * AutoDiff with update_mu on a point detector: eps and mu both torch
-  functions of the parameters, against central differences.
+  functions of the parameters, against central differences; and Topology
+  with update_mu and bounds on both, on the same detector.
 
 The gradient formulas are the reference's (-2 Re y^T dA x): they differ
 from the exact derivative of the time-stepped system by the leapfrog's
@@ -267,3 +268,62 @@ def test_autodiff_eps_and_mu_gradient_point_detector():
     print("autodiff %s, central differences %s" % (g, fd))
     assert np.all(np.abs(fd) > 0)
     np.testing.assert_allclose(g, fd, rtol=1e-2)
+
+
+def test_topology_eps_and_mu_gradient_point_detector():
+    """Topology over the plain AdjointMethod with update_mu and bounds on
+    both: eps and mu of every cell of a box are parameters (eps ones first);
+    the gradient (FDTD.calc_ydAx_topology with its mu terms) against central
+    differences on the two eps and two mu parameters with the largest
+    gradients."""
+    from emopt_amd.experimental.fdtd import FDTD
+    from emopt_amd.experimental.adjoint_method import Topology
+    from emopt_amd.experimental.grid import TopologyMaterial3D
+    from emopt_amd.grid import ConstantMaterial3D
+    from emopt_amd.misc import DomainCoordinates as DC
+    from tests.cases import span
+
+    N, d, w = (36, 30, 28), 0.05, 6
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim = FDTD(span(N[0], d), span(N[1], d), span(N[2], d), d, d, d, 1.2, rtol=1e-6)
+        sim.w_pml = [w * d + 1e-9 * d] * 6
+    sim.verbose = 0
+    box = DC.from_indices(15, 21, 13, 18, 12, 16, d, d, d)
+    sim.set_materials(TopologyMaterial3D(ConstantMaterial3D(2.0), box),
+                      TopologyMaterial3D(ConstantMaterial3D(1.0), box))
+    src = [np.zeros((1, 1, 1), complex) for _ in range(6)]
+    src[2][0, 0, 0] = 1.0
+    sim.set_sources(src, DC.from_indices(12, 13, 14, 15, 13, 14, d, d, d))
+    det = DC.from_indices(24, 25, 16, 17, 15, 16, d, d, d)
+
+    class Detector(Topology):
+        def calc_fom(self, sim, params):
+            return float(abs(sim.get_field("Ez", det)[0, 0, 0]) ** 2)
+
+        def calc_dFdx(self, sim, params):
+            a = [np.zeros((1, 1, 1), complex) for _ in range(6)]
+            a[2][0, 0, 0] = np.conj(sim.get_field("Ez", det)[0, 0, 0])
+            return [[a], [det]]
+
+    am = Detector(sim, update_mu=True, eps_bounds=(1.5, 3.0), mu_bounds=(0.8, 1.6))
+    p = am.get_params(squish=0.05)
+    ncell = box.Nx * box.Ny * box.Nz
+    assert p.size == 2 * ncell
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        am.fom(p)
+        n0 = sim.last_solve["n0"]
+        sim.fixed_n0 = 4 * n0 - (4 * n0) % sim.Tn
+        g = am.gradient(p)
+        idx = list(np.argsort(-np.abs(g[:ncell]))[:2]) + \
+            list(ncell + np.argsort(-np.abs(g[ncell:]))[:2])
+        h = 1e-3
+        fd = []
+        for q in idx:
+            e = np.zeros_like(p)
+            e[q] = h
+            fd.append((am.fom(p + e) - am.fom(p - e)) / (2 * h))
+    fd = np.array(fd)
+    print("parameters %s: gradient %s, central differences %s" % (idx, g[idx], fd))
+    np.testing.assert_allclose(g[idx], fd, rtol=1e-2)
