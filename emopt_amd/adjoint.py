@@ -793,5 +793,9 @@ class FDTDAdjointFunction:
 
 def fdtd_fom(params, am):
     """Differentiable FOM: ``F = fdtd_fom(p, am); F.backward()`` gives
-    p.grad from one forward and one adjoint solve."""
+    p.grad from one forward and one adjoint solve.  The gradient is
+    gradient_device's: with am.exact_adjoint (the default) the gradient of
+    the time-stepped system, which differs from EMopt's formula where the
+    design crosses a CPML slab; am.exact_adjoint = False gives the
+    formula."""
     return FDTDAdjointFunction.get().apply(params, am)
