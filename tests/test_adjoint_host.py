@@ -192,3 +192,22 @@ def test_rect_edge_deps():
         # moving the trench edge turns Si (12) into air (1) over the etch
         # depth: |d eps| summed over the column x cell width = 11 * (area)
         assert tot > 0
+
+
+def test_fomutils_smooth_step_and_rect():
+    """step / rect of emopt/fomutils.py:38-140 by their defining properties
+    (limits, midpoint, the 1 % width) and their derivatives by central
+    differences."""
+    from emopt_amd import fomutils as fu
+    x = np.linspace(-3, 3, 121)
+    assert abs(fu.step(0.0, 4.0, y0=0.5, A=2.0) - 1.5) < 1e-15
+    assert abs(fu.step(50.0, 4.0, y0=0.5, A=2.0) - 2.5) < 1e-12
+    assert abs(fu.step(-50.0, 4.0, y0=0.5, A=2.0) - 0.5) < 1e-12
+    w1p, ws = 2.0, 0.1
+    np.testing.assert_allclose(fu.rect(np.array([-w1p / 2, w1p / 2]), w1p, ws), 0.01, rtol=1e-9)
+    assert abs(fu.rect(0.0, w1p, ws) - 1.0) < 1e-9
+    h = 1e-6
+    for f, df, args in ((fu.step, fu.step_derivative, (4.0, 0.5, 2.0)),
+                        (fu.rect, fu.rect_derivative, (1.3, 0.4))):
+        fd = (f(x + h, *args) - f(x - h, *args)) / (2 * h)
+        np.testing.assert_allclose(df(x, *args), fd, rtol=1e-6, atol=1e-8)
