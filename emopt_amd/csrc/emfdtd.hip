@@ -791,12 +791,29 @@ void launch_e_wide(const emf_solver *s, const FusedArgs &A, const Geo &g, const 
     else launch_e_wide_zu<CM, 0, NT>(s, A, g, ps, done);
 }
 
+// kind 4: the H pass with its y-neighbour row staged in LDS (uniform mu)
+template <bool NT>
+void launch_h_lds(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
+                  const int *done)
+{
+    const int ty = pass_ty_h(s);
+    const dim3 gr((s->Nx + PX - 1) / PX, (s->Ny + ty - 1) / ty, (s->Nz + g.zc - 1) / g.zc);
+    switch (ty) {
+    case 2: PASS_LAUNCH((k_pass_h_lds<2, NT>), gr, dim3(PX, 2), A, g, ps, done); break;
+    case 8: PASS_LAUNCH((k_pass_h_lds<8, NT>), gr, dim3(PX, 8), A, g, ps, done); break;
+    default: PASS_LAUNCH((k_pass_h_lds<4, NT>), gr, dim3(PX, 4), A, g, ps, done);
+    }
+}
+
 // kind 3 = the two passes compiled for 5 waves per SIMD
 template <bool MU>
 void launch_h_t(const emf_solver *s, const FusedArgs &A, const Geo &g, const PassSrc &ps,
                 const int *done)
 {
-    if (s->wide) {
+    if (s->kind == 4 && !MU && !s->wide && pass_ty_h(s) > 1) {
+        if (stream_psi(s)) launch_h_lds<true>(s, A, g, ps, done);
+        else launch_h_lds<false>(s, A, g, ps, done);
+    } else if (s->wide) {
         if (stream_psi(s)) launch_h_wide<MU, true>(s, A, g, ps, done);
         else launch_h_wide<MU, false>(s, A, g, ps, done);
     } else if (s->kind == 3) launch_h_occ<MU, 5, false>(s, A, g, ps, done);
@@ -2415,9 +2432,10 @@ int emf_set_streaming(emf_solver *s, int mode)
 int emf_set_tuning(emf_solver *s, int kind, int tyg, int zc, int order)
 {
     LOCK(s);
-    if (kind < 0 || kind > 3)
-        EMF_FAIL(EMF_ERR_ARG, "kind must be 0 (fused, row recompute), 1 (two passes), 2 (fused, LDS) "
-                              "or 3 (two passes, 5 waves per SIMD)");
+    if (kind < 0 || kind > 4)
+        EMF_FAIL(EMF_ERR_ARG, "kind must be 0 (fused, row recompute), 1 (two passes), 2 (fused, LDS), "
+                              "3 (two passes, 5 waves per SIMD) or 4 (two passes, the H pass's "
+                              "y-neighbour row through LDS)");
     const bool ok = kind == 2 ? (tyg == 4 || tyg == 8 || tyg == 16)
                               : (tyg == 1 || tyg == 2 || tyg == 4 || tyg == 8);
     if (!ok) EMF_FAIL(EMF_ERR_ARG, "tyg must be 1, 2, 4 or 8 (kinds 0, 1) or 4, 8, 16 (kind 2)");

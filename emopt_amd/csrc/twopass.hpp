@@ -126,6 +126,75 @@ __global__ __launch_bounds__(PX * TY, OCC) void k_pass_h(FusedArgs A, Geo g, Pas
     h_column<MU_ARRAY, NT, OT>(A, g, sh, k, j, z0, z1, nullptr, 0, smask);
 }
 
+// The H pass (uniform mu, 32-bit offsets) with the y-neighbour row staged in
+// LDS: each wave (one row) puts its own row's E_x, E_z of the plane into a
+// double-buffered LDS slot, the work-group passes one barrier per plane, and
+// rows 0 .. TY-2 read E_x, E_z at j + 1 from there; the last row loads them
+// from memory as h_column does.  The values are those h_column loads (the H
+// pass does not write E), so the results are identical.  Every wave of the
+// work-group runs every plane of the column range, which depends on
+// blockIdx only: rows past the grid and lanes past Nx skip their loads,
+// arithmetic and stores but not the barriers.  A fast wave cannot overwrite a
+// buffer before the slow ones have read it: it writes a plane's buffer only
+// after passing the previous plane's barrier.
+template <int TY, bool NT>
+__global__ __launch_bounds__(PX * TY, 4) void k_pass_h_lds(FusedArgs A, Geo g, PassSrc sh,
+                                                          const int *__restrict__ done)
+{
+    __shared__ double lx[2][TY][PX], lz[2][TY][PX];
+    if (done && *done) return;                 // the same for every thread of the grid
+    int k, j, z0, z1;
+    pass_tile(g, TY, k, j, z0, z1);
+    j = __builtin_amdgcn_readfirstlane(j);   // one row per wave
+    const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+    const bool act = k < g.Nx && j < g.Ny;
+    const unsigned long long smask = act ? col_src_mask(sh, k, j, z0, z1) : 0ULL;
+    const Layout L = make_layout(g);
+    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    const unsigned cs = cstride<unsigned>(A), cs2 = 2 * cstride<unsigned>(A);
+    const unsigned oz = (unsigned)g.nxy * 8u, oy = (unsigned)Nx * 8u;
+    const bool jn = j + 1 < Ny, kn = k + 1 < Nx;
+    const bool col_src = smask != 0;
+    unsigned o = act ? ((unsigned)z0 * (unsigned)Ny + (unsigned)j) * (unsigned)Nx * 8u +
+                           (unsigned)k * 8u
+                     : 0u;
+    double ex = 0, ey = 0;
+    if (act) { ex = ldo(A.Eo, o); ey = ldo(A.Eo, o + cs); }
+    for (int i = z0; i < z1; ++i, o += oz) {
+        const int b = (i - z0) & 1;
+        HIn d;
+        d.ex = ex; d.ey = ey; d.ez = 0;
+        d.exi = d.eyi = d.ezj = d.exj = d.ezk = d.eyk = 0;
+        double ezk = 0, eyk = 0;
+        if (act) {
+            if (i + 1 < Nz) { d.exi = ldo(A.Eo, o + oz); d.eyi = ldo(A.Eo, o + oz + cs); }
+            d.ez = ldo(A.Eo, o + cs2);
+            if (jn && ty == TY - 1) { d.ezj = ldo(A.Eo, o + oy + cs2); d.exj = ldo(A.Eo, o + oy); }
+            d.hx = ldo(A.Ho, o); d.hy = ldo(A.Ho, o + cs); d.hz = ldo(A.Ho, o + cs2);
+            if (tx == PX - 1 && kn) { ezk = ldo(A.Eo, o + 8u + cs2); eyk = ldo(A.Eo, o + 8u + cs); }
+            h_load_aux<false, NT, unsigned>(g, L, A, i, j, k, o, d);
+        }
+        lx[b][ty][tx] = d.ex;
+        lz[b][ty][tx] = d.ez;
+        __syncthreads();
+        if (act) {
+            if (jn && ty < TY - 1) { d.exj = lx[b][ty + 1][tx]; d.ezj = lz[b][ty + 1][tx]; }
+            d.ezk = __shfl_down(d.ez, 1);
+            d.eyk = __shfl_down(d.ey, 1);
+            if (tx == PX - 1) { d.ezk = ezk; d.eyk = eyk; }
+            if (!kn) d.ezk = d.eyk = 0;   // zero ghost past the max face
+            double hx, hy, hz, np[6];
+            h_compute(g, d, hx, hy, hz, np);
+            if (col_src) h_sources<false, unsigned>(g, sh, A, i, j, k, o, hx, hy, hz, smask);
+            sto(A.Hn, o, hx); sto(A.Hn, o + cs, hy); sto(A.Hn, o + cs2, hz);
+            if (d.sx.in) { stp<NT>(A.phn, psof(L, 0, d.sx.lo, 0) + d.sx.io, np[0]); stp<NT>(A.phn, psof(L, 0, d.sx.lo, 1) + d.sx.io, np[1]); }
+            if (d.sy.in) { stp<NT>(A.phn, psof(L, 1, d.sy.lo, 0) + d.sy.io, np[2]); stp<NT>(A.phn, psof(L, 1, d.sy.lo, 1) + d.sy.io, np[3]); }
+            if (d.sz.in) { stp<NT>(A.phn, psof(L, 2, d.sz.lo, 0) + d.sz.io, np[4]); stp<NT>(A.phn, psof(L, 2, d.sz.lo, 1) + d.sz.io, np[5]); }
+        }
+        ex = d.exi; ey = d.eyi;
+    }
+}
+
 // ------------------------------------------------------------------ E pass
 // One cell (i, j, k) at byte offset o; hxm/hym hold H_x, H_y at i-1 and are
 // advanced to plane i.  With CM_INDEX, ci is the cell's coefficient index,

@@ -356,9 +356,11 @@ EMF_API int emf_time_steps(emf_solver *s, int64_t n_start, int64_t nsteps, int64
                            int per_kernel, double *out);
 
 /* Step kernel selection: kind 1 = separate H and E passes (default), 3 = the
- * same passes compiled for 5 waves per SIMD (a few registers spilled), 0 =
- * one fused H+E launch per step recomputing the H row it needs, 2 = one fused
- * launch sharing H through LDS; tyg = rows (wavefronts) per work-group (1, 2,
+ * same passes compiled for 5 waves per SIMD (a few registers spilled), 4 =
+ * the two passes with the H pass taking its y-neighbour row of E through LDS
+ * (uniform mu; the plain H pass otherwise), 0 = one fused H+E launch per step
+ * recomputing the H row it needs, 2 = one fused launch sharing H through
+ * LDS; tyg = rows (wavefronts) per work-group (1, 2,
  * 4 or 8; 4, 8 or 16 for kind 2); zc = z planes per work-group; order =
  * work-group order of the two passes (0 x-fastest, 1 z-fastest, 2 z-fastest
  * grouped by XCD, 3 x-fastest with the E pass walking backwards).  Defaults kind 1, tyg 4, zc 2, order 3.  Results are

@@ -323,8 +323,9 @@ def _run_steps(case, nsteps, kind, tyg=4, zc=8, order=0, compress=1, stream=-1, 
 def test_step_kernels_agree(variant):
     """Every step kernel and launch shape against the default two-pass one:
     the fused H+E kernels (ping-pong buffers; kind 0 recomputes the halo row,
-    kind 2 shares it through LDS) for every work-group shape and z-chunk, and
-    the two passes for other row counts, z-chunks and work-group orders."""
+    kind 2 shares it through LDS) for every work-group shape and z-chunk, the
+    two passes for other row counts, z-chunks and work-group orders, and the
+    H pass with its y-neighbour row through LDS (kind 4)."""
     kw = dict(w=(5, 4, 3, 6, 4, 5), seed=31, nsrc=2, mu_rand=False)
     if variant == "ceps":
         kw["ceps"] = True
@@ -351,6 +352,15 @@ def test_step_kernels_agree(variant):
                     got = _run_steps(case, 33, kind=kind, tyg=tyg, zc=zc, order=order)
                     errs = rel_l2_set(got, ref)
                     assert max(errs) < 1e-13, (variant, kind, tyg, zc, order, errs)
+    # kind 4 (the H pass's y-neighbour row through LDS) reads the values kind 1
+    # loads: identical fields for the same shape, partial rows and lanes
+    # included (37 rows, 70 columns)
+    for tyg in (2, 4, 8):
+        for zc in (1, 5):
+            for order in (0, 3):
+                a = _run_steps(case, 33, kind=1, tyg=tyg, zc=zc, order=order)
+                b = _run_steps(case, 33, kind=4, tyg=tyg, zc=zc, order=order)
+                assert all(np.array_equal(x, y) for x, y in zip(a, b)), (variant, tyg, zc, order)
 
 
 @pytest.mark.parametrize("tuning", [(0, 2, 8, 0), (2, 8, 4, 0), (1, 8, 8, 2)])
