@@ -268,22 +268,25 @@ struct ConvState {
     int nchecks;
 };
 
-__device__ inline double phase3(double t0, double t1, double t2, double f0, double f1, double f2)
+// The 3T phase and amplitude of src/fdtd.cpp:1465-1493, given the sample
+// times' differences ds21 = sin t2 - sin t1, ds10 = sin t1 - sin t0,
+// dc10 = cos t1 - cos t0, dc21 = cos t2 - cos t1 (the same for every point
+// of a check, so computed once per check).
+__device__ inline double phase3(double ds21, double ds10, double dc10, double dc21, double f0,
+                                double f1, double f2)
 {
     double f10 = f1 - f0, f21 = f2 - f1;
     if (f10 == 0 && f21 == 0) return 0.0;
-    return atan2(f10 * (sin(t2) - sin(t1)) - f21 * (sin(t1) - sin(t0)),
-                 f21 * (cos(t1) - cos(t0)) - f10 * (cos(t2) - cos(t1)));
+    return atan2(f10 * ds21 - f21 * ds10, f21 * dc10 - f10 * dc21);
 }
 
-__device__ inline double amp3(double t0, double t1, double t2, double f0, double f1, double f2,
-                              double ph)
+__device__ inline double amp3(double ds21, double ds10, double dc10, double dc21, double f0,
+                              double f1, double f2, double ph)
 {
     double f21 = f2 - f1, f10 = f1 - f0;
     if (f21 == 0 && f10 == 0) return 0.0;
-    if (f21 * f21 >= f10 * f10)
-        return f21 / (cos(ph) * (sin(t2) - sin(t1)) + sin(ph) * (cos(t2) - cos(t1)));
-    return f10 / (cos(ph) * (sin(t1) - sin(t0)) + sin(ph) * (cos(t1) - cos(t0)));
+    if (f21 * f21 >= f10 * f10) return f21 / (cos(ph) * ds21 + sin(ph) * dc21);
+    return f10 / (cos(ph) * ds10 + sin(ph) * dc10);
 }
 
 // One check of emopt/fdtd.py:1072-1144 by one 64-lane wave (every lane of it
@@ -306,6 +309,9 @@ __device__ inline int conv_check(const double *__restrict__ Ex, const double *__
     const int lane = (int)(threadIdx.x & 63);
     const double t0 = S->t1, t1 = S->t2, t2 = tnow;
     const int h = S->nchecks;
+    const double st1 = sin(t1), ct1 = cos(t1);
+    const double ds21 = sin(t2) - st1, ds10 = st1 - sin(t0);
+    const double dc10 = ct1 - cos(t0), dc21 = cos(t2) - ct1;
     double sa = 0, s0 = 0, sp = 0, sp0 = 0;
     // lane l handles sample points l, l + 64, l + 128, ...
     for (int q = lane; q < nc; q += 64) {
@@ -318,8 +324,8 @@ __device__ inline int conv_check(const double *__restrict__ Ex, const double *__
             S->eh[0][c][q] = f0;
             S->eh[1][c][q] = f1;
             S->eh[2][c][q] = f2;
-            ph[c] = phase3(t0, t1, t2, f0, f1, f2);
-            am[c] = amp3(t0, t1, t2, f0, f1, f2, ph[c]);
+            ph[c] = phase3(ds21, ds10, dc10, dc21, f0, f1, f2);
+            am[c] = amp3(ds21, ds10, dc10, dc21, f0, f1, f2, ph[c]);
         }
         for (int c = 0; c < 3; ++c)
             if (am[c] < 0) { am[c] *= -1; ph[c] += M_PI; }
