@@ -14,8 +14,9 @@ experimental 3D examples.
   eps): every design column is a parameter; check_gradient on the four
   columns with the largest gradient, and on a run with a volume penalty.
 * AutoDiff with update_mu on a point detector: eps and mu both torch
-  functions of the parameters, against central differences; and Topology
-  with update_mu and bounds on both, on the same detector.
+  functions of the parameters, against central differences; Topology
+  with update_mu and bounds on both; and AutoDiff over a HybridMaterial3D
+  (structured background, function box as the design domain).
 
 The gradient formulas are the reference's (-2 Re y^T dA x): they differ
 from the exact derivative of the time-stepped system by the leapfrog's
@@ -327,3 +328,67 @@ def test_topology_eps_and_mu_gradient_point_detector():
     fd = np.array(fd)
     print("parameters %s: gradient %s, central differences %s" % (idx, g[idx], fd))
     np.testing.assert_allclose(g[idx], fd, rtol=1e-2)
+
+
+def test_autodiff_hybrid_material_gradient_point_detector():
+    """AutoDiff over a HybridMaterial3D: a structured background with a box
+    whose eps is a torch function of the parameters (a block whose x edges
+    and eps are the parameters, seeing the background as bg).  The method's
+    domain defaults to the box; the gradient against central differences."""
+    import emopt_amd.experimental.autodiff_geometry as adg
+    from emopt_amd import grid
+    from emopt_amd.experimental.fdtd import FDTD
+    from emopt_amd.experimental.adjoint_method import AutoDiff
+    from emopt_amd.experimental.grid import AutoDiffMaterial3D, HybridMaterial3D
+    from emopt_amd.misc import DomainCoordinates as DC
+    from tests.cases import span
+
+    N, d, w = (40, 30, 28), 0.05, 6
+    X, Y, Z = span(N[0], d), span(N[1], d), span(N[2], d)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        sim = FDTD(X, Y, Z, d, d, d, 1.2, rtol=1e-6)
+        sim.w_pml = [w * d + 1e-9 * d] * 6
+    sim.verbose = 0
+    bg = grid.StructuredMaterial3D(X, Y, Z, d, d, d)
+    bg.add_primitive(grid.Rectangle(X / 2, Y / 2, 2 * X, 2 * Y, material_value=1.8, layer=2),
+                     -Z, 2 * Z)
+    bg.add_primitive(grid.Rectangle(X / 2, Y / 2, 2 * X, 0.4, material_value=6.0, layer=1),
+                     Z / 2 - 0.2, Z / 2 + 0.2)
+    box = DC(0.7, 1.3, 0.45, 1.0, 0.45, 0.95, d, d, d)
+
+    def func(v, coords, bg=None):
+        z, y, x = coords
+        block = adg.rect1d(1 / d, x, v[:2], nl=adg.nl_lin).reshape(1, 1, -1)
+        base = 1.8 if bg is None else torch.as_tensor(np.real(np.asarray(bg)))
+        return base + (v[2] - 1.8) * block * torch.ones(len(z), len(y), 1, dtype=torch.float64)
+
+    p = np.array([0.83, 1.12, 4.0])
+    eps = HybridMaterial3D(bg, AutoDiffMaterial3D(d, d, d, func, p), box)
+    sim.set_materials(eps, grid.ConstantMaterial3D(1.0))
+    src = [np.zeros((1, 1, 1), complex) for _ in range(6)]
+    src[2][0, 0, 0] = 1.0
+    sim.set_sources(src, DC.from_indices(11, 12, 14, 15, 14, 15, d, d, d))
+    det = DC.from_indices(30, 31, 15, 16, 14, 15, d, d, d)
+
+    class Detector(AutoDiff):
+        def calc_fom(self, sim, params):
+            return float(abs(sim.get_field("Ez", det)[0, 0, 0]) ** 2)
+
+        def calc_dFdx(self, sim, params):
+            a = [np.zeros((1, 1, 1), complex) for _ in range(6)]
+            a[2][0, 0, 0] = np.conj(sim.get_field("Ez", det)[0, 0, 0])
+            return [[a], [det]]
+
+    am = Detector(sim)
+    assert am._domain is box
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        am.fom(p)
+        n0 = sim.last_solve["n0"]
+        sim.fixed_n0 = 4 * n0 - (4 * n0) % sim.Tn
+        g = am.gradient(p)
+        h = 1e-4
+        fd = np.array([(am.fom(p + h * e) - am.fom(p - h * e)) / (2 * h) for e in np.eye(3)])
+    print("hybrid autodiff %s, central differences %s" % (g, fd))
+    np.testing.assert_allclose(g, fd, rtol=1e-2, atol=1e-2 * np.abs(fd).max())
