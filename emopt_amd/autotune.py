@@ -1,7 +1,8 @@
 """Launch-shape autotuning for the step kernels.
 
 Single solves: the step kernel and its work-group shape -- kind (two passes,
-fused with row recompute, fused through LDS), rows per work-group, z planes per
+two passes with the H pass's neighbour row through LDS, fused with row
+recompute, fused through LDS), rows per work-group, z planes per
 work-group and work-group order -- are timed with HIP events on the solver
 itself (``emf_time_steps``), on its own grid, materials and CPML, and the
 fastest is kept.  Every candidate computes bit-for-bit the same fields (the
@@ -28,7 +29,7 @@ __all__ = ["SOLVE_CANDIDATES", "tune_solver", "BATCH_CANDIDATES", "tune_batch"]
 # (kind, rows per work-group, z planes per work-group, order)
 SOLVE_CANDIDATES = [(1, 4, 2, 3), (1, 2, 2, 3), (1, 2, 4, 3), (1, 2, 2, 0), (1, 4, 2, 0),
                     (1, 8, 2, 3), (1, 2, 1, 3), (1, 4, 2, 1), (3, 2, 2, 3), (2, 8, 4, 0),
-                    (2, 4, 4, 0), (0, 1, 8, 0)]
+                    (2, 4, 4, 0), (0, 1, 8, 0), (4, 4, 2, 3), (4, 2, 2, 3)]
 _cache = {}
 
 
