@@ -211,3 +211,21 @@ def test_fomutils_smooth_step_and_rect():
                         (fu.rect, fu.rect_derivative, (1.3, 0.4))):
         fd = (f(x + h, *args) - f(x - h, *args)) / (2 * h)
         np.testing.assert_allclose(df(x, *args), fd, rtol=1e-6, atol=1e-8)
+
+
+def test_mode_match_backward_overlap_by_symmetry():
+    """Reversing H turns the forward-propagating mode into the backward one:
+    the backward overlap of (E, -H) is the forward overlap of (E, H), and
+    the other overlaps vanish (emopt/fomutils.py:1036-1116)."""
+    from emopt_amd.adjoint import ModeMatch
+    rng = np.random.default_rng(8)
+    shp = (7, 9, 1)
+    m = [rng.normal(size=shp) + 1j * rng.normal(size=shp) for _ in range(6)]
+    mm = ModeMatch([1, 0, 0], 0.05, 0.04, *m)
+    mm.compute(*m)
+    fwd, back = mm.get_mode_match_forward(1.0), mm.get_mode_match_back(1.0)
+    mm.compute(*m[:3], *[-h for h in m[3:]])
+    fwd_r, back_r = mm.get_mode_match_forward(1.0), mm.get_mode_match_back(1.0)
+    assert abs(back_r - fwd) <= 1e-12 * abs(fwd) and fwd > 0
+    assert abs(back) <= 1e-12 * fwd and abs(fwd_r) <= 1e-12 * fwd
+    assert abs(mm.get_mode_match_back(2.0) - back_r / 2) <= 1e-15 * abs(back_r)

@@ -199,3 +199,37 @@ def test_module_surface():
     assert emopt.misc.NOT_PARALLEL and emopt.misc.run_on_master(lambda: 3)() == 3
     assert emopt.fomutils.ModeMatch is emopt.adjoint.ModeMatch
     assert emopt.adjoint_method.AdjointMethodMO is emopt.adjoint.AdjointMethodMO
+
+
+def test_remaining_shapes_by_identities():
+    """rectnd over [z, y, x] equals rect3d; step1d reversed is the
+    complement for the odd-symmetric sigmoid; step2d with normal (1, 0) is
+    step1d along x broadcast over y; union_c / intersection_c of 0/1 masks
+    with a steep ramp are the logical or / and."""
+    d = 0.05
+    x = torch.arange(0, 20, dtype=F64) * d
+    y = torch.arange(0, 14, dtype=F64) * d
+    z = torch.arange(0, 9, dtype=F64) * d
+    v = torch.tensor([0.21, 0.74, 0.12, 0.55, 0.08, 0.33], dtype=F64)
+    r3 = adg.rect3d(1 / d, x, y, z, v, nl=adg.nl_lin)
+    rn = adg.rectnd(1 / d, [z, y, x], torch.stack([v[4:6], v[2:4], v[0:2]]), nl=adg.nl_lin)
+    assert rn.shape == r3.shape == (9, 14, 20)
+    np.testing.assert_allclose(rn.numpy(), r3.numpy(), rtol=0, atol=1e-15)
+    s = adg.step1d(7.0, x, 0.4)
+    np.testing.assert_allclose(adg.step1d(7.0, x, 0.4, reverse=True).numpy(), 1 - s.numpy(),
+                               atol=1e-15)
+    s2 = adg.step2d(7.0, x, y, torch.tensor([1.0, 0.0, 0.4, 0.3], dtype=F64))
+    np.testing.assert_allclose(s2.numpy(), np.broadcast_to(s.numpy(), (14, 20)), atol=1e-15)
+    rng = np.random.default_rng(9)
+    m = [torch.tensor((rng.uniform(size=(4, 6)) > 0.5).astype(np.float64)) for _ in range(3)]
+    any_ = np.logical_or.reduce([t.numpy() > 0.5 for t in m])
+    all_ = np.logical_and.reduce([t.numpy() > 0.5 for t in m])
+    np.testing.assert_allclose(adg.union_c(200.0, m).numpy(), any_.astype(float), atol=1e-40)
+    np.testing.assert_allclose(adg.intersection_c(200.0, m).numpy(), all_.astype(float), atol=1e-40)
+
+
+def test_polygon_set_points():
+    from emopt_amd.grid import Polygon
+    p = Polygon()
+    p.set_points([0, 1, 1, 0], [0, 0, 1, 1])
+    assert p.Np == 4 and p.contains_point(0.5, 0.5) and p.bbox() == (0.0, 1.0, 0.0, 1.0)

@@ -213,3 +213,25 @@ def test_ellipse_and_polygon_areas():
     v = m.get_values(0, 30, 0, 26, 1, 2)[0].real
     shoelace = 0.5 * abs(np.dot(xs, np.roll(ys, -1)) - np.dot(ys, np.roll(xs, -1)))
     assert np.sum(v - 1.0) * d * d == pytest.approx(shoelace, rel=2e-3)
+
+
+def test_gaussian_fields_closed_forms():
+    """misc.gaussian_fields (emopt/misc.py:580-636): at the waist (theta = 0)
+    Ez = exp(-(x^2 + z^2) / w0^2), Hx = n0 Ez, Hy = 0; along the axis one
+    Rayleigh range from the waist (theta = pi/2, x = y_R) the amplitude is
+    1/sqrt(2) and the phase k0 y_R - pi/4 (the Gouy shift)."""
+    from emopt_amd.misc import gaussian_fields
+    w0, wl, n0 = 1.3, 1.55, 1.444
+    x = np.linspace(-3, 3, 13)
+    z = np.linspace(-2, 2, 13)
+    Ez, Hx, Hy = gaussian_fields(x, z, 0.0, w0, 0.0, wl, n0)
+    np.testing.assert_allclose(Ez, np.exp(-(x ** 2 + z ** 2) / w0 ** 2), rtol=1e-14)
+    np.testing.assert_allclose(Hx, n0 * Ez, rtol=1e-14)
+    assert np.all(Hy == 0)
+    yR = np.pi * w0 ** 2 / wl
+    Ez, Hx, Hy = gaussian_fields(np.array([yR]), np.array([0.0]), 0.0, w0, np.pi / 2, wl, n0)
+    assert abs(abs(Ez[0]) - 1 / np.sqrt(2)) < 1e-14
+    k0 = 2 * np.pi * n0 / wl
+    dphi = np.angle(Ez[0] * np.exp(-1j * (k0 * yR - np.pi / 4)))
+    assert abs(dphi) < 1e-12
+    assert abs(Hy[0] + n0 * Ez[0]) < 1e-14
