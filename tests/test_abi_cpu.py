@@ -42,6 +42,9 @@ def test_host_closed_forms_match_oracle():
         assert p1 == p2
         assert L.emf_calc_amplitude_3T(*t, *f, p1) == o.FDTD_calc_amplitude_3T(*t, *f, p2)
         assert L.emf_calc_phase_2T(t[0], t[1], f[0], f[1]) == o.FDTD_calc_phase_2T(t[0], t[1], f[0], f[1])
+        p2t = L.emf_calc_phase_2T(t[0], t[1], f[0], f[1])
+        assert (L.emf_calc_amplitude_2T(t[0], t[1], f[0], f[1], p2t)
+                == o.FDTD_calc_amplitude_2T(t[0], t[1], f[0], f[1], p2t))
 
 
 def test_error_path_without_handle():
