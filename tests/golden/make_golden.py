@@ -12,8 +12,9 @@ tests/cases.py; only outputs are stored):
               step counts, random materials / complex J+M blocks, every bc of
               {000, 0H0, E00, HE0}, real and complex eps, CPML widths 0/3/4,
               more source blocks than the GPU kernel carries inline
-* solve_strip_24x20x16.npz  a full converged solve: n0, Tn, the convergence
-              history and the six recovered complex components
+* solve_strip_24x20x16.npz, solve_strip_40x32x24.npz  full converged solves
+              of the strip waveguide: n0, Tn, the convergence history and the
+              six recovered complex components
 * kats.npz    scalar known answers of calc_phase_3T / calc_amplitude_3T and
               src_func_t
 
@@ -36,9 +37,9 @@ STEP_CASES = []
 for bc in ("000", "0H0", "E00", "HE0"):
     for ceps in (0, 1):
         STEP_CASES.append(dict(Nx=12, Ny=10, Nz=8, seed=10 + len(STEP_CASES), ceps=ceps, nsrc=2,
-                               w=(3, 3, 4, 3, 3, 4), bc=bc, steps=(1, 10, 50)))
+                               w=(3, 3, 4, 3, 3, 4), bc=bc, steps=(1, 2, 10, 50)))
 STEP_CASES.append(dict(Nx=24, Ny=20, Nz=16, seed=40, ceps=0, nsrc=1, w=(4, 3, 3, 4, 4, 3),
-                       bc="000", steps=(10, 50)))
+                       bc="000", steps=(1, 2, 10, 50)))
 STEP_CASES.append(dict(Nx=12, Ny=10, Nz=8, seed=41, ceps=0, nsrc=6, w=(0, 0, 0, 0, 0, 0),
                        bc="0H0", steps=(20,)))
 
@@ -65,10 +66,10 @@ def main():
         out[key + "_fields"] = np.stack(fields)
     np.savez_compressed(os.path.join(HERE, "steps.npz"), **out)
 
-    case = strip_case(24, 20, 16, w=4)
-    r = run_oracle(case, lib="ref")
-    np.savez_compressed(os.path.join(HERE, "solve_strip_24x20x16.npz"), n0=r["n0"], Tn=r["Tn"],
-                        history=r["history"], fields=np.stack(r["fields"]))
+    for dims, w in (((24, 20, 16), 4), ((40, 32, 24), 5)):
+        r = run_oracle(strip_case(*dims, w=w), lib="ref")
+        np.savez_compressed(os.path.join(HERE, "solve_strip_%dx%dx%d.npz" % dims), n0=r["n0"],
+                            Tn=r["Tn"], history=r["history"], fields=np.stack(r["fields"]))
 
     rng = np.random.default_rng(99)
     ph_in, ph_out, am_in, am_out = [], [], [], []

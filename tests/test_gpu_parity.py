@@ -347,9 +347,10 @@ def test_gpu_matches_reference_golden_steps():
             assert max(errs) < STEP_TOL, (key, int(n_target), errs)
 
 
-def test_gpu_matches_reference_golden_solve():
-    g = _golden("solve_strip_24x20x16.npz")
-    r = run_gpu(strip_case(24, 20, 16, w=4))
+@pytest.mark.parametrize("dims,w", [((24, 20, 16), 4), ((40, 32, 24), 5)])
+def test_gpu_matches_reference_golden_solve(dims, w):
+    g = _golden("solve_strip_%dx%dx%d.npz" % dims)
+    r = run_gpu(strip_case(*dims, w=w))
     assert r["n0"] == int(g["n0"]) and r["Tn"] == int(g["Tn"])
     np.testing.assert_allclose(r["history"], g["history"], rtol=1e-8)
     errs = [rel_l2(a, b) for a, b in zip(r["fields"], g["fields"])]

@@ -183,9 +183,10 @@ def test_oracle_matches_golden_steps(olib):
         assert done == nmax
 
 
-def test_oracle_matches_golden_solve():
-    g = _golden("solve_strip_24x20x16.npz")
-    case = strip_case(24, 20, 16, w=4)
+@pytest.mark.parametrize("dims,w", [((24, 20, 16), 4), ((40, 32, 24), 5)])
+def test_oracle_matches_golden_solve(dims, w):
+    g = _golden("solve_strip_%dx%dx%d.npz" % dims)
+    case = strip_case(*dims, w=w)
     a = run_oracle(case, lib="oracle")
     assert a["n0"] == int(g["n0"])
     np.testing.assert_allclose(a["history"], g["history"], rtol=1e-9)
