@@ -42,6 +42,9 @@ STEP_CASES.append(dict(Nx=24, Ny=20, Nz=16, seed=40, ceps=0, nsrc=1, w=(4, 3, 3,
                        bc="000", steps=(1, 2, 10, 50)))
 STEP_CASES.append(dict(Nx=12, Ny=10, Nz=8, seed=41, ceps=0, nsrc=6, w=(0, 0, 0, 0, 0, 0),
                        bc="0H0", steps=(20,)))
+for q, bc in enumerate(("0H0", "E00", "HE0")):
+    STEP_CASES.append(dict(Nx=24, Ny=20, Nz=16, seed=42 + q, ceps=q % 2, nsrc=2,
+                           w=(4, 3, 3, 4, 4, 3), bc=bc, steps=(1, 2, 10, 50)))
 
 
 def main():
